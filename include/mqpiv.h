@@ -1,0 +1,124 @@
+/*
+ * mqpiv.h -- C ABI of libmqpiv, the MI355X-native cross-correlation PIV hot path.
+ *
+ * Plain pointers and sizes only.  Every image/volume buffer is column-major with
+ * dim 1 fastest (exactly a Julia Array's memory), and the outputs use the
+ * reference layout:
+ *   VF : ndims x nVF_1 x nVF_2 [x nVF_3], component index fastest
+ *        (src/utils/piv_ops.jl:25-31, allocate_outputs)
+ *   SN : nVF_1 x nVF_2 [x nVF_3]
+ * IAs skipped by the threshold test keep VF = 0 and SN = 0, as in the reference.
+ *
+ * Entry points and the reference interface each one replaces (files relative
+ * to the reference repository root):
+ *
+ *   mqpiv_ctx_create   allocate_tmp_data(alg, pivparams, T) + FFTW planning
+ *                      (src/crosscorrelation_algorithms/fftcc.jl:26-51,
+ *                      zncc.jl:27-53, nsqecc.jl:30-56, mask_nsqecc.jl:30-57;
+ *                      src/utils/fft_ops.jl:11-55).  Done once per process
+ *                      instead of once per PIV call.
+ *   mqpiv_ctx_destroy  destroy_fftw_plans (fftcc.jl:78-82, fft_ops.jl:72-83)
+ *   mqpiv_vf_size      get_vectorfield_size (src/utils/piv_ops.jl:6-14)
+ *   mqpiv_piv          PIV_CPU (src/multi_quickPIV.jl:40-96): for every IA,
+ *                      prepare_inputs! + crosscorrelation! (the plugin quartet of
+ *                      piv_parameters.jl:2-7) + gaussian_displacement
+ *                      (piv_ops.jl:249-273) + compute_SN (piv_ops.jl:333-367),
+ *                      host buffers in and out.  With corr_alg = MQPIV_MASK_NSQECC
+ *                      and a mask, the masked drivers' intended semantics
+ *                      (multi_quickPIV.jl:99-154).
+ *   mqpiv_piv_device   the same on device-resident buffers, asynchronous on the
+ *                      caller's HIP stream (no reference counterpart: it is the
+ *                      resident-data form used by batching and serving).
+ *   mqpiv_last_error   the message of the last failure on this context
+ *                      (the reference raises AssertionError/MethodError instead).
+ */
+#ifndef MQPIV_H
+#define MQPIV_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define MQPIV_OK 0
+#define MQPIV_ERR_SIZE_MISMATCH 1 /* inputs differ in size (reference: AssertionError) */
+#define MQPIV_ERR_BAD_PARAMS 2    /* invalid parameter values                           */
+#define MQPIV_ERR_UNSUPPORTED 3   /* valid in the reference but not built here          */
+#define MQPIV_ERR_OOM 4           /* device allocation failed / over the HBM budget     */
+#define MQPIV_ERR_DEVICE 5        /* a HIP runtime error                                */
+#define MQPIV_ERR_NULL 6          /* a required pointer was NULL                         */
+
+/* Correlation algorithms (src/piv_parameters.jl:2-7, parseCorr :14-23). */
+#define MQPIV_FFTCC 0
+#define MQPIV_ZNCC 1
+#define MQPIV_NSQECC 2
+#define MQPIV_MASK_NSQECC 3
+
+/* Element types of image and output buffers. */
+#define MQPIV_F32 0
+#define MQPIV_F64 1
+
+/* Precision modes (mqpiv_params.precision_mode). */
+#define MQPIV_PLAIN_FP32 0 /* FP32 transforms; FP64 window sums, sum(F^2) and denominators    */
+#define MQPIV_FP64 2       /* every step in FP64 (2-D search sizes up to 96)                     */
+#define MQPIV_AUTO 3       /* FP64 where FP32 transforms miss the parity bar (masked NSQECC, and
+                              ZNCC when SN is requested) and the size allows it; FP32 otherwise */
+
+/* The fields of PIVParameters (src/piv_parameters.jl:29-46) that the device
+ * needs, already resolved: step = interSize - overlap when the reference's
+ * step is `nothing` (piv_parameters.jl:88).  Unused trailing dims are ignored. */
+typedef struct mqpiv_params {
+  int32_t corr_alg;
+  int32_t ndims;             /* 2 or 3 */
+  int64_t inter_size[3];
+  int64_t search_margin[3];
+  int64_t step[3];
+  int32_t compute_sn;
+  int32_t precision_mode;
+  double threshold;          /* reference default -1.0 (no skipping) */
+  double mask_threshold;     /* reference default -1.0 */
+} mqpiv_params;
+
+typedef struct mqpiv_ctx mqpiv_ctx;
+
+/* Create a context on HIP device `device`.  hbm_budget_bytes = 0 means no limit
+ * beyond the device's memory.  Returns MQPIV_OK and sets *out. */
+int mqpiv_ctx_create(int device, uint64_t hbm_budget_bytes, mqpiv_ctx** out);
+int mqpiv_ctx_destroy(mqpiv_ctx* ctx);
+const char* mqpiv_last_error(const mqpiv_ctx* ctx);
+
+/* nVF per dim and the IA count for an image of size dims[0..ndims-1]. */
+int mqpiv_vf_size(const mqpiv_params* p, const int64_t* dims, int64_t* vf_out3, int64_t* n_ia);
+
+/* Host buffers.  img1/img2 (and mask) have in_dtype elements; VF/SN have
+ * out_dtype elements and are fully written (SN may be NULL; it is required
+ * when compute_sn is set).  skip_mask (nullable, one byte per IA) overrides the
+ * threshold test -- used when the caller's filtFun is not the default maxval. */
+int mqpiv_piv(mqpiv_ctx* ctx, const mqpiv_params* p, const int64_t* dims, int32_t in_dtype,
+              const void* img1, const void* img2, const void* mask, const uint8_t* skip_mask,
+              int32_t out_dtype, void* VF, void* SN);
+
+/* mqpiv_piv plus a diagnostic output: peaks (nullable) receives, per IA, the
+ * ndims 1-based integer peak indices of first_fullovp_peak
+ * (src/utils/common_ops.jl:98-124), 0 when no value beats 0 or the IA is skipped. */
+int mqpiv_piv_ex(mqpiv_ctx* ctx, const mqpiv_params* p, const int64_t* dims, int32_t in_dtype,
+                 const void* img1, const void* img2, const void* mask, const uint8_t* skip_mask,
+                 int32_t out_dtype, void* VF, void* SN, int32_t* peaks);
+
+/* Device buffers, enqueued on `stream` (a hipStream_t; NULL = the context's
+ * stream) and not synchronised. */
+int mqpiv_piv_device(mqpiv_ctx* ctx, const mqpiv_params* p, const int64_t* dims, int32_t in_dtype,
+                     const void* d_img1, const void* d_img2, const void* d_mask,
+                     const uint8_t* d_skip_mask, int32_t out_dtype, void* d_VF, void* d_SN,
+                     void* stream);
+
+/* Library build identification (for logs). */
+const char* mqpiv_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MQPIV_H */

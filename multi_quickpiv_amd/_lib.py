@@ -1,0 +1,143 @@
+"""ctypes binding of libmqpiv.so (the C ABI of include/mqpiv.h).
+
+The library is loaded from the package directory.  There is no fallback: if it
+is missing, importing the PIV entry points raises, so a GPU run can never
+silently take another path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import threading
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libmqpiv.so"
+
+MQPIV_OK = 0
+MQPIV_ERR_SIZE_MISMATCH = 1
+MQPIV_ERR_BAD_PARAMS = 2
+MQPIV_ERR_UNSUPPORTED = 3
+MQPIV_ERR_OOM = 4
+MQPIV_ERR_DEVICE = 5
+MQPIV_ERR_NULL = 6
+
+MQPIV_F32 = 0
+MQPIV_F64 = 1
+
+MQPIV_PLAIN_FP32 = 0
+MQPIV_FP64 = 2
+MQPIV_AUTO = 3
+PRECISION_MODES = {"fp32": MQPIV_PLAIN_FP32, "fp64": MQPIV_FP64, "auto": MQPIV_AUTO}
+
+# Every symbol include/mqpiv.h declares.
+EXPORTS = (
+    "mqpiv_ctx_create",
+    "mqpiv_ctx_destroy",
+    "mqpiv_last_error",
+    "mqpiv_vf_size",
+    "mqpiv_piv",
+    "mqpiv_piv_ex",
+    "mqpiv_piv_device",
+    "mqpiv_version",
+)
+
+
+class MqpivParams(C.Structure):
+    _fields_ = [
+        ("corr_alg", C.c_int32),
+        ("ndims", C.c_int32),
+        ("inter_size", C.c_int64 * 3),
+        ("search_margin", C.c_int64 * 3),
+        ("step", C.c_int64 * 3),
+        ("compute_sn", C.c_int32),
+        ("precision_mode", C.c_int32),
+        ("threshold", C.c_double),
+        ("mask_threshold", C.c_double),
+    ]
+
+
+class MqpivError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"mqpiv error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def load() -> C.CDLL:
+    """Load libmqpiv.so (raises OSError when it has not been built)."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not LIB_PATH.exists():
+            raise OSError(
+                f"{LIB_PATH} is missing: build it with `python -m multi_quickpiv_amd.build`"
+            )
+        lib = C.CDLL(str(LIB_PATH))
+        vp = C.c_void_p
+        lib.mqpiv_version.restype = C.c_char_p
+        lib.mqpiv_version.argtypes = []
+        lib.mqpiv_ctx_create.restype = C.c_int
+        lib.mqpiv_ctx_create.argtypes = [C.c_int, C.c_uint64, C.POINTER(vp)]
+        lib.mqpiv_ctx_destroy.restype = C.c_int
+        lib.mqpiv_ctx_destroy.argtypes = [vp]
+        lib.mqpiv_last_error.restype = C.c_char_p
+        lib.mqpiv_last_error.argtypes = [vp]
+        lib.mqpiv_vf_size.restype = C.c_int
+        lib.mqpiv_vf_size.argtypes = [C.POINTER(MqpivParams), C.POINTER(C.c_int64),
+                                      C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        lib.mqpiv_piv.restype = C.c_int
+        lib.mqpiv_piv.argtypes = [vp, C.POINTER(MqpivParams), C.POINTER(C.c_int64), C.c_int32,
+                                  vp, vp, vp, vp, C.c_int32, vp, vp]
+        lib.mqpiv_piv_ex.restype = C.c_int
+        lib.mqpiv_piv_ex.argtypes = [vp, C.POINTER(MqpivParams), C.POINTER(C.c_int64), C.c_int32,
+                                     vp, vp, vp, vp, C.c_int32, vp, vp, vp]
+        lib.mqpiv_piv_device.restype = C.c_int
+        lib.mqpiv_piv_device.argtypes = [vp, C.POINTER(MqpivParams), C.POINTER(C.c_int64), C.c_int32,
+                                         vp, vp, vp, vp, C.c_int32, vp, vp, vp]
+        _lib = lib
+        return lib
+
+
+class Context:
+    """One libmqpiv context (a HIP stream plus device staging) on one device."""
+
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 0):
+        self.lib = load()
+        h = C.c_void_p()
+        rc = self.lib.mqpiv_ctx_create(device, hbm_budget_bytes, C.byref(h))
+        if rc != MQPIV_OK:
+            raise MqpivError(rc, f"mqpiv_ctx_create(device={device}) failed")
+        self.handle = h
+        self.device = device
+
+    def check(self, rc: int) -> None:
+        if rc != MQPIV_OK:
+            msg = self.lib.mqpiv_last_error(self.handle).decode(errors="replace")
+            raise MqpivError(rc, msg)
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.mqpiv_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self) -> None:  # pragma: no cover - interpreter shutdown order
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_ctx: dict[int, Context] = {}
+
+
+def context(device: int = 0) -> Context:
+    with _lock:
+        c = _ctx.get(device)
+    if c is None:
+        c = Context(device)
+        with _lock:
+            _ctx[device] = c
+    return c

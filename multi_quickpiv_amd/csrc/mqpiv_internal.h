@@ -1,0 +1,46 @@
+// mqpiv_internal.h -- descriptors shared by the host runtime and the kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mqpiv {
+
+enum Alg : int32_t { ALG_FFTCC = 0, ALG_ZNCC = 1, ALG_NSQECC = 2, ALG_MASK_NSQECC = 3 };
+enum DType : int32_t { DT_F32 = 0, DT_F64 = 1 };
+
+// One frame pair (a "job") as the kernels see it.  Geometry follows the
+// reference's per-IA coordinates (src/utils/piv_ops.jl:38-85), 0-based here.
+struct Job {
+  const void* img1;
+  const void* img2;
+  const void* mask;        // masked NSQECC only
+  const uint8_t* skip;     // optional host-computed skip flags (one per IA)
+  void* vf;                // N x nVF, component fastest
+  void* sn;                // nVF, or nullptr
+  int32_t* peaks;          // optional: N int32 per IA, 1-based integer peak (0 = none)
+  int64_t dims[3];
+  int32_t is[3], sm[3], st[3];
+  int32_t vf_n[3];         // vector-field size per dim
+  int64_t n_ia;
+  int64_t ia_base;         // first global IA index of this job in a batch
+  int32_t ndims;
+  int32_t alg;
+  int32_t in_dtype;
+  int32_t out_dtype;
+  int32_t compute_sn;
+  int32_t has_threshold;   // unmasked: threshold >= 0; masked: threshold > 0
+  int32_t has_mask_threshold;
+  int32_t precision_mode;  // host-side launch choice (MQPIV_PLAIN_FP32 / FP64 / AUTO)
+  double threshold;
+  double mask_threshold;
+};
+
+// Launch-wide description of a batch of jobs sharing one kernel instance.
+struct Batch {
+  const Job* jobs;         // device array
+  int32_t n_jobs;
+  int32_t pad_;
+  int64_t n_ia_total;
+};
+
+}  // namespace mqpiv

@@ -1,0 +1,624 @@
+// mqpiv_kernels2d.hpp -- fused 2-D cross-correlation PIV kernel for gfx950.
+//
+// One thread group processes one interrogation area (IA) end to end:
+//   gather IA and search window from HBM (coalesced along dim 1)
+//   -> pack Z = F + iG in LDS -> line FFTs along dim 1 then dim 2 in VGPRs
+//   -> P = conj(F^) G^ (partner bins read back through LDS)
+//   -> inverse line FFTs (dim 2, then dim 1 only for the 2SM+1 needed columns)
+//   -> FP64 box sums of G^2 for the ZNCC/NSQECC denominators
+//   -> similarity on the fully-overlapping window -> argmax -> Gaussian
+//   subpixel -> SN -> VF/SN in the reference's layout.
+// T is the transform's scalar type (float: FP32 transforms with FP64 sums and
+// denominators; double: every step in FP64).  LPL lanes share one line.
+//
+// Reference behaviour restated (files relative to the reference repo root):
+//   coordinates/offsets     src/utils/piv_ops.jl:38-85 (incl. SR_TLF_off, :72)
+//   padded copies           src/utils/piv_ops.jl:99-127, zncc_ops.jl:3-25,
+//                           masked_nsqecc_ops.jl:19-42
+//   correlation             src/crosscorrelation_algorithms/fftcc.jl:101-126,
+//                           src/utils/fft_ops.jl:95-105
+//   ZNCC / NSQECC / masked  zncc_ops.jl:29-51, nsqecc_ops.jl:1-26,
+//                           masked_nsqecc_ops.jl:123-143, mask_nsqecc.jl:141-160
+//   window sums             src/utils/common_ops.jl:142-245
+//   peak                    src/utils/common_ops.jl:98-124
+//   subpixel                src/utils/piv_ops.jl:249-326
+//   SN                      src/utils/piv_ops.jl:333-367
+//   skip                    src/utils/piv_ops.jl:134-157 (filtFun = maxval)
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mqpiv_fft.hpp"
+#include "mqpiv_internal.h"
+
+namespace mqpiv {
+
+template <int L, typename T, int LPL>
+struct Cfg2 {
+  static constexpr int H = L / LPL;                       // samples per lane
+  static constexpr int LANES = L * LPL;                   // lanes carrying lines
+  static constexpr int TPI = LANES <= 32 ? LANES : ((LANES + 63) / 64) * 64;  // lanes per IA
+  static constexpr int IPB = LANES <= 32 ? 64 / LANES : 1;                   // IAs per block
+  static constexpr int NT = TPI * IPB;                    // threads per block
+  static constexpr int LD = L + 1;                        // LDS row stride (complex)
+  static constexpr int NW = (TPI + 63) / 64;              // waves per IA
+  static_assert(NW <= 4, "reduction slots hold 4 waves");
+};
+
+__device__ __forceinline__ double load_px(const void* img, int dt, int64_t i) {
+  return dt == DT_F32 ? (double)((const float*)img)[i] : ((const double*)img)[i];
+}
+
+// Reductions over the lanes of one IA.  For groups of <= 64 lanes a butterfly
+// over the whole wave (idle lanes carry the identity); for multi-wave groups
+// the wave results are combined through LDS.
+template <int TPI>
+struct Group {
+  static constexpr int kSpan = TPI <= 32 ? TPI : 64;
+
+  template <typename V, typename Op>
+  __device__ __forceinline__ static V reduce(V v, Op op, double* red, int t) {
+#pragma unroll
+    for (int m = 1; m < kSpan; m <<= 1) v = op(v, __shfl_xor(v, m));
+    if constexpr (TPI > 64) {
+      V* rv = reinterpret_cast<V*>(red);
+      __syncthreads();
+      if ((t & 63) == 0) rv[t >> 6] = v;
+      __syncthreads();
+      V r = rv[0];
+#pragma unroll
+      for (int w = 1; w < (TPI + 63) / 64; ++w) r = op(r, rv[w]);
+      v = r;
+    }
+    return v;
+  }
+};
+
+struct OpAdd {
+  __device__ double operator()(double a, double b) const { return a + b; }
+};
+struct OpMax {
+  __device__ double operator()(double a, double b) const { return a > b ? a : b; }
+};
+// max that propagates NaN (Julia's `maximum`)
+struct OpMaxNaN {
+  __device__ double operator()(double a, double b) const {
+    return (a != a || b != b) ? (a + b) : (a > b ? a : b);
+  }
+};
+
+// (value, linear index) argmax with the reference's tie-break: the first element
+// in column-major order whose value is strictly above everything before it,
+// starting from 0 (common_ops.jl:112-121).  i = INT32_MAX means "none".
+template <typename T>
+struct Best {
+  T v;
+  int32_t i;
+};
+template <typename T>
+__device__ __forceinline__ Best<T> best_of(Best<T> a, Best<T> b) {
+  if (b.v > a.v || (b.v == a.v && b.i < a.i)) return b;
+  return a;
+}
+
+// Offsets of one IA in one dim (0-based).
+struct Geo {
+  int64_t tlf;     // IA start
+  int64_t sr;      // clipped search-region start
+  int32_t gsz;     // clipped search-region length
+  int32_t goff;    // placement of the search region in the padded buffer (quirk Q1)
+  int32_t off_hi;  // SR_BRB_off
+};
+
+__device__ __forceinline__ Geo geo_of(int64_t v, int32_t is, int32_t sm, int32_t st, int64_t dim) {
+  Geo g;
+  g.tlf = v * st;                // TLF - 1
+  int64_t brb = g.tlf + is - 1;  // BRB - 1
+  int64_t lo = g.tlf - sm;
+  int64_t hi = brb + sm;
+  g.sr = lo > 0 ? lo : 0;
+  int64_t srb = hi < dim - 1 ? hi : dim - 1;
+  g.gsz = (int32_t)(srb - g.sr + 1);
+  // SR_TLF_off = |min(0, TLF - (SM - 1))| with 1-based TLF (piv_ops.jl:72)
+  int64_t a = (g.tlf + 1) - (sm - 1);
+  g.goff = (int32_t)(a < 0 ? -a : 0);
+  // SR_BRB_off = |min(0, dim - (BRB + SM))| (piv_ops.jl:73)
+  int64_t b = dim - ((brb + 1) + sm);
+  g.off_hi = (int32_t)(b < 0 ? -b : 0);
+  return g;
+}
+
+// Which job of a batch owns global IA index ia.
+__device__ __forceinline__ int find_job(const int64_t* __restrict__ prefix, int n_jobs, int64_t ia) {
+  int lo = 0, hi = n_jobs - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= ia) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// XCD-aware remap: consecutive hardware block ids land on different XCDs; give
+// each XCD a contiguous range of IAs so neighbouring windows share its L2.
+__device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t nb) {
+  constexpr int64_t kX = 8;
+  int64_t q = nb / kX, r = nb % kX;
+  int64_t x = b % kX, k = b / kX;
+  return x * q + (x < r ? x : r) + k;
+}
+
+// LDS layout of one IA: a work region (the L x L complex transform buffer, which
+// after the transform holds the similarity window followed by the FP64 box sums
+// for the denominators), then the first correlation of the masked measure.
+__host__ __device__ inline size_t sim_bytes(int ns0, int ns1, int tsize) {
+  return (((size_t)ns0 * ns1 * tsize + 15) & ~(size_t)15);
+}
+__host__ __device__ inline size_t region_bytes_2d(int L, int tsize, int ns0, int ns1, int ss0) {
+  size_t z = (size_t)L * (L + 1) * 2 * tsize;
+  size_t b = sim_bytes(ns0, ns1, tsize) + (size_t)ns1 * (ss0 + 1) * sizeof(double);
+  return ((z > b ? z : b) + 15) & ~(size_t)15;
+}
+constexpr int kRedBytes = 64;  // reduction slots per IA
+
+// ---------------------------------------------------------------------------
+// Forward 2-D transform of the packed window in Z (L x L complex, LD stride),
+// the conjugate product, and the inverse transform down to the (ns0 x ns1)
+// fully-overlapping window of real correlation values, left (scaled by 1/L^2)
+// as T at the start of the Z region, column-major with ns0 fastest.
+template <int L, int LPL, typename T>
+__device__ __forceinline__ void correlate_2d(cpx<T>* __restrict__ Z, int t, int ns0, int ns1) {
+  constexpr int LD = L + 1;
+  constexpr int H = L / LPL;
+  const int line = t / LPL;
+  const int h = t % LPL;
+  auto xchg = [](cpx<T> v, int) { return shfl_pair(v); };
+  cpx<T> x[H];
+  const bool lane = line < L;
+
+  // --- forward along dim 1: the line at dim-2 index `line`
+  if (lane) {
+    if constexpr (LPL == 1) {
+#pragma unroll
+      for (int i = 0; i < L; ++i) x[i] = Z[line * LD + i];
+      fft_line<L, -1>(x);
+    } else {
+#pragma unroll
+      for (int k = 0; k < H; ++k) x[k] = Z[line * LD + 2 * k + h];
+      fft_pair_dit<L, -1, T>(x, h, xchg);
+    }
+#pragma unroll
+    for (int k = 0; k < H; ++k) Z[line * LD + h * H + k] = x[k];
+  }
+  __syncthreads();
+  // --- forward along dim 2: the line at dim-1 frequency k0 = `line`
+  if (lane) {
+    if constexpr (LPL == 1) {
+#pragma unroll
+      for (int i = 0; i < L; ++i) x[i] = Z[i * LD + line];
+      fft_line<L, -1>(x);
+    } else {
+#pragma unroll
+      for (int k = 0; k < H; ++k) x[k] = Z[(2 * k + h) * LD + line];
+      fft_pair_dit<L, -1, T>(x, h, xchg);
+    }
+#pragma unroll
+    for (int k = 0; k < H; ++k) Z[(h * H + k) * LD + line] = x[k];
+  }
+  __syncthreads();
+  if (lane) {
+    // --- P(k) = conj(F^(k)) G^(k) with Z^ = F^ + i G^ (fft_ops.jl:95-105)
+    const int tn = (L - line) % L;
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+      const int k1 = h * H + k;
+      const int k1n = (L - k1) % L;
+      cpx<T> z = x[k];
+      cpx<T> n = Z[k1n * LD + tn];
+      T ar = T(0.5) * (z.x + n.x), ai = T(0.5) * (z.y - n.y);
+      T br = T(0.5) * (z.y + n.y), bi = T(0.5) * (n.x - z.x);
+      x[k] = cpx<T>{ar * br + ai * bi, ar * bi - ai * br};
+    }
+    // --- inverse along dim 2 (only outputs s1 < ns1 are kept)
+    if constexpr (LPL == 1) fft_line<L, 1>(x);
+    else fft_pair_dif<L, 1, T>(x, h, xchg);
+  }
+  __syncthreads();
+  if (lane) {
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+      const int s1 = LPL == 1 ? k : 2 * k + h;
+      if (s1 < ns1) Z[s1 * LD + line] = x[k];
+    }
+  }
+  __syncthreads();
+  // --- inverse along dim 1 for the ns1 needed columns
+  const bool act = line < ns1;
+  if (act) {
+#pragma unroll
+    for (int k = 0; k < H; ++k) x[k] = Z[line * LD + h * H + k];
+    if constexpr (LPL == 1) fft_line<L, 1>(x);
+    else fft_pair_dif<L, 1, T>(x, h, xchg);
+  }
+  __syncthreads();
+  if (act) {
+    T* S = reinterpret_cast<T*>(Z);
+    const T kScale = T(1) / T(L * L);
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+      const int s0 = LPL == 1 ? k : 2 * k + h;
+      if (s0 < ns0) S[line * ns0 + s0] = x[k].x * kScale;
+    }
+  }
+  __syncthreads();
+}
+
+// Value of the padded search buffer at (k0, k1) (0-based), zero outside the
+// clipped search region; ZNCC subtracts the search-region mean (zncc_ops.jl:22-24,
+// in the input element type, quirk Q12).
+template <int ALG>
+__device__ __forceinline__ double gval(const Job& J, const Geo& g0, const Geo& g1, int k0, int k1,
+                                       float meanG32, double meanG64) {
+  if (k0 < g0.goff || k0 >= g0.goff + g0.gsz || k1 < g1.goff || k1 >= g1.goff + g1.gsz) return 0.0;
+  int64_t i = (g0.sr + (k0 - g0.goff)) + J.dims[0] * (g1.sr + (k1 - g1.goff));
+  if constexpr (ALG == ALG_ZNCC) {
+    if (J.in_dtype == DT_F32) return (double)(((const float*)J.img2)[i] - meanG32);
+    return ((const double*)J.img2)[i] - meanG64;
+  } else {
+    return load_px(J.img2, J.in_dtype, i);
+  }
+}
+
+// Value of the interrogation window at (x0, x1): F, F - mean(F) (ZNCC) or F * m
+// (masked NSQECC, masked_nsqecc_ops.jl:19-25).
+template <int ALG>
+__device__ __forceinline__ double fval(const Job& J, const Geo& g0, const Geo& g1, int x0, int x1,
+                                       float meanF32, double meanF64) {
+  int64_t i = (g0.tlf + x0) + J.dims[0] * (g1.tlf + x1);
+  if constexpr (ALG == ALG_ZNCC) {
+    if (J.in_dtype == DT_F32) return (double)(((const float*)J.img1)[i] - meanF32);
+    return ((const double*)J.img1)[i] - meanF64;
+  } else if constexpr (ALG == ALG_MASK_NSQECC) {
+    return load_px(J.img1, J.in_dtype, i) * load_px(J.mask, J.in_dtype, i);
+  } else {
+    return load_px(J.img1, J.in_dtype, i);
+  }
+}
+
+template <int L, int ALG, typename T, int LPL>
+__global__ __launch_bounds__((Cfg2<L, T, LPL>::NT)) void piv2d_kernel(Job j0, const Job* __restrict__ jobs,
+                                                                    const int64_t* __restrict__ prefix,
+                                                                    int n_jobs, int64_t n_ia_total) {
+  using C = Cfg2<L, T, LPL>;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int t = threadIdx.x % C::TPI;
+  const int sub = threadIdx.x / C::TPI;
+
+  const int64_t b = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t ia_g = b * C::IPB + sub;
+  const bool valid = ia_g < n_ia_total;
+  const int64_t ia_c = valid ? ia_g : n_ia_total - 1;
+  Job J = j0;
+  int64_t ia = ia_c;
+  if (jobs) {  // batched launch: locate the owning job
+    const int jb = find_job(prefix, n_jobs, ia_c);
+    J = jobs[jb];
+    ia = ia_c - prefix[jb];
+  }
+
+  const int sm0 = J.sm[0], sm1 = J.sm[1];
+  const int is0 = J.is[0], is1 = J.is[1];
+  const int ns0 = 2 * sm0 + 1, ns1 = 2 * sm1 + 1;
+  const int ss0 = is0 + 2 * sm0, ss1 = is1 + 2 * sm1;
+
+  const size_t zbytes = region_bytes_2d(L, sizeof(T), ns0, ns1, ss0);
+  const size_t c1bytes = (ALG == ALG_MASK_NSQECC) ? sim_bytes(ns0, ns1, sizeof(T)) : 0;
+  const size_t per_ia = zbytes + c1bytes;
+  unsigned char* base = smem + (size_t)sub * per_ia;
+  cpx<T>* Z = reinterpret_cast<cpx<T>*>(base);
+  T* S = reinterpret_cast<T*>(base);
+  T* C1 = reinterpret_cast<T*>(base + zbytes);
+  double* red = reinterpret_cast<double*>(smem + (size_t)C::IPB * per_ia + (size_t)sub * kRedBytes);
+
+  const int64_t v0 = ia % J.vf_n[0];
+  const int64_t v1 = ia / J.vf_n[0];
+  const Geo g0 = geo_of(v0, is0, sm0, J.st[0], J.dims[0]);
+  const Geo g1 = geo_of(v1, is1, sm1, J.st[1], J.dims[1]);
+  const int dt = J.in_dtype;
+
+  // ---- skip test (piv_ops.jl:134-157)
+  int skip = 0;
+  if (J.skip) {
+    skip = J.skip[ia] != 0;
+  } else if (J.has_threshold || J.has_mask_threshold) {
+    double m1 = -INFINITY, m2 = -INFINITY;
+    for (int e = t; e < is0 * is1; e += C::TPI) {
+      int64_t i = (g0.tlf + e % is0) + J.dims[0] * (g1.tlf + e / is0);
+      if (J.has_threshold) m1 = fmax(m1, load_px(J.img1, dt, i));
+      if (J.has_mask_threshold) m2 = fmax(m2, load_px(J.mask, dt, i));
+    }
+    m1 = Group<C::TPI>::reduce(m1, OpMax{}, red, t);
+    m2 = Group<C::TPI>::reduce(m2, OpMax{}, red, t);
+    if (J.has_threshold && m1 < J.threshold) skip = 1;
+    if (J.has_mask_threshold && m2 < J.mask_threshold) skip = 1;
+  }
+
+  // ---- IA constants: means (ZNCC) and sum(F^2) in FP64
+  float meanF32 = 0.f, meanG32 = 0.f;
+  double meanF64 = 0.0, meanG64 = 0.0;
+  if constexpr (ALG == ALG_ZNCC) {
+    double sf = 0.0, sg = 0.0;
+    for (int e = t; e < is0 * is1; e += C::TPI)
+      sf += load_px(J.img1, dt, (g0.tlf + e % is0) + J.dims[0] * (g1.tlf + e / is0));
+    for (int e = t; e < g0.gsz * g1.gsz; e += C::TPI)
+      sg += load_px(J.img2, dt, (g0.sr + e % g0.gsz) + J.dims[0] * (g1.sr + e / g0.gsz));
+    sf = Group<C::TPI>::reduce(sf, OpAdd{}, red, t);
+    sg = Group<C::TPI>::reduce(sg, OpAdd{}, red, t);
+    const int64_t nF = (int64_t)is0 * is1;
+    const int64_t nG = (int64_t)g0.gsz * g1.gsz;
+    meanF32 = (float)sf / (float)nF;
+    meanG32 = (float)sg / (float)nG;
+    meanF64 = sf / (double)nF;
+    meanG64 = sg / (double)nG;
+  }
+  double sumF2 = 0.0;
+  if constexpr (ALG != ALG_FFTCC) {
+    for (int e = t; e < is0 * is1; e += C::TPI) {
+      const int x0 = e % is0, x1 = e / is0;
+      if constexpr (ALG == ALG_MASK_NSQECC) {
+        int64_t i = (g0.tlf + x0) + J.dims[0] * (g1.tlf + x1);
+        double m = load_px(J.mask, dt, i);
+        double fm = load_px(J.img1, dt, i) * m;
+        double wi = m > 0 ? m : 1.0;
+        sumF2 += fm * fm / wi;
+      } else {
+        double f = fval<ALG>(J, g0, g1, x0, x1, meanF32, meanF64);
+        sumF2 += f * f;
+      }
+    }
+    sumF2 = Group<C::TPI>::reduce(sumF2, OpAdd{}, red, t);
+  }
+
+  // ---- correlation(s)
+  constexpr int NPASS = (ALG == ALG_MASK_NSQECC) ? 2 : 1;
+#pragma unroll 1
+  for (int pass = 0; pass < NPASS; ++pass) {
+    // pack Z = A + iB (zero padded to L x L); consecutive lanes take
+    // consecutive dim-1 indices, so global reads are coalesced
+#pragma unroll 2
+    for (int e = t; e < L * L; e += C::TPI) {
+      const int k0 = e % L, k1 = e / L;
+      T a = T(0), bb = T(0);
+      if (k0 < is0 && k1 < is1) {
+        if (ALG == ALG_MASK_NSQECC && pass == 1)
+          a = (T)load_px(J.mask, dt, (g0.tlf + k0) + J.dims[0] * (g1.tlf + k1));
+        else
+          a = (T)fval<ALG>(J, g0, g1, k0, k1, meanF32, meanF64);
+      }
+      if (k0 < ss0 && k1 < ss1) {
+        double g = gval<ALG>(J, g0, g1, k0, k1, meanG32, meanG64);
+        bb = (ALG == ALG_MASK_NSQECC && pass == 1) ? (T)(g * g) : (T)g;
+      }
+      Z[k1 * C::LD + k0] = cpx<T>{a, bb};
+    }
+    __syncthreads();
+    correlate_2d<L, LPL, T>(Z, t, ns0, ns1);
+    if (ALG == ALG_MASK_NSQECC && pass == 0) {
+      for (int i = t; i < ns0 * ns1; i += C::TPI) C1[i] = S[i];
+      __syncthreads();
+    }
+  }
+
+  // ---- normalisation on the fully-overlapping window
+  if constexpr (ALG == ALG_ZNCC || ALG == ALG_NSQECC) {
+    // FP64 box sums of G^2 (the window sums of common_ops.jl:203-245):
+    // A(k0, s1) = sum_{j1 in [s1, s1+is1)} g(k0, j1)^2 by lanes along dim 1
+    // (coalesced), then W(s0, s1) = sum_{k0 in [s0, s0+is0)} A(k0, s1) by lanes
+    // along dim 2.
+    double* A = reinterpret_cast<double*>(base + sim_bytes(ns0, ns1, sizeof(T)));
+    const int lda = ss0 + 1;
+    if (t < ss0) {
+      double acc = 0.0;
+      for (int j1 = 0; j1 < is1; ++j1) {
+        double g = gval<ALG>(J, g0, g1, t, j1, meanG32, meanG64);
+        acc += g * g;
+      }
+      for (int s1 = 0; s1 < ns1; ++s1) {
+        A[s1 * lda + t] = acc;
+        if (s1 + 1 < ns1) {
+          double gin = gval<ALG>(J, g0, g1, t, s1 + is1, meanG32, meanG64);
+          double gout = gval<ALG>(J, g0, g1, t, s1, meanG32, meanG64);
+          acc += gin * gin;
+          acc -= gout * gout;
+        }
+      }
+    }
+    __syncthreads();
+    if (t < ns1) {
+      const double sqF = sqrt(sumF2);
+      double acc = 0.0;
+      for (int k0 = 0; k0 < is0; ++k0) acc += A[t * lda + k0];
+      for (int s0 = 0; s0 < ns0; ++s0) {
+        const double W = acc;
+        double c = (double)S[t * ns0 + s0];
+        double den = sqrt(W) * sqF;
+        double r;
+        if constexpr (ALG == ALG_ZNCC) {
+          r = c / den;
+        } else {
+          double num = (sumF2 - 2.0 * c) + W;
+          r = den / (num + den);
+        }
+        S[t * ns0 + s0] = (T)r;
+        if (s0 + 1 < ns0) {
+          acc += A[t * lda + s0 + is0];
+          acc -= A[t * lda + s0];
+        }
+      }
+    }
+    __syncthreads();
+  } else if constexpr (ALG == ALG_MASK_NSQECC) {
+    const double sqF = sqrt(sumF2);
+    for (int i = t; i < ns0 * ns1; i += C::TPI) {
+      double c1 = (double)C1[i];
+      double c2 = (double)S[i];
+      double num = (-2.0 * c1 + sumF2) + c2;
+      double den = sqrt(fabs(c2)) * sqF;
+      S[i] = (T)(den / (num + den));
+    }
+    __syncthreads();
+  }
+
+  // ---- peak over [off_lo, 2SM - off_hi] per dim (common_ops.jl:98-124)
+  const int lo0 = g0.goff, hi0 = 2 * sm0 - g0.off_hi;
+  const int lo1 = g1.goff, hi1 = 2 * sm1 - g1.off_hi;
+  Best<T> bst{T(0), INT32_MAX};
+  const int w0 = hi0 - lo0 + 1, w1 = hi1 - lo1 + 1;
+  const int nwin = (w0 > 0 && w1 > 0) ? w0 * w1 : 0;
+  for (int e = t; e < nwin; e += C::TPI) {
+    int p0 = lo0 + e % w0, p1 = lo1 + e / w0;
+    int li = p0 + ns0 * p1;
+    T v = S[li];
+    if (v > bst.v) bst = Best<T>{v, li};
+  }
+  {
+#pragma unroll
+    for (int m = 1; m < Group<C::TPI>::kSpan; m <<= 1) {
+      Best<T> o{__shfl_xor(bst.v, m), __shfl_xor(bst.i, m)};
+      bst = best_of(bst, o);
+    }
+    if constexpr (C::TPI > 64) {
+      __syncthreads();
+      T* rv = reinterpret_cast<T*>(red);
+      int32_t* ri = reinterpret_cast<int32_t*>(red + 4);
+      if ((t & 63) == 0) {
+        rv[t >> 6] = bst.v;
+        ri[t >> 6] = bst.i;
+      }
+      __syncthreads();
+      Best<T> r{rv[0], ri[0]};
+      for (int w = 1; w < C::NW; ++w) r = best_of(r, Best<T>{rv[w], ri[w]});
+      bst = r;
+    }
+  }
+
+  // ---- SN over [off_lo, 2SM-1-off_hi] (piv_ops.jl:336-367)
+  double sn = 0.0;
+  if (J.compute_sn) {
+    const int shi0 = 2 * sm0 - 1 - g0.off_hi, shi1 = 2 * sm1 - 1 - g1.off_hi;
+    const int u0 = shi0 - lo0 + 1, u1 = shi1 - lo1 + 1;
+    const int nsn = (u0 > 0 && u1 > 0) ? u0 * u1 : 0;
+    double mx = -INFINITY, sm = 0.0;
+    for (int e = t; e < nsn; e += C::TPI) {
+      int p0 = lo0 + e % u0, p1 = lo1 + e / u0;
+      double v = (double)S[p0 + ns0 * p1];
+      mx = (v > mx || v != v) ? v : mx;
+      sm += v;
+    }
+    mx = Group<C::TPI>::reduce(mx, OpMaxNaN{}, red, t);
+    sm = Group<C::TPI>::reduce(sm, OpAdd{}, red, t);
+    sn = nsn > 0 ? mx / ((sm - mx) / (double)(nsn - 1)) : (double)NAN;
+  }
+
+  // ---- Gaussian subpixel + write (piv_ops.jl:249-326, 236-243)
+  if (valid && t == 0) {
+    double d0, d1;
+    if (skip) {
+      d0 = d1 = 0.0;
+      sn = 0.0;
+    } else if (bst.i == INT32_MAX) {
+      d0 = -(double)(sm0 + 1);
+      d1 = -(double)(sm1 + 1);
+    } else {
+      const int p0 = bst.i % ns0, p1 = bst.i / ns0;  // 0-based; reference peak = p + 1
+      const bool interior = (p0 > lo0) && (p0 < hi0) && (p1 > lo1) && (p1 < hi1);
+      d0 = (double)(p0 + 1) - (double)(sm0 + 1);
+      d1 = (double)(p1 + 1) - (double)(sm1 + 1);
+      if (interior) {
+        double mn = INFINITY;
+        bool nan = false;
+        for (int b1 = -1; b1 <= 1; ++b1)
+          for (int b0 = -1; b0 <= 1; ++b0) {
+            double v = (double)S[(p0 + b0) + ns0 * (p1 + b1)];
+            if (v != v) nan = true;
+            mn = v < mn ? v : mn;
+          }
+        if (nan) mn = NAN;
+        const double mxv = (double)S[p0 + ns0 * p1];
+        const double lmx = log(1.0 + mxv - mn);
+        const double up = log(1.0 + (double)S[(p0 - 1) + ns0 * p1] - mn);
+        const double dn = log(1.0 + (double)S[(p0 + 1) + ns0 * p1] - mn);
+        const double lf = log(1.0 + (double)S[p0 + ns0 * (p1 - 1)] - mn);
+        const double rt = log(1.0 + (double)S[p0 + ns0 * (p1 + 1)] - mn);
+        const double g0v = (up - dn) / (2 * up - 4 * lmx + 2 * dn);
+        const double g1v = (lf - rt) / (2 * lf - 4 * lmx + 2 * rt);
+        d0 = ((double)(p0 + 1) + g0v) - (double)(sm0 + 1);
+        d1 = ((double)(p1 + 1) + g1v) - (double)(sm1 + 1);
+      }
+    }
+    if (J.peaks) {
+      const bool has = !skip && bst.i != INT32_MAX;
+      J.peaks[2 * ia + 0] = has ? bst.i % ns0 + 1 : 0;
+      J.peaks[2 * ia + 1] = has ? bst.i / ns0 + 1 : 0;
+    }
+    if (J.out_dtype == DT_F32) {
+      float* vf = (float*)J.vf;
+      vf[2 * ia + 0] = (float)d0;
+      vf[2 * ia + 1] = (float)d1;
+      if (J.sn) ((float*)J.sn)[ia] = (float)sn;
+    } else {
+      double* vf = (double*)J.vf;
+      vf[2 * ia + 0] = d0;
+      vf[2 * ia + 1] = d1;
+      if (J.sn) ((double*)J.sn)[ia] = sn;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launch helpers (explicitly instantiated per (T, L) in mqpiv_k2d_*.hip).
+
+template <int L, typename T, int LPL>
+size_t lds_bytes_2d(int alg, int ns0, int ns1, int ss0) {
+  using C = Cfg2<L, T, LPL>;
+  size_t z = region_bytes_2d(L, sizeof(T), ns0, ns1, ss0);
+  size_t c1 = (alg == ALG_MASK_NSQECC) ? sim_bytes(ns0, ns1, sizeof(T)) : 0;
+  return (z + c1) * C::IPB + (size_t)C::IPB * kRedBytes;
+}
+
+template <int L, int ALG, typename T, int LPL>
+hipError_t launch_2d_t(const Job& j0, const Job* d_jobs, const int64_t* d_prefix, int n_jobs,
+                       int64_t n_ia_total, int ns0, int ns1, int ss0, hipStream_t s) {
+  using C = Cfg2<L, T, LPL>;
+  size_t lds = lds_bytes_2d<L, T, LPL>(ALG, ns0, ns1, ss0);
+  if (lds > 160 * 1024) return hipErrorInvalidValue;
+  static bool attr_set = false;
+  if (!attr_set) {
+    (void)hipFuncSetAttribute((const void*)piv2d_kernel<L, ALG, T, LPL>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    attr_set = true;
+  }
+  int64_t nb = (n_ia_total + C::IPB - 1) / C::IPB;
+  hipLaunchKernelGGL((piv2d_kernel<L, ALG, T, LPL>), dim3((unsigned)nb), dim3(C::NT), lds, s, j0,
+                     d_jobs, d_prefix, n_jobs, n_ia_total);
+  return hipGetLastError();
+}
+
+template <int L, typename T, int LPL>
+hipError_t launch_2d_l(int alg, const Job& j0, const Job* d_jobs, const int64_t* d_prefix, int n_jobs,
+                       int64_t n_ia_total, int ns0, int ns1, int ss0, hipStream_t s) {
+  switch (alg) {
+    case ALG_FFTCC:
+      return launch_2d_t<L, ALG_FFTCC, T, LPL>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, s);
+    case ALG_ZNCC:
+      return launch_2d_t<L, ALG_ZNCC, T, LPL>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, s);
+    case ALG_NSQECC:
+      return launch_2d_t<L, ALG_NSQECC, T, LPL>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, s);
+    case ALG_MASK_NSQECC:
+      return launch_2d_t<L, ALG_MASK_NSQECC, T, LPL>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1,
+                                                    ss0, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+}  // namespace mqpiv

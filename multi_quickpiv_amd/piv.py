@@ -1,0 +1,173 @@
+"""``PIV(img1, img2[, mask[, mask_IA]], pivparams; precision)`` on the MI355X path.
+
+Mirrors src/multi_quickPIV.jl:17-29 (the three ``PIV`` methods) with the
+outputs of src/utils/piv_ops.jl:25-31:
+
+* ``VF`` has shape ``(N, nVF_1, ..., nVF_N)``, component first (U = VF[0], ...),
+* ``SN`` has shape ``nVF`` or is ``None`` when ``computeSN`` is false.
+
+Arrays use the reference's dimension order: axis 0 is Julia's dim 1 (rows, U).
+They are handed to the C ABI in column-major order (dim 1 fastest), exactly the
+memory of a Julia array, so VF/SN come back in the reference's element order.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Any, Optional, Tuple
+
+import numpy as np
+
+from . import _lib
+from .params import (
+    PIVParameters,
+    _isize,
+    _smarg,
+    _step,
+    get_vectorfield_size,
+    mask_NSQECC,
+    maxval,
+    setPIVParameters,
+)
+
+
+def to_c_params(p: PIVParameters, corr_code: Optional[int] = None, mode: str = "auto") -> _lib.MqpivParams:
+    """Resolve a PIVParameters into the C struct (step = interSize - overlap when
+    ``step`` is ``nothing``, piv_parameters.jl:88).  ``mode`` picks the transform
+    precision: "fp32", "fp64" or "auto" (FP64 where FP32 misses the parity bar)."""
+    if not (p.unpadded and p.good_pad):
+        # piv_ops.jl:261-268: the padded branch prints an error and mis-indexes.
+        raise NotImplementedError("only unpadded = good_pad = true is supported (as in the reference)")
+    if p.multipass != 1:
+        # multi_quickPIV.jl:57 with piv_ops.jl:254: multipass > 1 is inconsistent.
+        raise NotImplementedError("multipass > 1 is not supported")
+    cp = _lib.MqpivParams()
+    cp.corr_alg = p.corr_alg.code if corr_code is None else corr_code
+    cp.ndims = p.ndims
+    isz, smg, stp = _isize(p), _smarg(p), _step(p)
+    for d in range(3):
+        cp.inter_size[d] = isz[d] if d < p.ndims else 1
+        cp.search_margin[d] = smg[d] if d < p.ndims else 0
+        cp.step[d] = stp[d] if d < p.ndims else 1
+    cp.compute_sn = 1 if p.computeSN else 0
+    cp.precision_mode = _lib.PRECISION_MODES[mode]
+    cp.threshold = p.threshold
+    cp.mask_threshold = p.mask_threshold
+    return cp
+
+
+def _as_input(a: Any) -> np.ndarray:
+    a = np.asarray(a)
+    if not np.issubdtype(a.dtype, np.number) or np.issubdtype(a.dtype, np.complexfloating):
+        raise TypeError("PIV inputs must be real arrays")
+    return a
+
+
+def _common_dtype(*arrs: np.ndarray) -> np.dtype:
+    if all(a.dtype == np.float32 for a in arrs):
+        return np.dtype(np.float32)
+    return np.dtype(np.float64)
+
+
+def _ia_slices(vf_idx: Tuple[int, ...], p: PIVParameters) -> Tuple[slice, ...]:
+    isz, stp = _isize(p), _step(p)
+    return tuple(slice(v * s, v * s + i) for v, s, i in zip(vf_idx, stp, isz))
+
+
+def host_skip_mask(img1: np.ndarray, mask: Optional[np.ndarray], p: PIVParameters,
+                   masked: bool) -> Optional[np.ndarray]:
+    """Per-IA skip flags for filters the device does not evaluate itself:
+    a user ``filtFun``/``mask_filtFun`` other than ``maxval`` (piv_ops.jl:134-157),
+    or the double-mask variant where the skip mask differs from the IA mask."""
+    vf = get_vectorfield_size(img1.shape, p)
+    n = int(np.prod(vf))
+    out = np.zeros(n, dtype=np.uint8)
+    for lin in range(n):
+        idx = np.unravel_index(lin, vf, order="F")
+        sl = _ia_slices(idx, p)
+        if masked:
+            skip = False
+            if p.threshold > 0:
+                skip = p.filtFun(img1[sl]) < p.threshold
+            if p.mask_threshold > 0:
+                skip = skip or (p.mask_filtFun(mask[sl]) < p.mask_threshold)
+        else:
+            skip = p.threshold >= 0 and p.filtFun(img1[sl]) < p.threshold
+        out[lin] = 1 if skip else 0
+    return out
+
+
+def PIV(input1: Any, input2: Any, *args: Any, precision: Any = np.float64,  # noqa: N802
+        device: int = 0, return_peaks: bool = False, mode: str = "auto"):
+    """PIV(img1, img2[, pivparams]) / PIV(img1, img2, mask[, pivparams]) /
+    PIV(img1, img2, mask_VF, mask_IA[, pivparams]) -> (VF, SN).
+
+    With return_peaks=True also returns the 1-based integer correlation peaks,
+    shape (N, nVF...) int32 (0 where no value beats 0 or the IA is skipped).
+    ``mode``: transform precision, "auto" (default), "fp32" or "fp64"."""
+    args = list(args)
+    pivparams = setPIVParameters()
+    if args and isinstance(args[-1], PIVParameters):
+        pivparams = args.pop()
+    if len(args) > 2:
+        raise TypeError("PIV takes at most two masks")
+    img1 = _as_input(input1)
+    img2 = _as_input(input2)
+    masks = [_as_input(m) for m in args]
+    # multi_quickPIV.jl:48/108/167
+    if img1.shape != img2.shape or any(m.shape != img1.shape for m in masks):
+        raise AssertionError("PIV inputs need to have the same size.")
+    N = img1.ndim
+    pivparams.ndims = N  # multi_quickPIV.jl:50 mutates the caller's parameters too
+    masked = len(masks) > 0
+    corr_code = mask_NSQECC.code if masked else None
+
+    dt = _common_dtype(img1, img2, *masks)
+    in_code = _lib.MQPIV_F32 if dt == np.float32 else _lib.MQPIV_F64
+    a1 = np.asfortranarray(img1, dtype=dt)
+    a2 = np.asfortranarray(img2, dtype=dt)
+    m_ia = np.asfortranarray(masks[-1], dtype=dt) if masked else None
+    m_vf = masks[0] if masked else None
+
+    out_dt = np.dtype(precision)
+    if out_dt not in (np.float32, np.float64):
+        raise TypeError("precision must be Float32 or Float64")
+    out_code = _lib.MQPIV_F32 if out_dt == np.float32 else _lib.MQPIV_F64
+
+    vf = get_vectorfield_size(img1.shape, pivparams)
+    VF = np.zeros((N, *vf), dtype=out_dt, order="F")
+    SN = np.zeros(vf, dtype=out_dt, order="F") if pivparams.computeSN else None
+
+    PK = np.zeros((N, *vf), dtype=np.int32, order="F") if return_peaks else None
+    skip = None
+    custom = (pivparams.filtFun is not maxval) or (masked and pivparams.mask_filtFun is not maxval)
+    if custom or len(masks) == 2:
+        skip = host_skip_mask(img1, m_vf, pivparams, masked)
+    if int(np.prod(vf)) == 0:
+        return (VF, SN, PK) if return_peaks else (VF, SN)
+
+    cp = to_c_params(pivparams, corr_code, mode)
+    dims = (C.c_int64 * 3)(*(list(img1.shape) + [1] * (3 - N)))
+    ctx = _lib.context(device)
+    rc = ctx.lib.mqpiv_piv_ex(
+        ctx.handle, C.byref(cp), dims, in_code,
+        a1.ctypes.data, a2.ctypes.data,
+        m_ia.ctypes.data if m_ia is not None else None,
+        skip.ctypes.data if skip is not None else None,
+        out_code, VF.ctypes.data, SN.ctypes.data if SN is not None else None,
+        PK.ctypes.data if PK is not None else None,
+    )
+    ctx.check(rc)
+    return (VF, SN, PK) if return_peaks else (VF, SN)
+
+
+def piv_device(ctx: "_lib.Context", pivparams: PIVParameters, shape: Tuple[int, ...], in_dtype: int,
+               d_img1: int, d_img2: int, d_mask: Optional[int], out_dtype: int, d_vf: int,
+               d_sn: Optional[int], stream: Optional[int] = None, masked: bool = False,
+               mode: str = "auto") -> None:
+    """Asynchronous PIV over device-resident buffers (raw device pointers, e.g.
+    ``tensor.data_ptr()``; tensors laid out column-major as in ``PIV``)."""
+    cp = to_c_params(pivparams, mask_NSQECC.code if masked else None, mode)
+    dims = (C.c_int64 * 3)(*(list(shape) + [1] * (3 - len(shape))))
+    rc = ctx.lib.mqpiv_piv_device(ctx.handle, C.byref(cp), dims, in_dtype, d_img1, d_img2, d_mask,
+                                  None, out_dtype, d_vf, d_sn, stream)
+    ctx.check(rc)

@@ -1,0 +1,192 @@
+"""Python access to the Float64 CPU oracle (oracle/mqpiv_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as the
+checker, and by bench.py for the timed CPU baseline.  The product package never
+imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+from typing import Optional, Sequence, Tuple
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liborc.so"
+
+FFTCC, ZNCC, NSQECC, MASK_NSQECC = 0, 1, 2, 3
+DIRECT, FFT = 0, 1
+ALG_CODES = {"fft": FFTCC, "fftcc": FFTCC, "zncc": ZNCC, "nsqecc": NSQECC, "mask_nsqecc": MASK_NSQECC}
+
+
+class OrcParams(C.Structure):
+    _fields_ = [
+        ("alg", C.c_int32),
+        ("ndims", C.c_int32),
+        ("is_", C.c_int64 * 3),
+        ("sm", C.c_int64 * 3),
+        ("st", C.c_int64 * 3),
+        ("compute_sn", C.c_int32),
+        ("in_dtype", C.c_int32),
+        ("threshold", C.c_double),
+        ("mask_threshold", C.c_double),
+    ]
+
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+        lib = C.CDLL(str(LIB_PATH))
+        vp = C.c_void_p
+        lib.orc_good_size_real.restype = C.c_int64
+        lib.orc_good_size_real.argtypes = [C.c_int64]
+        lib.orc_vf_size.restype = C.c_int64
+        lib.orc_vf_size.argtypes = [C.POINTER(OrcParams), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        lib.orc_piv.restype = C.c_int
+        lib.orc_piv.argtypes = [C.POINTER(OrcParams), C.POINTER(C.c_int64), vp, vp, vp, vp, C.c_int,
+                                C.c_int, C.c_int64, C.c_int64, vp, vp, vp]
+        lib.orc_ia_similarity.restype = C.c_int
+        lib.orc_ia_similarity.argtypes = [C.POINTER(OrcParams), C.POINTER(C.c_int64), vp, vp, vp,
+                                          C.c_int64, C.c_int, vp, vp]
+        lib.orc_ia_correlation.restype = C.c_int
+        lib.orc_ia_correlation.argtypes = [C.POINTER(OrcParams), C.POINTER(C.c_int64), vp, vp,
+                                           C.c_int64, C.c_int, vp]
+        lib.orc_coords_of.restype = None
+        lib.orc_coords_of.argtypes = [C.POINTER(OrcParams), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                      C.c_int64, vp]
+        lib.orc_peak.restype = None
+        lib.orc_peak.argtypes = [vp, C.POINTER(C.c_int64), C.c_int, C.POINTER(C.c_int64),
+                                 C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                 C.POINTER(C.c_double)]
+        lib.orc_subpixel.restype = None
+        lib.orc_subpixel.argtypes = [vp, C.POINTER(C.c_int64), C.c_int, C.POINTER(C.c_int64),
+                                     C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                     C.c_double, C.POINTER(C.c_double)]
+        lib.orc_sn.restype = C.c_double
+        lib.orc_sn.argtypes = [vp, C.POINTER(C.c_int64), C.c_int, C.POINTER(C.c_int64),
+                               C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        _lib = lib
+    return _lib
+
+
+def _i3(v: Sequence[int], fill: int) -> "C.Array":
+    v = list(v) + [fill] * (3 - len(v))
+    return (C.c_int64 * 3)(*v)
+
+
+def make_params(alg, ndims: int, inter_size, search_margin, step, compute_sn: bool = True,
+                in_dtype=np.float64, threshold: float = -1.0, mask_threshold: float = -1.0) -> OrcParams:
+    p = OrcParams()
+    p.alg = ALG_CODES[alg.lower()] if isinstance(alg, str) else int(alg)
+    p.ndims = ndims
+    tup = lambda x: (x,) * ndims if isinstance(x, (int, np.integer)) else tuple(x)
+    p.is_ = _i3(tup(inter_size), 1)
+    p.sm = _i3(tup(search_margin), 0)
+    p.st = _i3(tup(step), 1)
+    p.compute_sn = 1 if compute_sn else 0
+    p.in_dtype = 0 if np.dtype(in_dtype) == np.float32 else 1
+    p.threshold = threshold
+    p.mask_threshold = mask_threshold
+    return p
+
+
+def good_size_real(n: int) -> int:
+    return int(load().orc_good_size_real(n))
+
+
+def vf_size(p: OrcParams, shape: Sequence[int]) -> Tuple[int, ...]:
+    vf = (C.c_int64 * 3)()
+    load().orc_vf_size(C.byref(p), _i3(shape, 1), vf)
+    return tuple(vf[d] for d in range(p.ndims))
+
+
+def _prep(img: Optional[np.ndarray], dtype) -> Optional[np.ndarray]:
+    if img is None:
+        return None
+    return np.asfortranarray(img, dtype=dtype)
+
+
+def piv(img1, img2, p: OrcParams, mask=None, skip_mask=None, mode: int = DIRECT, nthreads: int = 1,
+        ia_begin: int = 0, ia_end: int = -1, want_aux: bool = False):
+    """PIV_CPU restated in Float64.  Returns (VF, SN, aux) with VF of shape
+    (N, nVF...), SN of shape nVF (or None), aux of shape (6, nVF...) holding the
+    1-based integer peak (r, c, z), the peak value and the runner-up value in the
+    peak window (or None)."""
+    lib = load()
+    dt = np.float32 if p.in_dtype == 0 else np.float64
+    a1, a2, am = _prep(img1, dt), _prep(img2, dt), _prep(mask, dt)
+    shape = a1.shape
+    vf = vf_size(p, shape)
+    n = int(np.prod(vf))
+    VF = np.zeros((p.ndims, *vf), dtype=np.float64, order="F")
+    SN = np.zeros(vf, dtype=np.float64, order="F") if p.compute_sn else None
+    aux = np.zeros((6, *vf), dtype=np.float64, order="F") if want_aux else None
+    sk = np.ascontiguousarray(skip_mask, dtype=np.uint8) if skip_mask is not None else None
+    if n:
+        rc = lib.orc_piv(C.byref(p), _i3(shape, 1), a1.ctypes.data, a2.ctypes.data,
+                         am.ctypes.data if am is not None else None,
+                         sk.ctypes.data if sk is not None else None, mode, nthreads, ia_begin, ia_end,
+                         VF.ctypes.data, SN.ctypes.data if SN is not None else None,
+                         aux.ctypes.data if aux is not None else None)
+        if rc != 0:
+            raise ValueError(f"orc_piv failed ({rc})")
+    return VF, SN, aux
+
+
+def ia_similarity(img1, img2, p: OrcParams, vf_idx: int, mask=None, mode: int = DIRECT):
+    """Similarity window of the IA with 1-based linear index vf_idx, shape
+    (2SM+1,)*N in column-major order, and its 18 coordinates
+    (TLF, BRB, SR_TLF, SR_BRB, SR_TLF_off, SR_BRB_off; 3 each, 1-based)."""
+    lib = load()
+    dt = np.float32 if p.in_dtype == 0 else np.float64
+    a1, a2, am = _prep(img1, dt), _prep(img2, dt), _prep(mask, dt)
+    ns = tuple(2 * p.sm[d] + 1 for d in range(p.ndims))
+    out = np.zeros(ns, dtype=np.float64, order="F")
+    co = np.zeros(18, dtype=np.int64)
+    lib.orc_ia_similarity(C.byref(p), _i3(a1.shape, 1), a1.ctypes.data, a2.ctypes.data,
+                          am.ctypes.data if am is not None else None, vf_idx, mode, out.ctypes.data,
+                          co.ctypes.data)
+    return out, co.reshape(6, 3)
+
+
+def ia_correlation(img1, img2, p: OrcParams, vf_idx: int, mode: int = DIRECT):
+    lib = load()
+    dt = np.float32 if p.in_dtype == 0 else np.float64
+    a1, a2 = _prep(img1, dt), _prep(img2, dt)
+    ns = tuple(2 * p.sm[d] + 1 for d in range(p.ndims))
+    out = np.zeros(ns, dtype=np.float64, order="F")
+    lib.orc_ia_correlation(C.byref(p), _i3(a1.shape, 1), a1.ctypes.data, a2.ctypes.data, vf_idx, mode,
+                           out.ctypes.data)
+    return out
+
+
+def coords(p: OrcParams, shape: Sequence[int], vf_idx: int) -> np.ndarray:
+    lib = load()
+    vf = (C.c_int64 * 3)()
+    lib.orc_vf_size(C.byref(p), _i3(shape, 1), vf)
+    co = np.zeros(18, dtype=np.int64)
+    lib.orc_coords_of(C.byref(p), _i3(shape, 1), vf, vf_idx, co.ctypes.data)
+    return co.reshape(6, 3)
+
+
+def peak_subpixel_sn(S: np.ndarray, sm: Sequence[int], off_lo: Sequence[int], off_hi: Sequence[int]):
+    """first_fullovp_peak + gaussian refinement + compute_SN on a given window."""
+    lib = load()
+    S = np.asfortranarray(S, dtype=np.float64)
+    nd = S.ndim
+    ns = _i3(S.shape, 1)
+    sm3, lo3, hi3 = _i3(sm, 0), _i3(off_lo, 0), _i3(off_hi, 0)
+    pk = (C.c_int64 * 3)()
+    val = C.c_double()
+    lib.orc_peak(S.ctypes.data, ns, nd, sm3, lo3, hi3, pk, C.byref(val))
+    disp = (C.c_double * 3)()
+    lib.orc_subpixel(S.ctypes.data, ns, nd, sm3, lo3, hi3, pk, val.value, disp)
+    sn = lib.orc_sn(S.ctypes.data, ns, nd, sm3, lo3, hi3)
+    return tuple(pk[d] for d in range(nd)), val.value, tuple(disp[d] for d in range(nd)), sn
