@@ -1,0 +1,227 @@
+"""Benchmark of the 2-D cross-correlation PIV hot path on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): one FP32
+2048 x 2048 frame pair at 0.05 particles/px (Gaussian sigma 1.2 px, peak
+U[0.5, 1], N(0, 0.02) noise), frame 2 displaced by a Lamb-Oseen vortex (core
+radius 300 px, peak 8 px) plus a (1.5, 0.5) px drift.  One step runs PIV on that
+resident pair four times: ZNCC and NSQECC, each at interSize 32 and 64, with
+searchMargin 16, step 16 and computeSN -- 16,129 + 16,129 + 15,625 + 15,625 =
+63,508 IA correlations (vectors with subpixel peak and SN).
+
+value = IA correlations per second over the timed steps (inputs resident in
+HBM, outputs written to HBM).  Each kernel is timed with HIP events on the
+stream it runs on; the roofline entry is for the kernel with the largest total
+time.  The CPU baseline is the Float64 oracle's FFT mode (a restatement of the
+reference's PIV_CPU loop with per-IA transforms) over the same four runs,
+OpenMP over IAs on the host cores.
+
+    python bench.py [--steps K] [--warmup W] [--mode auto|fp32|fp64] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "IA correlations/sec (2D 32², 3D 32³) at stated px error; p50/p99 latency"
+RUNS = [("zncc", 32), ("nsqecc", 32), ("zncc", 64), ("nsqecc", 64)]
+SM, STEP, SIZE = 16, 16, 2048
+
+# MI355X published peaks
+PEAK_FP32_TFLOPS = 157.3
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def flop_fft_per_ia(cs: int, ndims: int = 2, transforms: int = 3) -> float:
+    """Algorithmic flops of one IA on the plain transform path at the reference
+    transform size cs: T*2.5*P*log2(P) + (T/3)*(6*Pc + P)."""
+    P = cs ** ndims
+    h = cs // 2 + 1
+    Pc = h * cs ** (ndims - 1)
+    return transforms * 2.5 * P * math.log2(P) + (transforms / 3) * (6 * Pc + P)
+
+
+def uses_fp64(mode: str, alg: str, compute_sn: bool, ss: int) -> bool:
+    if mode == "fp64":
+        return True
+    if mode == "auto":
+        return (alg == "mask_nsqecc" or (alg == "zncc" and compute_sn)) and ss <= 96
+    return False
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--mode", default="auto", choices=["auto", "fp32", "fp64"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
+                          "n_gpus": args.gpus}))
+        return 2
+
+    import torch
+
+    from multi_quickpiv_amd import _lib, setPIVParameters, synth
+    from multi_quickpiv_amd.params import get_vectorfield_size
+    from multi_quickpiv_amd.piv import piv_device
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    a, b = synth.config_c2(SIZE, device="cuda")
+    # column-major device buffers (dim 1 fastest): a (d1, d2) tensor's transpose
+    # stored contiguously has exactly that memory
+    A = a.t().contiguous()
+    B = b.t().contiguous()
+    ctx = _lib.context(0)
+    # a dedicated (default-priority) stream: the kernels and the timing events
+    # share it
+    tstream = torch.cuda.Stream()
+    torch.cuda.set_stream(tstream)
+    stream = tstream.cuda_stream
+
+    jobs = []
+    for alg, isz in RUNS:
+        p = setPIVParameters(corr_alg=alg, interSize=isz, searchMargin=SM, step=STEP, computeSN=True)
+        p.ndims = 2
+        vf = get_vectorfield_size((SIZE, SIZE), p)
+        n = int(np.prod(vf))
+        VF = torch.zeros((n * 2,), dtype=torch.float64, device=dev)
+        SNb = torch.zeros((n,), dtype=torch.float64, device=dev)
+        jobs.append(dict(alg=alg, isz=isz, p=p, vf=vf, n=n, VF=VF, SN=SNb,
+                         fp64=uses_fp64(args.mode, alg, True, isz + 2 * SM)))
+
+    def launch(j):
+        piv_device(ctx, j["p"], (SIZE, SIZE), _lib.MQPIV_F32, A.data_ptr(), B.data_ptr(), None,
+                   _lib.MQPIV_F64, j["VF"].data_ptr(), j["SN"].data_ptr(), stream, mode=args.mode)
+
+    for _ in range(args.warmup):
+        for j in jobs:
+            launch(j)
+    torch.cuda.synchronize()
+
+    ev = [[(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in jobs] for _ in range(args.steps)]
+    step_ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    ctx.profile_begin()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        step_ev[s][0].record()
+        for k, j in enumerate(jobs):
+            ev[s][k][0].record()
+            launch(j)
+            ev[s][k][1].record()
+        step_ev[s][1].record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kt = ctx.profile_end()  # correlation-kernel durations, launch order
+
+    n_step = sum(j["n"] for j in jobs)
+    value = n_step * args.steps / wall
+    call_ms = [float(np.mean([ev[s][k][0].elapsed_time(ev[s][k][1]) for s in range(args.steps)]))
+               for k in range(len(jobs))]
+    assert len(kt) == args.steps * len(jobs), len(kt)
+    kern_ms = [float(np.mean(kt[k::len(jobs)])) for k in range(len(jobs))]
+    step_ms = np.array([step_ev[s][0].elapsed_time(step_ev[s][1]) for s in range(args.steps)])
+
+    # roofline of the dominant kernel
+    kd = int(np.argmax(kern_ms))
+    jd = jobs[kd]
+    cs = jd["isz"] + 2 * SM  # reference transform size (good_size_real is the identity here)
+    flop = flop_fft_per_ia(cs) * jd["n"]
+    achieved = flop / (kern_ms[kd] * 1e-3) / 1e12
+    peak = PEAK_FP64_TFLOPS if jd["fp64"] else PEAK_FP32_TFLOPS
+    traffic = None
+    tfile = ROOT / "profiles" / "hbm_traffic.md"
+    if tfile.exists():
+        for line in tfile.read_text().splitlines():
+            if line.startswith(f"traffic_bytes[{jd['alg']}{jd['isz']}_{'f64' if jd['fp64'] else 'f32'}]"):
+                traffic = float(line.split("=")[1].split()[0])
+
+    # parity spot check of the timed outputs against the resident data (the
+    # tests hold the full parity; here: finite vectors, the vortex is resolved)
+    VF0 = jobs[1]["VF"].view(jobs[1]["vf"][1], jobs[1]["vf"][0], 2).cpu().numpy()
+    finite = bool(np.isfinite(VF0).all())
+
+    breakdown = []
+    for k, j in enumerate(jobs):
+        breakdown.append({"alg": j["alg"], "interSize": j["isz"], "n_ia": j["n"],
+                          "transforms": "f64" if j["fp64"] else "f32",
+                          "kernel_ms": round(kern_ms[k], 4),
+                          "call_ms": round(call_ms[k], 4),
+                          "ia_per_s": round(j["n"] / (kern_ms[k] * 1e-3), 1)})
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+
+        nthreads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        ha = a.cpu().numpy()
+        hb = b.cpu().numpy()
+        t_cpu = 0.0
+        n_cpu = 0
+        for j in jobs:
+            op = O.make_params(j["alg"], 2, j["isz"], SM, STEP, True, np.float32)
+            t1 = time.perf_counter()
+            O.piv(ha, hb, op, mode=O.FFT, nthreads=nthreads)
+            t_cpu += time.perf_counter() - t1
+            n_cpu += j["n"]
+        cpu = {"value": round(n_cpu / t_cpu, 1), "unit": "IA/s", "cores": nthreads, "kind": "port",
+               "sample": f"the full step ({n_cpu} IAs: the four runs once), Float64 oracle FFT mode "
+                         f"(per-IA transforms at the reference size), OpenMP over IAs, {t_cpu:.1f} s"}
+
+    dtype = "f32+f64" if any(j["fp64"] for j in jobs) and not all(j["fp64"] for j in jobs) else (
+        "f64" if all(j["fp64"] for j in jobs) else "f32")
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "IA/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic",
+        "config": {
+            "workload": "2D ZNCC+NSQECC, FP32 2048x2048 Lamb-Oseen particle pair, interSize 32 and 64, "
+                        "searchMargin 16, step 16, computeSN",
+            "ia_per_step": n_step,
+            "precision_mode": args.mode,
+        },
+        "latency_ms": {"p50": round(float(np.percentile(step_ms, 50)), 4),
+                       "p99": round(float(np.percentile(step_ms, 99)), 4),
+                       "what": "one frame pair through all four runs"},
+        "roofline": {"bound": "valu", "kernel": f"piv2d_kernel {jd['alg']} interSize {jd['isz']} "
+                                                 f"({'f64' if jd['fp64'] else 'f32'} transforms)",
+                     "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+                     "frac": round(achieved / peak, 4), "traffic": traffic,
+                     "flop_per_ia": round(flop_fft_per_ia(cs)), "ia_per_launch": jd["n"],
+                     "kernel_ms": round(kern_ms[kd], 4)},
+        "breakdown": breakdown,
+        "outputs_finite": finite,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

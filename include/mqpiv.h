@@ -114,6 +114,13 @@ int mqpiv_piv_device(mqpiv_ctx* ctx, const mqpiv_params* p, const int64_t* dims,
                      const uint8_t* d_skip_mask, int32_t out_dtype, void* d_VF, void* d_SN,
                      void* stream);
 
+/* Measurement hooks (no reference counterpart).  Between profile_begin and
+ * profile_end every correlation-kernel launch is bracketed by HIP events on the
+ * stream it runs on; profile_end synchronises and returns the durations in
+ * launch order (milliseconds; *n = count, at most cap are written). */
+int mqpiv_profile_begin(mqpiv_ctx* ctx);
+int mqpiv_profile_end(mqpiv_ctx* ctx, double* kernel_ms, int32_t cap, int32_t* n);
+
 /* Library build identification (for logs). */
 const char* mqpiv_version(void);
 

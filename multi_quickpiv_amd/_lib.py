@@ -38,6 +38,8 @@ EXPORTS = (
     "mqpiv_piv_ex",
     "mqpiv_piv_device",
     "mqpiv_version",
+    "mqpiv_profile_begin",
+    "mqpiv_profile_end",
 )
 
 
@@ -97,6 +99,10 @@ def load() -> C.CDLL:
         lib.mqpiv_piv_device.restype = C.c_int
         lib.mqpiv_piv_device.argtypes = [vp, C.POINTER(MqpivParams), C.POINTER(C.c_int64), C.c_int32,
                                          vp, vp, vp, vp, C.c_int32, vp, vp, vp]
+        lib.mqpiv_profile_begin.restype = C.c_int
+        lib.mqpiv_profile_begin.argtypes = [vp]
+        lib.mqpiv_profile_end.restype = C.c_int
+        lib.mqpiv_profile_end.argtypes = [vp, C.POINTER(C.c_double), C.c_int32, C.POINTER(C.c_int32)]
         _lib = lib
         return lib
 
@@ -117,6 +123,16 @@ class Context:
         if rc != MQPIV_OK:
             msg = self.lib.mqpiv_last_error(self.handle).decode(errors="replace")
             raise MqpivError(rc, msg)
+
+    def profile_begin(self) -> None:
+        self.check(self.lib.mqpiv_profile_begin(self.handle))
+
+    def profile_end(self, cap: int = 1 << 16) -> list:
+        """Durations (ms) of the correlation kernels launched since profile_begin."""
+        buf = (C.c_double * cap)()
+        n = C.c_int32()
+        self.check(self.lib.mqpiv_profile_end(self.handle, buf, cap, C.byref(n)))
+        return [buf[i] for i in range(min(n.value, cap))]
 
     def close(self) -> None:
         if self.handle:

@@ -48,6 +48,7 @@ SOURCES = [
     "mqpiv_k2d_f64_16.hip",
     "mqpiv_kernels2d.hip",
     "mqpiv_kernels3d.hip",
+    "mqpiv_sat.hip",
     "mqpiv_api.cpp",
 ]
 

@@ -12,6 +12,8 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "../../include/mqpiv.h"
 #include "mqpiv_internal.h"
@@ -21,6 +23,10 @@ int fft_len_2d(int fp64, int ss);
 size_t lds_2d(int fp64, int L, int alg, int ns0, int ns1, int ss0);
 hipError_t launch_piv2d(int fp64, int L, int alg, const Job& j0, const Job* d_jobs, const int64_t* d_prefix,
                         int n_jobs, int64_t n_ia_total, int ns0, int ns1, int ss0, hipStream_t s);
+size_t sat_bytes(const int64_t* dims, int nd);
+size_t sat_tmp_bytes(const int64_t* dims, int nd);
+hipError_t build_sat(const void* img, int dt, int power, const int64_t* dims, int nd, double* S, double* tmp,
+                     hipStream_t s);
 int fft_len_3d(int ss);
 hipError_t launch_piv3d(const Job& j, void* scratch, size_t scratch_bytes, hipStream_t s);
 size_t scratch_3d_bytes(const Job& j, int64_t chunk);
@@ -40,7 +46,48 @@ struct mqpiv_ctx {
   // scratch for multi-kernel (3-D) paths
   void* d_scratch = nullptr;
   size_t scratch_bytes = 0;
+  // FP64 integral arrays of the search image (sum, sum of squares) + scan scratch
+  void* d_sat = nullptr;
+  size_t sat_cap = 0;
+  // measurement hooks
+  bool profiling = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
 };
+
+// Brackets one correlation-kernel launch with events when profiling is on.
+struct KernelTimer {
+  mqpiv_ctx* c;
+  hipStream_t s;
+  hipEvent_t a = nullptr, b = nullptr;
+  KernelTimer(mqpiv_ctx* c_, hipStream_t s_) : c(c_), s(s_) {
+    if (c->profiling && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess)
+      hipEventRecord(a, s);
+  }
+  ~KernelTimer() {
+    if (a && b) {
+      hipEventRecord(b, s);
+      c->events.emplace_back(a, b);
+    }
+  }
+};
+
+static int fail(mqpiv_ctx* c, int code, const char* fmt, ...) __attribute__((format(printf, 3, 4)));
+
+// Grow-only device buffer owned by the context.
+static int ensure(mqpiv_ctx* c, void** buf, size_t* cap, size_t need, hipStream_t s) {
+  if (need <= *cap) return MQPIV_OK;
+  if (*buf) {
+    hipStreamSynchronize(s);
+    hipFree(*buf);
+    *buf = nullptr;
+    *cap = 0;
+  }
+  hipError_t e = hipMalloc(buf, need);
+  if (e != hipSuccess)
+    return fail(c, MQPIV_ERR_OOM, "hipMalloc(%zu) failed: %s", need, hipGetErrorString(e));
+  *cap = need;
+  return MQPIV_OK;
+}
 
 static int fail(mqpiv_ctx* c, int code, const char* fmt, ...) __attribute__((format(printf, 3, 4)));
 static int fail(mqpiv_ctx* c, int code, const char* fmt, ...) {
@@ -87,6 +134,7 @@ extern "C" int mqpiv_ctx_destroy(mqpiv_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   if (c->d_stage) hipFree(c->d_stage);
   if (c->d_scratch) hipFree(c->d_scratch);
+  if (c->d_sat) hipFree(c->d_sat);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return MQPIV_OK;
@@ -185,8 +233,29 @@ static int build_job(mqpiv_ctx* c, const mqpiv_params* p, const int64_t* dims, i
   return MQPIV_OK;
 }
 
-static int launch_job(mqpiv_ctx* c, const Job& j, hipStream_t s) {
+// Integral arrays the normalisation reads: NSQECC needs sum(G^2), ZNCC also sum(G).
+static int prepare_sats(mqpiv_ctx* c, Job& j, hipStream_t s) {
+  j.sat1 = j.sat2 = nullptr;
+  if (j.alg != ALG_ZNCC && j.alg != ALG_NSQECC) return MQPIV_OK;
+  const size_t one = (sat_bytes(j.dims, j.ndims) + 255) & ~(size_t)255;
+  const size_t tmp = (sat_tmp_bytes(j.dims, j.ndims) + 255) & ~(size_t)255;
+  const int nsat = j.alg == ALG_ZNCC ? 2 : 1;
+  int rc = ensure(c, &c->d_sat, &c->sat_cap, nsat * one + tmp, s);
+  if (rc) return rc;
+  char* b = (char*)c->d_sat;
+  double* t = (double*)(b + nsat * one);
+  hipError_t e = build_sat(j.img2, j.in_dtype, 2, j.dims, j.ndims, (double*)b, t, s);
+  if (e == hipSuccess && nsat == 2) e = build_sat(j.img2, j.in_dtype, 1, j.dims, j.ndims, (double*)(b + one), t, s);
+  if (e != hipSuccess) return fail(c, MQPIV_ERR_DEVICE, "integral array build failed: %s", hipGetErrorString(e));
+  j.sat2 = (const double*)b;
+  j.sat1 = nsat == 2 ? (const double*)(b + one) : nullptr;
+  return MQPIV_OK;
+}
+
+static int launch_job(mqpiv_ctx* c, Job& j, hipStream_t s) {
   if (j.n_ia == 0) return MQPIV_OK;
+  int src = prepare_sats(c, j, s);
+  if (src) return src;
   if (j.ndims == 2) {
     const int ss0 = j.is[0] + 2 * j.sm[0], ss1 = j.is[1] + 2 * j.sm[1];
     const int ss = std::max(ss0, ss1);
@@ -207,7 +276,11 @@ static int launch_job(mqpiv_ctx* c, const Job& j, hipStream_t s) {
     if (!fits(fp64))
       return fail(c, MQPIV_ERR_UNSUPPORTED, "search size %d exceeds the largest 2-D transform (128)", ss);
     const int L = fft_len_2d(fp64, ss);
-    hipError_t e = launch_piv2d(fp64, L, j.alg, j, nullptr, nullptr, 1, j.n_ia, ns0, ns1, ss0, s);
+    hipError_t e;
+    {
+      KernelTimer kt(c, s);
+      e = launch_piv2d(fp64, L, j.alg, j, nullptr, nullptr, 1, j.n_ia, ns0, ns1, ss0, s);
+    }
     if (e != hipSuccess) return fail(c, MQPIV_ERR_DEVICE, "2-D launch failed: %s", hipGetErrorString(e));
     return MQPIV_OK;
   }
@@ -312,5 +385,36 @@ extern "C" int mqpiv_piv_ex(mqpiv_ctx* c, const mqpiv_params* p, const int64_t* 
       HIP_OR_FAIL(c, hipMemcpyAsync(peaks, dp, j.n_ia * j.ndims * sizeof(int32_t), hipMemcpyDeviceToHost, s));
   }
   HIP_OR_FAIL(c, hipStreamSynchronize(s));
+  return MQPIV_OK;
+}
+
+extern "C" int mqpiv_profile_begin(mqpiv_ctx* c) {
+  if (!c) return MQPIV_ERR_NULL;
+  std::lock_guard<std::mutex> lk(c->mu);
+  for (auto& e : c->events) {
+    hipEventDestroy(e.first);
+    hipEventDestroy(e.second);
+  }
+  c->events.clear();
+  c->profiling = true;
+  return MQPIV_OK;
+}
+
+extern "C" int mqpiv_profile_end(mqpiv_ctx* c, double* kernel_ms, int32_t cap, int32_t* n) {
+  if (!c) return MQPIV_ERR_NULL;
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->profiling = false;
+  int32_t k = 0;
+  for (auto& e : c->events) {
+    hipEventSynchronize(e.second);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e.first, e.second);
+    if (kernel_ms && k < cap) kernel_ms[k] = ms;
+    ++k;
+    hipEventDestroy(e.first);
+    hipEventDestroy(e.second);
+  }
+  c->events.clear();
+  if (n) *n = k;
   return MQPIV_OK;
 }

@@ -1,14 +1,12 @@
 // mqpiv_fft.hpp -- register-resident complex FFTs for gfx950.
 //
-// A line of L complex samples is transformed in VGPRs by one lane (LPL = 1) or
-// split over a lane pair (LPL = 2): a self-sorting (Stockham) sequence of
+// A line of L complex samples is transformed in VGPRs by one lane or split over
+// an aligned group of 2 or 4 lanes: a self-sorting (Stockham) sequence of
 // radix-4/3/2 passes, fully unrolled, with every twiddle factor a compile-time
-// constant (an instruction literal, not a memory load).  For a lane pair the
-// forward direction is decimation in time (each lane transforms the even or odd
-// samples, then one radix-2 stage across the pair) and the inverse direction is
-// decimation in frequency (radix-2 across the pair first, then each lane
-// produces the even or odd outputs).  A 2-D transform is two such sweeps with an
-// LDS transpose in between; see mqpiv_kernels2d.hpp.
+// constant.  A lane group runs decimation in time in the forward direction and
+// decimation in frequency in the inverse direction, with radix-2 stages across
+// the group exchanged through DPP quad permutations.  A 2-D transform is two
+// such sweeps with an LDS transpose in between; see mqpiv_kernels2d.hpp.
 //
 // The transforms replace FFTW's in-place r2c/c2r pair that the reference drives
 // through its C API (src/utils/fft_ops.jl:11-83).  The lengths used here are the
@@ -159,6 +157,7 @@ __host__ __device__ __forceinline__ void stockham_pass(const cpx<T> (&a)[L], cpx
 template <int L>
 struct Plan;
 template <> struct Plan<8> { static constexpr int n = 2; static constexpr int r[4] = {4, 2, 1, 1}; };
+template <> struct Plan<12> { static constexpr int n = 2; static constexpr int r[4] = {4, 3, 1, 1}; };
 template <> struct Plan<16> { static constexpr int n = 2; static constexpr int r[4] = {4, 4, 1, 1}; };
 template <> struct Plan<24> { static constexpr int n = 3; static constexpr int r[4] = {4, 3, 2, 1}; };
 template <> struct Plan<32> { static constexpr int n = 3; static constexpr int r[4] = {4, 4, 2, 1}; };
@@ -192,47 +191,115 @@ __host__ __device__ __forceinline__ void fft_line(cpx<T> (&x)[L]) {
 }
 
 // ---------------------------------------------------------------------------
-// Lane-pair transforms of a line of length L (each lane holds H = L/2 samples).
-// The partner exchange is a function argument so the same code runs on the CPU
-// in unit tests (where both halves live in one thread).
+// Lane-group transforms: a line of length L split over P = 2 or 4 adjacent
+// lanes (an aligned pair / quad), each holding M = L/P samples.
+//
+// Forward (decimation in time): on entry lane q holds x[P*k + q], k < M; each
+// lane transforms its decimated subsequence, then log2(P) radix-2 stages across
+// the group (partner = q ^ 2 first, then q ^ 1 for P = 4) combine them; on exit
+// lane q holds X[chunk(q)*M + k] with chunk(q) = bit-reverse of q.
+// Inverse direction (decimation in frequency): on entry lane q holds
+// x[chunk(q)*M + k]; radix-2 stages across the group (q ^ 1, then q ^ 2) come
+// first, then the local transform; on exit lane q holds X[P*m + q].
+//
+// The partner exchange is a callable so the same code runs on the CPU in unit
+// tests.  On the GPU it is a DPP quad permutation (one VALU move per dword).
 
-template <typename T>
-__device__ __forceinline__ cpx<T> shfl_pair(cpx<T> v) {
-  return cpx<T>{__shfl_xor(v.x, 1), __shfl_xor(v.y, 1)};
+template <int P>
+__host__ __device__ constexpr int chunk_of(int q) {
+  return P == 2 ? q : ((q & 1) << 1) | (q >> 1);
 }
 
-// Forward (decimation in time).  On entry lane h holds x[2k + h], k < H; on
-// exit it holds X[h*H + k].
-template <int L, int SIGN, typename T, typename Xchg>
-__host__ __device__ __forceinline__ void fft_pair_dit(cpx<T> (&v)[L / 2], int h, Xchg xchg) {
-  constexpr int H = L / 2;
-  fft_line<H, SIGN, T>(v);
-  if (h) {
+#if defined(__HIP_DEVICE_COMPILE__)
+template <int MASK>
+__device__ __forceinline__ float dpp_xor(float v) {
+  // quad_perm selectors: lane ^ 1 -> [1,0,3,2] = 0xB1, lane ^ 2 -> [2,3,0,1] = 0x4E
+  constexpr int ctrl = MASK == 1 ? 0xB1 : 0x4E;
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), ctrl, 0xF, 0xF, false));
+}
+template <int MASK>
+__device__ __forceinline__ double dpp_xor(double v) {
+  constexpr int ctrl = MASK == 1 ? 0xB1 : 0x4E;
+  long long u = __builtin_bit_cast(long long, v);
+  int lo = __builtin_amdgcn_mov_dpp((int)(u & 0xffffffffll), ctrl, 0xF, 0xF, false);
+  int hi = __builtin_amdgcn_mov_dpp((int)(u >> 32), ctrl, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+#endif
+
+// Device exchange with the lane q ^ MASK of the same quad.
+struct DppXchg {
+  template <int MASK, typename T>
+  __device__ __forceinline__ cpx<T> get(cpx<T> v, int) const {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return cpx<T>{dpp_xor<MASK>(v.x), dpp_xor<MASK>(v.y)};
+#else
+    return v;
+#endif
+  }
+};
+
+// One radix-2 stage across lanes q and q ^ MASK (DIT form): the member with
+// (q & MASK) != 0 first multiplies by W_len^(k + off), then both exchange;
+// the low member keeps E + W*O, the high member E - W*O.
+template <int LEN, int SIGN, int MASK, int M, typename T, typename X>
+__host__ __device__ __forceinline__ void group_dit_stage(cpx<T> (&v)[M], int q, int off, const X& xchg) {
+  const bool hi = (q & MASK) != 0;
+  if (hi) {
 #pragma unroll
-    for (int k = 0; k < H; ++k) v[k] = twiddle_mul<L, SIGN>(v[k], k);
+    for (int k = 0; k < M; ++k) v[k] = twiddle_mul<LEN, SIGN>(v[k], k + off);
   }
 #pragma unroll
-  for (int k = 0; k < H; ++k) {
-    cpx<T> u = xchg(v[k], k);
-    v[k] = h ? csub(u, v[k]) : cadd(v[k], u);
+  for (int k = 0; k < M; ++k) {
+    cpx<T> u = xchg.template get<MASK>(v[k], k);
+    v[k] = hi ? csub(u, v[k]) : cadd(v[k], u);
   }
 }
 
-// Inverse-side split (decimation in frequency).  On entry lane h holds
-// x[h*H + k]; on exit it holds X[2m + h], m < H.
-template <int L, int SIGN, typename T, typename Xchg>
-__host__ __device__ __forceinline__ void fft_pair_dif(cpx<T> (&v)[L / 2], int h, Xchg xchg) {
-  constexpr int H = L / 2;
+// One radix-2 stage across lanes q and q ^ MASK (DIF form): exchange, the low
+// member keeps a + b, the high member (a - b) * W_len^(k + off).
+template <int LEN, int SIGN, int MASK, int M, typename T, typename X>
+__host__ __device__ __forceinline__ void group_dif_stage(cpx<T> (&v)[M], int q, int off, const X& xchg) {
+  const bool hi = (q & MASK) != 0;
 #pragma unroll
-  for (int k = 0; k < H; ++k) {
-    cpx<T> u = xchg(v[k], k);
-    v[k] = h ? csub(u, v[k]) : cadd(v[k], u);
+  for (int k = 0; k < M; ++k) {
+    cpx<T> u = xchg.template get<MASK>(v[k], k);
+    v[k] = hi ? csub(u, v[k]) : cadd(v[k], u);
   }
-  if (h) {
+  if (hi) {
 #pragma unroll
-    for (int k = 0; k < H; ++k) v[k] = twiddle_mul<L, SIGN>(v[k], k);
+    for (int k = 0; k < M; ++k) v[k] = twiddle_mul<LEN, SIGN>(v[k], k + off);
   }
-  fft_line<H, SIGN, T>(v);
+}
+
+// Forward transform over a group of P lanes (see above).
+template <int L, int P, int SIGN, typename T, typename X>
+__host__ __device__ __forceinline__ void fft_group_dit(cpx<T> (&v)[L / P], int q, const X& xchg) {
+  constexpr int M = L / P;
+  fft_line<M, SIGN, T>(v);
+  if constexpr (P == 2) {
+    group_dit_stage<L, SIGN, 1, M>(v, q, 0, xchg);
+  } else {
+    static_assert(P == 4, "groups of 2 or 4 lanes");
+    // sub-transforms of length L/2 over lanes {q, q^2}
+    group_dit_stage<L / 2, SIGN, 2, M>(v, q, 0, xchg);
+    // lanes q (even) and q^1 now hold E/O at chunk c = q >> 1 of length-L/2 halves
+    group_dit_stage<L, SIGN, 1, M>(v, q, (q >> 1) * M, xchg);
+  }
+}
+
+// Inverse-direction transform over a group of P lanes (see above).
+template <int L, int P, int SIGN, typename T, typename X>
+__host__ __device__ __forceinline__ void fft_group_dif(cpx<T> (&v)[L / P], int q, const X& xchg) {
+  constexpr int M = L / P;
+  if constexpr (P == 2) {
+    group_dif_stage<L, SIGN, 1, M>(v, q, 0, xchg);
+  } else {
+    static_assert(P == 4, "groups of 2 or 4 lanes");
+    group_dif_stage<L, SIGN, 1, M>(v, q, (q >> 1) * M, xchg);
+    group_dif_stage<L / 2, SIGN, 2, M>(v, q, 0, xchg);
+  }
+  fft_line<M, SIGN, T>(v);
 }
 
 }  // namespace mqpiv

@@ -28,6 +28,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "mqpiv_fft.hpp"
 #include "mqpiv_internal.h"
 
@@ -42,7 +44,8 @@ struct Cfg2 {
   static constexpr int NT = TPI * IPB;                    // threads per block
   static constexpr int LD = L + 1;                        // LDS row stride (complex)
   static constexpr int NW = (TPI + 63) / 64;              // waves per IA
-  static_assert(NW <= 4, "reduction slots hold 4 waves");
+  static_assert(NW <= 8, "reduction slots hold 8 waves");
+  static_assert(LPL == 1 || LPL == 2 || LPL == 4, "1, 2 or 4 lanes per line");
 };
 
 __device__ __forceinline__ double load_px(const void* img, int dt, int64_t i) {
@@ -148,151 +151,128 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t nb) {
   return x * q + (x < r ? x : r) + k;
 }
 
-// LDS layout of one IA: a work region (the L x L complex transform buffer, which
-// after the transform holds the similarity window followed by the FP64 box sums
-// for the denominators), then the first correlation of the masked measure.
+// LDS layout of one IA: the L x L complex transform buffer (which after the
+// transform holds the similarity window), then the first correlation of the
+// masked measure, then the reduction slots.
 __host__ __device__ inline size_t sim_bytes(int ns0, int ns1, int tsize) {
   return (((size_t)ns0 * ns1 * tsize + 15) & ~(size_t)15);
 }
 __host__ __device__ inline size_t region_bytes_2d(int L, int tsize, int ns0, int ns1, int ss0) {
-  size_t z = (size_t)L * (L + 1) * 2 * tsize;
-  size_t b = sim_bytes(ns0, ns1, tsize) + (size_t)ns1 * (ss0 + 1) * sizeof(double);
-  return ((z > b ? z : b) + 15) & ~(size_t)15;
+  (void)ns0;
+  (void)ns1;
+  (void)ss0;
+  return ((size_t)L * (L + 1) * 2 * tsize + 15) & ~(size_t)15;
 }
-constexpr int kRedBytes = 64;  // reduction slots per IA
+constexpr int kRedBytes = 128;  // reduction slots per IA
 
 // ---------------------------------------------------------------------------
-// Forward 2-D transform of the packed window in Z (L x L complex, LD stride),
-// the conjugate product, and the inverse transform down to the (ns0 x ns1)
-// fully-overlapping window of real correlation values, left (scaled by 1/L^2)
-// as T at the start of the Z region, column-major with ns0 fastest.
+// The correlation of one IA.  LPL lanes (a "group", q = lane within it) share a
+// line of L samples, M = L/LPL each.  On entry lane (line = k0, q) holds the
+// packed window Z = A + iB at k1 = LPL*k + q, k < M.  The forward transform runs
+// along dim 2 in registers, then along dim 1 through an LDS transpose; the
+// product P = conj(A^) B^ reads each partner bin back from LDS; the inverse runs
+// along dim 1 (keeping s0 < ns0) and dim 2 (keeping s1 < ns1).  On exit the
+// fully-overlapping window of real correlation values (scaled by 1/L^2) is at
+// the start of the Z region as T, column-major with ns0 fastest.
 template <int L, int LPL, typename T>
-__device__ __forceinline__ void correlate_2d(cpx<T>* __restrict__ Z, int t, int ns0, int ns1) {
+__device__ __forceinline__ void line_fwd(cpx<T> (&x)[L / LPL], int q) {
+  if constexpr (LPL == 1) fft_line<L, -1>(x);
+  else fft_group_dit<L, LPL, -1, T>(x, q, DppXchg{});
+}
+template <int L, int LPL, typename T>
+__device__ __forceinline__ void line_inv(cpx<T> (&x)[L / LPL], int q) {
+  if constexpr (LPL == 1) fft_line<L, 1>(x);
+  else fft_group_dif<L, LPL, 1, T>(x, q, DppXchg{});
+}
+
+template <int L, int LPL, typename T>
+__device__ __forceinline__ void correlate_2d(cpx<T> (&x)[L / LPL], cpx<T>* __restrict__ Z, int t, int ns0,
+                                             int ns1) {
   constexpr int LD = L + 1;
-  constexpr int H = L / LPL;
+  constexpr int M = L / LPL;
   const int line = t / LPL;
-  const int h = t % LPL;
-  auto xchg = [](cpx<T> v, int) { return shfl_pair(v); };
-  cpx<T> x[H];
+  const int q = t % LPL;
+  const int cM = (LPL == 1 ? 0 : chunk_of<LPL>(q)) * M;  // this lane's chunk after a forward pass
   const bool lane = line < L;
 
-  // --- forward along dim 1: the line at dim-2 index `line`
+  // --- forward along dim 2 (registers): line = k0
   if (lane) {
-    if constexpr (LPL == 1) {
+    line_fwd<L, LPL, T>(x, q);
 #pragma unroll
-      for (int i = 0; i < L; ++i) x[i] = Z[line * LD + i];
-      fft_line<L, -1>(x);
-    } else {
-#pragma unroll
-      for (int k = 0; k < H; ++k) x[k] = Z[line * LD + 2 * k + h];
-      fft_pair_dit<L, -1, T>(x, h, xchg);
-    }
-#pragma unroll
-    for (int k = 0; k < H; ++k) Z[line * LD + h * H + k] = x[k];
+    for (int k = 0; k < M; ++k) Z[(cM + k) * LD + line] = x[k];
   }
   __syncthreads();
-  // --- forward along dim 2: the line at dim-1 frequency k0 = `line`
+  // --- forward along dim 1: line = k1
   if (lane) {
-    if constexpr (LPL == 1) {
 #pragma unroll
-      for (int i = 0; i < L; ++i) x[i] = Z[i * LD + line];
-      fft_line<L, -1>(x);
-    } else {
+    for (int k = 0; k < M; ++k) x[k] = Z[line * LD + LPL * k + q];
+    line_fwd<L, LPL, T>(x, q);
 #pragma unroll
-      for (int k = 0; k < H; ++k) x[k] = Z[(2 * k + h) * LD + line];
-      fft_pair_dit<L, -1, T>(x, h, xchg);
-    }
-#pragma unroll
-    for (int k = 0; k < H; ++k) Z[(h * H + k) * LD + line] = x[k];
+    for (int k = 0; k < M; ++k) Z[line * LD + cM + k] = x[k];
   }
   __syncthreads();
   if (lane) {
-    // --- P(k) = conj(F^(k)) G^(k) with Z^ = F^ + i G^ (fft_ops.jl:95-105)
-    const int tn = (L - line) % L;
+    // --- P(k) = conj(A^(k)) B^(k) with Z^ = A^ + i B^ (fft_ops.jl:95-105)
+    const int k1n = (L - line) % L;
 #pragma unroll
-    for (int k = 0; k < H; ++k) {
-      const int k1 = h * H + k;
-      const int k1n = (L - k1) % L;
+    for (int k = 0; k < M; ++k) {
+      const int k0 = cM + k;
+      const int k0n = (L - k0) % L;
       cpx<T> z = x[k];
-      cpx<T> n = Z[k1n * LD + tn];
+      cpx<T> n = Z[k1n * LD + k0n];
       T ar = T(0.5) * (z.x + n.x), ai = T(0.5) * (z.y - n.y);
       T br = T(0.5) * (z.y + n.y), bi = T(0.5) * (n.x - z.x);
       x[k] = cpx<T>{ar * br + ai * bi, ar * bi - ai * br};
     }
-    // --- inverse along dim 2 (only outputs s1 < ns1 are kept)
-    if constexpr (LPL == 1) fft_line<L, 1>(x);
-    else fft_pair_dif<L, 1, T>(x, h, xchg);
+    // --- inverse along dim 1: lane q then holds s0 = LPL*m + q
+    line_inv<L, LPL, T>(x, q);
   }
   __syncthreads();
   if (lane) {
 #pragma unroll
-    for (int k = 0; k < H; ++k) {
-      const int s1 = LPL == 1 ? k : 2 * k + h;
-      if (s1 < ns1) Z[s1 * LD + line] = x[k];
+    for (int m = 0; m < M; ++m) {
+      const int s0 = LPL * m + q;
+      if (s0 < ns0) Z[line * LD + s0] = x[m];
     }
   }
   __syncthreads();
-  // --- inverse along dim 1 for the ns1 needed columns
-  const bool act = line < ns1;
+  // --- inverse along dim 2 for the ns0 needed rows: line = s0
+  const bool act = line < ns0;
   if (act) {
 #pragma unroll
-    for (int k = 0; k < H; ++k) x[k] = Z[line * LD + h * H + k];
-    if constexpr (LPL == 1) fft_line<L, 1>(x);
-    else fft_pair_dif<L, 1, T>(x, h, xchg);
+    for (int k = 0; k < M; ++k) x[k] = Z[(cM + k) * LD + line];
+    line_inv<L, LPL, T>(x, q);
   }
   __syncthreads();
   if (act) {
     T* S = reinterpret_cast<T*>(Z);
     const T kScale = T(1) / T(L * L);
 #pragma unroll
-    for (int k = 0; k < H; ++k) {
-      const int s0 = LPL == 1 ? k : 2 * k + h;
-      if (s0 < ns0) S[line * ns0 + s0] = x[k].x * kScale;
+    for (int m = 0; m < M; ++m) {
+      const int s1 = LPL * m + q;
+      if (s1 < ns1) S[s1 * ns0 + line] = x[m].x * kScale;
     }
   }
   __syncthreads();
 }
 
-// Value of the padded search buffer at (k0, k1) (0-based), zero outside the
-// clipped search region; ZNCC subtracts the search-region mean (zncc_ops.jl:22-24,
-// in the input element type, quirk Q12).
-template <int ALG>
-__device__ __forceinline__ double gval(const Job& J, const Geo& g0, const Geo& g1, int k0, int k1,
-                                       float meanG32, double meanG64) {
-  if (k0 < g0.goff || k0 >= g0.goff + g0.gsz || k1 < g1.goff || k1 >= g1.goff + g1.gsz) return 0.0;
-  int64_t i = (g0.sr + (k0 - g0.goff)) + J.dims[0] * (g1.sr + (k1 - g1.goff));
-  if constexpr (ALG == ALG_ZNCC) {
-    if (J.in_dtype == DT_F32) return (double)(((const float*)J.img2)[i] - meanG32);
-    return ((const double*)J.img2)[i] - meanG64;
-  } else {
-    return load_px(J.img2, J.in_dtype, i);
-  }
-}
-
-// Value of the interrogation window at (x0, x1): F, F - mean(F) (ZNCC) or F * m
-// (masked NSQECC, masked_nsqecc_ops.jl:19-25).
-template <int ALG>
-__device__ __forceinline__ double fval(const Job& J, const Geo& g0, const Geo& g1, int x0, int x1,
-                                       float meanF32, double meanF64) {
-  int64_t i = (g0.tlf + x0) + J.dims[0] * (g1.tlf + x1);
-  if constexpr (ALG == ALG_ZNCC) {
-    if (J.in_dtype == DT_F32) return (double)(((const float*)J.img1)[i] - meanF32);
-    return ((const double*)J.img1)[i] - meanF64;
-  } else if constexpr (ALG == ALG_MASK_NSQECC) {
-    return load_px(J.img1, J.in_dtype, i) * load_px(J.mask, J.in_dtype, i);
-  } else {
-    return load_px(J.img1, J.in_dtype, i);
-  }
+// Box sum of an FP64 integral array over image rows [a0, b0) x [a1, b1).
+__device__ __forceinline__ double sat_box(const double* __restrict__ S, int64_t D0, int64_t a0, int64_t b0,
+                                          int64_t a1, int64_t b1) {
+  return ((S[b0 + D0 * b1] - S[a0 + D0 * b1]) - S[b0 + D0 * a1]) + S[a0 + D0 * a1];
 }
 
 template <int L, int ALG, typename T, int LPL>
 __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT)) void piv2d_kernel(Job j0, const Job* __restrict__ jobs,
-                                                                    const int64_t* __restrict__ prefix,
-                                                                    int n_jobs, int64_t n_ia_total) {
+                                                                      const int64_t* __restrict__ prefix,
+                                                                      int n_jobs, int64_t n_ia_total) {
   using C = Cfg2<L, T, LPL>;
+  constexpr int H = C::H;
   extern __shared__ __align__(16) unsigned char smem[];
   const int t = threadIdx.x % C::TPI;
   const int sub = threadIdx.x / C::TPI;
+  const int line = t / LPL;
+  const int h = t % LPL;  // lane within the line's group
 
   const int64_t b = xcd_remap(blockIdx.x, gridDim.x);
   const int64_t ia_g = b * C::IPB + sub;
@@ -309,7 +289,7 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT)) void piv2d_kernel(Job j0, co
   const int sm0 = J.sm[0], sm1 = J.sm[1];
   const int is0 = J.is[0], is1 = J.is[1];
   const int ns0 = 2 * sm0 + 1, ns1 = 2 * sm1 + 1;
-  const int ss0 = is0 + 2 * sm0, ss1 = is1 + 2 * sm1;
+  const int ss0 = is0 + 2 * sm0;
 
   const size_t zbytes = region_bytes_2d(L, sizeof(T), ns0, ns1, ss0);
   const size_t c1bytes = (ALG == ALG_MASK_NSQECC) ? sim_bytes(ns0, ns1, sizeof(T)) : 0;
@@ -325,33 +305,39 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT)) void piv2d_kernel(Job j0, co
   const Geo g0 = geo_of(v0, is0, sm0, J.st[0], J.dims[0]);
   const Geo g1 = geo_of(v1, is1, sm1, J.st[1], J.dims[1]);
   const int dt = J.in_dtype;
+  const int64_t d0 = J.dims[0];
 
-  // ---- skip test (piv_ops.jl:134-157)
-  int skip = 0;
-  if (J.skip) {
-    skip = J.skip[ia] != 0;
-  } else if (J.has_threshold || J.has_mask_threshold) {
-    double m1 = -INFINITY, m2 = -INFINITY;
-    for (int e = t; e < is0 * is1; e += C::TPI) {
-      int64_t i = (g0.tlf + e % is0) + J.dims[0] * (g1.tlf + e / is0);
-      if (J.has_threshold) m1 = fmax(m1, load_px(J.img1, dt, i));
-      if (J.has_mask_threshold) m2 = fmax(m2, load_px(J.mask, dt, i));
-    }
-    m1 = Group<C::TPI>::reduce(m1, OpMax{}, red, t);
-    m2 = Group<C::TPI>::reduce(m2, OpMax{}, red, t);
-    if (J.has_threshold && m1 < J.threshold) skip = 1;
-    if (J.has_mask_threshold && m2 < J.mask_threshold) skip = 1;
-  }
+  // Lane (line = k0, q) covers k1 = LPL*k + q, k < H.  Loads use 32-bit offsets
+  // from per-lane base pointers (the host checks dims[0] * L < 2^31).
+  const int k0 = line;
+  const bool f_row = k0 < is0;
+  const bool g_row = k0 >= g0.goff && k0 < g0.goff + g0.gsz;
+  const int64_t f_base = (g0.tlf + k0) + d0 * g1.tlf;
+  const int64_t g_base = (g0.sr + (k0 - g0.goff)) + d0 * (g1.sr - g1.goff);
+  const uint32_t ud0 = (uint32_t)d0;
+  auto k1_of = [&](int k) { return LPL * k + h; };
+  auto f_at = [&](int k1) -> bool { return f_row && k1 < is1; };
+  auto g_at = [&](int k1) -> bool { return g_row && k1 >= g1.goff && k1 < g1.goff + g1.gsz; };
 
-  // ---- IA constants: means (ZNCC) and sum(F^2) in FP64
+  // ---- ZNCC means over the IA and the clipped search region (zncc_ops.jl:10,22;
+  // computed in the input element type, quirk Q12)
   float meanF32 = 0.f, meanG32 = 0.f;
   double meanF64 = 0.0, meanG64 = 0.0;
   if constexpr (ALG == ALG_ZNCC) {
     double sf = 0.0, sg = 0.0;
-    for (int e = t; e < is0 * is1; e += C::TPI)
-      sf += load_px(J.img1, dt, (g0.tlf + e % is0) + J.dims[0] * (g1.tlf + e / is0));
-    for (int e = t; e < g0.gsz * g1.gsz; e += C::TPI)
-      sg += load_px(J.img2, dt, (g0.sr + e % g0.gsz) + J.dims[0] * (g1.sr + e / g0.gsz));
+    auto sums = [&](auto tin) {
+      using TIN = decltype(tin);
+      const TIN* pf = (const TIN*)J.img1 + f_base;
+      const TIN* pg = (const TIN*)J.img2 + g_base;
+#pragma unroll
+      for (int k = 0; k < H; ++k) {
+        const int k1 = k1_of(k);
+        if (f_at(k1)) sf += (double)pf[ud0 * (uint32_t)k1];
+        if (g_at(k1)) sg += (double)pg[ud0 * (uint32_t)k1];
+      }
+    };
+    if (dt == DT_F32) sums(float{});
+    else sums(double{});
     sf = Group<C::TPI>::reduce(sf, OpAdd{}, red, t);
     sg = Group<C::TPI>::reduce(sg, OpAdd{}, red, t);
     const int64_t nF = (int64_t)is0 * is1;
@@ -361,109 +347,137 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT)) void piv2d_kernel(Job j0, co
     meanF64 = sf / (double)nF;
     meanG64 = sg / (double)nG;
   }
-  double sumF2 = 0.0;
-  if constexpr (ALG != ALG_FFTCC) {
-    for (int e = t; e < is0 * is1; e += C::TPI) {
-      const int x0 = e % is0, x1 = e / is0;
-      if constexpr (ALG == ALG_MASK_NSQECC) {
-        int64_t i = (g0.tlf + x0) + J.dims[0] * (g1.tlf + x1);
-        double m = load_px(J.mask, dt, i);
-        double fm = load_px(J.img1, dt, i) * m;
-        double wi = m > 0 ? m : 1.0;
-        sumF2 += fm * fm / wi;
-      } else {
-        double f = fval<ALG>(J, g0, g1, x0, x1, meanF32, meanF64);
-        sumF2 += f * f;
+
+  // ---- load the packed window into registers; sum(F^2) and the IA maxima on the way
+  cpx<T> x[H];
+  double sumF2 = 0.0, mx1 = -INFINITY, mx2 = -INFINITY;
+  const bool want_max = J.has_threshold || J.has_mask_threshold;
+  auto load = [&](auto tin) {
+    using TIN = decltype(tin);
+    const TIN* pf = (const TIN*)J.img1 + f_base;
+    const TIN* pm = (const TIN*)J.mask + f_base;
+    const TIN* pg = (const TIN*)J.img2 + g_base;
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+      const int k1 = k1_of(k);
+      const uint32_t o = ud0 * (uint32_t)k1;
+      double a = 0.0, g = 0.0;
+      if (f_at(k1)) {
+        const TIN f = pf[o];
+        if (want_max) mx1 = fmax(mx1, (double)f);
+        if constexpr (ALG == ALG_ZNCC) {
+          a = std::is_same<TIN, float>::value ? (double)((float)f - meanF32) : (double)f - meanF64;
+          sumF2 += a * a;
+        } else if constexpr (ALG == ALG_MASK_NSQECC) {
+          const double m = (double)pm[o];
+          if (want_max) mx2 = fmax(mx2, m);
+          a = (double)f * m;  // masked_nsqecc_ops.jl:19-25
+          const double wi = m > 0 ? m : 1.0;
+          sumF2 += a * a / wi;  // mask_nsqecc.jl:144-149
+        } else {
+          a = (double)f;
+          if constexpr (ALG != ALG_FFTCC) sumF2 += a * a;
+        }
       }
+      if (g_at(k1)) {
+        const TIN gv = pg[o];
+        if constexpr (ALG == ALG_ZNCC)
+          g = std::is_same<TIN, float>::value ? (double)((float)gv - meanG32) : (double)gv - meanG64;
+        else
+          g = (double)gv;
+      }
+      x[k] = cpx<T>{(T)a, (T)g};
     }
-    sumF2 = Group<C::TPI>::reduce(sumF2, OpAdd{}, red, t);
+  };
+  if (dt == DT_F32) load(float{});
+  else load(double{});
+  if constexpr (ALG != ALG_FFTCC) sumF2 = Group<C::TPI>::reduce(sumF2, OpAdd{}, red, t);
+
+  // ---- skip test (piv_ops.jl:134-157)
+  int skip = 0;
+  if (J.skip) {
+    skip = J.skip[ia] != 0;
+  } else if (want_max) {
+    // (mx2, the mask maximum, is only gathered by the masked measure, which is
+    // the only one the host gives a mask threshold)
+    mx1 = Group<C::TPI>::reduce(mx1, OpMax{}, red, t);
+    mx2 = Group<C::TPI>::reduce(mx2, OpMax{}, red, t);
+    if (J.has_threshold && mx1 < J.threshold) skip = 1;
+    if (J.has_mask_threshold && mx2 < J.mask_threshold) skip = 1;
   }
 
   // ---- correlation(s)
-  constexpr int NPASS = (ALG == ALG_MASK_NSQECC) ? 2 : 1;
-#pragma unroll 1
-  for (int pass = 0; pass < NPASS; ++pass) {
-    // pack Z = A + iB (zero padded to L x L); consecutive lanes take
-    // consecutive dim-1 indices, so global reads are coalesced
-#pragma unroll 2
-    for (int e = t; e < L * L; e += C::TPI) {
-      const int k0 = e % L, k1 = e / L;
-      T a = T(0), bb = T(0);
-      if (k0 < is0 && k1 < is1) {
-        if (ALG == ALG_MASK_NSQECC && pass == 1)
-          a = (T)load_px(J.mask, dt, (g0.tlf + k0) + J.dims[0] * (g1.tlf + k1));
-        else
-          a = (T)fval<ALG>(J, g0, g1, k0, k1, meanF32, meanF64);
+  correlate_2d<L, LPL, T>(x, Z, t, ns0, ns1);
+  if constexpr (ALG == ALG_MASK_NSQECC) {
+    for (int i = t; i < ns0 * ns1; i += C::TPI) C1[i] = S[i];
+    // second correlation: mask (x) G^2 (mask_nsqecc.jl:155)
+    auto load2 = [&](auto tin) {
+      using TIN = decltype(tin);
+      const TIN* pm = (const TIN*)J.mask + f_base;
+      const TIN* pg = (const TIN*)J.img2 + g_base;
+#pragma unroll
+      for (int k = 0; k < H; ++k) {
+        const int k1 = k1_of(k);
+        const uint32_t o = ud0 * (uint32_t)k1;
+        double m = 0.0, g2 = 0.0;
+        if (f_at(k1)) m = (double)pm[o];
+        if (g_at(k1)) {
+          const double g = (double)pg[o];
+          g2 = g * g;
+        }
+        x[k] = cpx<T>{(T)m, (T)g2};
       }
-      if (k0 < ss0 && k1 < ss1) {
-        double g = gval<ALG>(J, g0, g1, k0, k1, meanG32, meanG64);
-        bb = (ALG == ALG_MASK_NSQECC && pass == 1) ? (T)(g * g) : (T)g;
-      }
-      Z[k1 * C::LD + k0] = cpx<T>{a, bb};
-    }
+    };
+    if (dt == DT_F32) load2(float{});
+    else load2(double{});
     __syncthreads();
-    correlate_2d<L, LPL, T>(Z, t, ns0, ns1);
-    if (ALG == ALG_MASK_NSQECC && pass == 0) {
-      for (int i = t; i < ns0 * ns1; i += C::TPI) C1[i] = S[i];
-      __syncthreads();
-    }
+    correlate_2d<L, LPL, T>(x, Z, t, ns0, ns1);
   }
 
   // ---- normalisation on the fully-overlapping window
   if constexpr (ALG == ALG_ZNCC || ALG == ALG_NSQECC) {
-    // FP64 box sums of G^2 (the window sums of common_ops.jl:203-245):
-    // A(k0, s1) = sum_{j1 in [s1, s1+is1)} g(k0, j1)^2 by lanes along dim 1
-    // (coalesced), then W(s0, s1) = sum_{k0 in [s0, s0+is0)} A(k0, s1) by lanes
-    // along dim 2.
-    double* A = reinterpret_cast<double*>(base + sim_bytes(ns0, ns1, sizeof(T)));
-    const int lda = ss0 + 1;
-    if (t < ss0) {
-      double acc = 0.0;
-      for (int j1 = 0; j1 < is1; ++j1) {
-        double g = gval<ALG>(J, g0, g1, t, j1, meanG32, meanG64);
-        acc += g * g;
-      }
-      for (int s1 = 0; s1 < ns1; ++s1) {
-        A[s1 * lda + t] = acc;
-        if (s1 + 1 < ns1) {
-          double gin = gval<ALG>(J, g0, g1, t, s1 + is1, meanG32, meanG64);
-          double gout = gval<ALG>(J, g0, g1, t, s1, meanG32, meanG64);
-          acc += gin * gin;
-          acc -= gout * gout;
-        }
-      }
-    }
-    __syncthreads();
-    if (t < ns1) {
-      const double sqF = sqrt(sumF2);
-      double acc = 0.0;
-      for (int k0 = 0; k0 < is0; ++k0) acc += A[t * lda + k0];
-      for (int s0 = 0; s0 < ns0; ++s0) {
-        const double W = acc;
-        double c = (double)S[t * ns0 + s0];
-        double den = sqrt(W) * sqF;
-        double r;
+    // W(s) = sum of G'^2 over the search-buffer window [s, s + is) (the SAT
+    // window sums of common_ops.jl:203-245), from the image's FP64 integral
+    // arrays after clipping the window to the part that holds image data.
+    const int64_t D0 = d0 + 1;
+    const double sqF = sqrt(sumF2);
+    const double mu = (dt == DT_F32) ? (double)meanG32 : meanG64;
+    for (int e = t; e < ns0 * ns1; e += C::TPI) {
+      const int s0 = e % ns0, s1 = e / ns0;
+      const int lo0 = s0 > g0.goff ? s0 : g0.goff;
+      const int hi0 = (s0 + is0) < (g0.goff + g0.gsz) ? (s0 + is0) : (g0.goff + g0.gsz);
+      const int lo1 = s1 > g1.goff ? s1 : g1.goff;
+      const int hi1 = (s1 + is1) < (g1.goff + g1.gsz) ? (s1 + is1) : (g1.goff + g1.gsz);
+      double W = 0.0;
+      if (hi0 > lo0 && hi1 > lo1) {
+        const int64_t a0 = g0.sr + (lo0 - g0.goff), b0 = g0.sr + (hi0 - g0.goff);
+        const int64_t a1 = g1.sr + (lo1 - g1.goff), b1 = g1.sr + (hi1 - g1.goff);
+        W = sat_box(J.sat2, D0, a0, b0, a1, b1);
         if constexpr (ALG == ALG_ZNCC) {
-          r = c / den;
-        } else {
-          double num = (sumF2 - 2.0 * c) + W;
-          r = den / (num + den);
+          const double n = (double)((b0 - a0) * (b1 - a1));
+          W = (W - 2.0 * mu * sat_box(J.sat1, D0, a0, b0, a1, b1)) + n * mu * mu;
         }
-        S[t * ns0 + s0] = (T)r;
-        if (s0 + 1 < ns0) {
-          acc += A[t * lda + s0 + is0];
-          acc -= A[t * lda + s0];
-        }
+        W = W > 0.0 ? W : 0.0;
       }
+      const double c = (double)S[e];
+      const double den = sqrt(W) * sqF;
+      double r;
+      if constexpr (ALG == ALG_ZNCC) {
+        r = c / den;  // zncc_ops.jl:39-48
+      } else {
+        const double num = (sumF2 - 2.0 * c) + W;  // nsqecc_ops.jl:58-60
+        r = den / (num + den);                     // nsqecc_ops.jl:72-73
+      }
+      S[e] = (T)r;
     }
     __syncthreads();
   } else if constexpr (ALG == ALG_MASK_NSQECC) {
     const double sqF = sqrt(sumF2);
     for (int i = t; i < ns0 * ns1; i += C::TPI) {
-      double c1 = (double)C1[i];
-      double c2 = (double)S[i];
-      double num = (-2.0 * c1 + sumF2) + c2;
-      double den = sqrt(fabs(c2)) * sqF;
+      const double c1 = (double)C1[i];
+      const double c2 = (double)S[i];
+      const double num = (-2.0 * c1 + sumF2) + c2;  // masked_nsqecc_ops.jl:130-132
+      const double den = sqrt(fabs(c2)) * sqF;      // masked_nsqecc_ops.jl:135-136
       S[i] = (T)(den / (num + den));
     }
     __syncthreads();
@@ -476,9 +490,9 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT)) void piv2d_kernel(Job j0, co
   const int w0 = hi0 - lo0 + 1, w1 = hi1 - lo1 + 1;
   const int nwin = (w0 > 0 && w1 > 0) ? w0 * w1 : 0;
   for (int e = t; e < nwin; e += C::TPI) {
-    int p0 = lo0 + e % w0, p1 = lo1 + e / w0;
-    int li = p0 + ns0 * p1;
-    T v = S[li];
+    const int p0 = lo0 + e % w0, p1 = lo1 + e / w0;
+    const int li = p0 + ns0 * p1;
+    const T v = S[li];
     if (v > bst.v) bst = Best<T>{v, li};
   }
   {
@@ -490,7 +504,7 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT)) void piv2d_kernel(Job j0, co
     if constexpr (C::TPI > 64) {
       __syncthreads();
       T* rv = reinterpret_cast<T*>(red);
-      int32_t* ri = reinterpret_cast<int32_t*>(red + 4);
+      int32_t* ri = reinterpret_cast<int32_t*>(red + 8);
       if ((t & 63) == 0) {
         rv[t >> 6] = bst.v;
         ri[t >> 6] = bst.i;
@@ -510,8 +524,8 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT)) void piv2d_kernel(Job j0, co
     const int nsn = (u0 > 0 && u1 > 0) ? u0 * u1 : 0;
     double mx = -INFINITY, sm = 0.0;
     for (int e = t; e < nsn; e += C::TPI) {
-      int p0 = lo0 + e % u0, p1 = lo1 + e / u0;
-      double v = (double)S[p0 + ns0 * p1];
+      const int p0 = lo0 + e % u0, p1 = lo1 + e / u0;
+      const double v = (double)S[p0 + ns0 * p1];
       mx = (v > mx || v != v) ? v : mx;
       sm += v;
     }
@@ -522,24 +536,24 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT)) void piv2d_kernel(Job j0, co
 
   // ---- Gaussian subpixel + write (piv_ops.jl:249-326, 236-243)
   if (valid && t == 0) {
-    double d0, d1;
+    double r0, r1;
     if (skip) {
-      d0 = d1 = 0.0;
+      r0 = r1 = 0.0;
       sn = 0.0;
     } else if (bst.i == INT32_MAX) {
-      d0 = -(double)(sm0 + 1);
-      d1 = -(double)(sm1 + 1);
+      r0 = -(double)(sm0 + 1);
+      r1 = -(double)(sm1 + 1);
     } else {
       const int p0 = bst.i % ns0, p1 = bst.i / ns0;  // 0-based; reference peak = p + 1
       const bool interior = (p0 > lo0) && (p0 < hi0) && (p1 > lo1) && (p1 < hi1);
-      d0 = (double)(p0 + 1) - (double)(sm0 + 1);
-      d1 = (double)(p1 + 1) - (double)(sm1 + 1);
+      r0 = (double)(p0 + 1) - (double)(sm0 + 1);
+      r1 = (double)(p1 + 1) - (double)(sm1 + 1);
       if (interior) {
         double mn = INFINITY;
         bool nan = false;
         for (int b1 = -1; b1 <= 1; ++b1)
           for (int b0 = -1; b0 <= 1; ++b0) {
-            double v = (double)S[(p0 + b0) + ns0 * (p1 + b1)];
+            const double v = (double)S[(p0 + b0) + ns0 * (p1 + b1)];
             if (v != v) nan = true;
             mn = v < mn ? v : mn;
           }
@@ -550,10 +564,10 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT)) void piv2d_kernel(Job j0, co
         const double dn = log(1.0 + (double)S[(p0 + 1) + ns0 * p1] - mn);
         const double lf = log(1.0 + (double)S[p0 + ns0 * (p1 - 1)] - mn);
         const double rt = log(1.0 + (double)S[p0 + ns0 * (p1 + 1)] - mn);
-        const double g0v = (up - dn) / (2 * up - 4 * lmx + 2 * dn);
-        const double g1v = (lf - rt) / (2 * lf - 4 * lmx + 2 * rt);
-        d0 = ((double)(p0 + 1) + g0v) - (double)(sm0 + 1);
-        d1 = ((double)(p1 + 1) + g1v) - (double)(sm1 + 1);
+        const double q0 = (up - dn) / (2 * up - 4 * lmx + 2 * dn);
+        const double q1 = (lf - rt) / (2 * lf - 4 * lmx + 2 * rt);
+        r0 = ((double)(p0 + 1) + q0) - (double)(sm0 + 1);
+        r1 = ((double)(p1 + 1) + q1) - (double)(sm1 + 1);
       }
     }
     if (J.peaks) {
@@ -563,13 +577,13 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT)) void piv2d_kernel(Job j0, co
     }
     if (J.out_dtype == DT_F32) {
       float* vf = (float*)J.vf;
-      vf[2 * ia + 0] = (float)d0;
-      vf[2 * ia + 1] = (float)d1;
+      vf[2 * ia + 0] = (float)r0;
+      vf[2 * ia + 1] = (float)r1;
       if (J.sn) ((float*)J.sn)[ia] = (float)sn;
     } else {
       double* vf = (double*)J.vf;
-      vf[2 * ia + 0] = d0;
-      vf[2 * ia + 1] = d1;
+      vf[2 * ia + 0] = r0;
+      vf[2 * ia + 1] = r1;
       if (J.sn) ((double*)J.sn)[ia] = sn;
     }
   }

@@ -1,0 +1,137 @@
+// mqpiv_sat.hip -- FP64 integral arrays (summed-area tables) of an image or
+// volume, built with on-device prefix scans.
+//
+// The reference builds, per IA, the squared integral array of the padded search
+// window (src/utils/common_ops.jl:142-180) and reads window sums from it by
+// inclusion-exclusion (common_ops.jl:203-245).  Here the tables are built once
+// per image: S(j0, j1[, j2]) = sum of v over [0, j0) x [0, j1) [x [0, j2)], with
+// v = pixel (power 1) or pixel^2 (power 2), so that any window sum -- including
+// the reference's zero-padded, quirk-shifted search windows, after clipping to
+// the image -- is 4 (2-D) or 8 (3-D) lookups.  Layout: (d0+1) x (d1+1) [x (d2+1)],
+// dim 0 fastest, FP64 throughout.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mqpiv_internal.h"
+
+namespace mqpiv {
+
+// Dim-0 pass: one wave per output line (j1, j2); an inclusive wave scan over
+// chunks of 64 pixels with a carried total.  Lines with j1 = 0 or j2 = 0 and the
+// j0 = 0 entries are the zero border.
+__global__ __launch_bounds__(64) void sat_dim0_kernel(const void* __restrict__ img, int dt, int power,
+                                                      int64_t d0, int64_t d1, int64_t d2, int nd,
+                                                      double* __restrict__ S) {
+  const int64_t D0 = d0 + 1, D1 = d1 + 1;
+  const int64_t line = blockIdx.x;
+  const int64_t j1 = line % D1;
+  const int64_t j2 = line / D1;
+  double* out = S + line * D0;
+  const int lane = threadIdx.x;
+  const bool zero = (j1 == 0) || (nd == 3 && j2 == 0);
+  if (lane == 0) out[0] = 0.0;
+  const int64_t src = (nd == 3 ? (j2 - 1) * d1 : 0) * d0 + (j1 - 1) * d0;
+  double carry = 0.0;
+  for (int64_t c = 0; c < d0; c += 64) {
+    const int64_t i0 = c + lane;
+    double v = 0.0;
+    if (!zero && i0 < d0) {
+      v = dt == DT_F32 ? (double)((const float*)img)[src + i0] : ((const double*)img)[src + i0];
+      if (power == 2) v = v * v;
+    }
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) {
+      double u = __shfl_up(v, m);
+      if (lane >= m) v += u;
+    }
+    v += carry;
+    if (i0 < d0) out[i0 + 1] = v;
+    carry = __shfl(v, 63);
+  }
+}
+
+// Strided pass along dim d (1 or 2), chunked: every thread scans `chunk`
+// consecutive entries of one column in place and records the chunk total.
+__global__ __launch_bounds__(256) void sat_chunk_scan_kernel(double* __restrict__ S, int64_t n_pos,
+                                                             int64_t len, int64_t stride, int64_t pos_mod,
+                                                             int64_t pos_mul, int chunk, int nch,
+                                                             double* __restrict__ tot) {
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t p = gid % n_pos;
+  const int64_t c = gid / n_pos;
+  if (c >= nch) return;
+  // positions enumerate every index whose dim-d coordinate is 0
+  const int64_t base = (p % pos_mod) + (p / pos_mod) * pos_mul;
+  const int64_t j0 = c * chunk;
+  const int64_t j1 = j0 + chunk < len ? j0 + chunk : len;
+  double acc = 0.0;
+  for (int64_t j = j0; j < j1; ++j) {
+    acc += S[base + j * stride];
+    S[base + j * stride] = acc;
+  }
+  tot[p * nch + c] = acc;
+}
+
+// Adds the running total of the preceding chunks to every entry of a chunk.
+__global__ __launch_bounds__(256) void sat_chunk_fix_kernel(double* __restrict__ S, int64_t n_pos, int64_t len,
+                                                            int64_t stride, int64_t pos_mod, int64_t pos_mul,
+                                                            int chunk, int nch, const double* __restrict__ tot) {
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t p = gid % n_pos;
+  const int64_t c = gid / n_pos + 1;
+  if (c >= nch) return;
+  double off = 0.0;
+  for (int k = 0; k < c; ++k) off += tot[p * nch + k];
+  const int64_t base = (p % pos_mod) + (p / pos_mod) * pos_mul;
+  const int64_t j0 = c * chunk;
+  const int64_t j1 = j0 + chunk < len ? j0 + chunk : len;
+  for (int64_t j = j0; j < j1; ++j) S[base + j * stride] += off;
+}
+
+size_t sat_bytes(const int64_t* dims, int nd) {
+  size_t n = (size_t)(dims[0] + 1) * (dims[1] + 1) * (nd == 3 ? dims[2] + 1 : 1);
+  return n * sizeof(double);
+}
+
+size_t sat_tmp_bytes(const int64_t* dims, int nd) {
+  const int chunk = 64;
+  size_t worst = 0;
+  for (int d = 1; d < nd; ++d) {
+    int64_t len = dims[d] + 1;
+    int64_t nch = (len + chunk - 1) / chunk;
+    int64_t npos = (dims[0] + 1) * (nd == 3 ? (d == 1 ? dims[2] + 1 : dims[1] + 1) : 1);
+    size_t b = (size_t)npos * nch * sizeof(double);
+    if (b > worst) worst = b;
+  }
+  return worst;
+}
+
+// Builds S for `img` (d0 x d1 [x d2], element type dt) with v^power.
+hipError_t build_sat(const void* img, int dt, int power, const int64_t* dims, int nd, double* S, double* tmp,
+                     hipStream_t s) {
+  const int64_t d0 = dims[0], d1 = dims[1], d2 = nd == 3 ? dims[2] : 1;
+  const int64_t D0 = d0 + 1, D1 = d1 + 1, D2 = nd == 3 ? d2 + 1 : 1;
+  hipLaunchKernelGGL(sat_dim0_kernel, dim3((unsigned)(D1 * D2)), dim3(64), 0, s, img, dt, power, d0, d1, d2, nd,
+                     S);
+  const int chunk = 64;
+  for (int d = 1; d < nd; ++d) {
+    const int64_t len = d == 1 ? D1 : D2;
+    const int64_t stride = d == 1 ? D0 : D0 * D1;
+    // positions: d = 1 -> (j0, j2) with base j0 + j2*D0*D1; d = 2 -> (j0, j1) with base j0 + j1*D0
+    const int64_t pos_mod = D0;
+    const int64_t pos_mul = d == 1 ? D0 * D1 : D0;
+    const int64_t n_pos = d == 1 ? D0 * D2 : D0 * D1;
+    const int nch = (int)((len + chunk - 1) / chunk);
+    const int64_t n1 = n_pos * nch;
+    hipLaunchKernelGGL(sat_chunk_scan_kernel, dim3((unsigned)((n1 + 255) / 256)), dim3(256), 0, s, S, n_pos, len,
+                       stride, pos_mod, pos_mul, chunk, nch, tmp);
+    if (nch > 1) {
+      const int64_t n2 = n_pos * (nch - 1);
+      hipLaunchKernelGGL(sat_chunk_fix_kernel, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, s, S, n_pos,
+                         len, stride, pos_mod, pos_mul, chunk, nch, tmp);
+    }
+  }
+  return hipGetLastError();
+}
+
+}  // namespace mqpiv
