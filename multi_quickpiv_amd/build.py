@@ -46,6 +46,14 @@ SOURCES = [
     "mqpiv_k2d_f64_32.hip",
     "mqpiv_k2d_f32_16.hip",
     "mqpiv_k2d_f64_16.hip",
+    "mqpiv_k3d_f32_64.hip",
+    "mqpiv_k3d_f64_64.hip",
+    "mqpiv_k3d_f32_48.hip",
+    "mqpiv_k3d_f64_48.hip",
+    "mqpiv_k3d_f32_32.hip",
+    "mqpiv_k3d_f64_32.hip",
+    "mqpiv_k3d_f32_16.hip",
+    "mqpiv_k3d_f64_16.hip",
     "mqpiv_kernels2d.hip",
     "mqpiv_kernels3d.hip",
     "mqpiv_sat.hip",

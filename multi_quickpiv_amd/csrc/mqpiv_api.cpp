@@ -27,9 +27,9 @@ size_t sat_bytes(const int64_t* dims, int nd);
 size_t sat_tmp_bytes(const int64_t* dims, int nd);
 hipError_t build_sat(const void* img, int dt, int power, const int64_t* dims, int nd, double* S, double* tmp,
                      hipStream_t s);
-int fft_len_3d(int ss);
-hipError_t launch_piv3d(const Job& j, void* scratch, size_t scratch_bytes, hipStream_t s);
-size_t scratch_3d_bytes(const Job& j, int64_t chunk);
+int fft_len_3d(int fp64, int ss);
+size_t scratch_3d_bytes(int fp64, int L, const Job& j);
+hipError_t launch_piv3d(int fp64, int L, const Job& j, void* scratch, const double* sat1_img1, hipStream_t s);
 }  // namespace mqpiv
 
 using namespace mqpiv;
@@ -234,21 +234,25 @@ static int build_job(mqpiv_ctx* c, const mqpiv_params* p, const int64_t* dims, i
 }
 
 // Integral arrays the normalisation reads: NSQECC needs sum(G^2), ZNCC also sum(G).
+// (3-D ZNCC also reads the means from integral arrays: sum(F) of image 1.)
 static int prepare_sats(mqpiv_ctx* c, Job& j, hipStream_t s) {
-  j.sat1 = j.sat2 = nullptr;
+  j.sat1 = j.sat2 = j.sat1f = nullptr;
   if (j.alg != ALG_ZNCC && j.alg != ALG_NSQECC) return MQPIV_OK;
   const size_t one = (sat_bytes(j.dims, j.ndims) + 255) & ~(size_t)255;
   const size_t tmp = (sat_tmp_bytes(j.dims, j.ndims) + 255) & ~(size_t)255;
-  const int nsat = j.alg == ALG_ZNCC ? 2 : 1;
+  const int nsat = j.alg == ALG_ZNCC ? (j.ndims == 3 ? 3 : 2) : 1;
   int rc = ensure(c, &c->d_sat, &c->sat_cap, nsat * one + tmp, s);
   if (rc) return rc;
   char* b = (char*)c->d_sat;
   double* t = (double*)(b + nsat * one);
   hipError_t e = build_sat(j.img2, j.in_dtype, 2, j.dims, j.ndims, (double*)b, t, s);
-  if (e == hipSuccess && nsat == 2) e = build_sat(j.img2, j.in_dtype, 1, j.dims, j.ndims, (double*)(b + one), t, s);
+  if (e == hipSuccess && nsat >= 2) e = build_sat(j.img2, j.in_dtype, 1, j.dims, j.ndims, (double*)(b + one), t, s);
+  if (e == hipSuccess && nsat >= 3)
+    e = build_sat(j.img1, j.in_dtype, 1, j.dims, j.ndims, (double*)(b + 2 * one), t, s);
   if (e != hipSuccess) return fail(c, MQPIV_ERR_DEVICE, "integral array build failed: %s", hipGetErrorString(e));
   j.sat2 = (const double*)b;
-  j.sat1 = nsat == 2 ? (const double*)(b + one) : nullptr;
+  j.sat1 = nsat >= 2 ? (const double*)(b + one) : nullptr;
+  j.sat1f = nsat >= 3 ? (const double*)(b + 2 * one) : nullptr;
   return MQPIV_OK;
 }
 
@@ -284,21 +288,21 @@ static int launch_job(mqpiv_ctx* c, Job& j, hipStream_t s) {
     if (e != hipSuccess) return fail(c, MQPIV_ERR_DEVICE, "2-D launch failed: %s", hipGetErrorString(e));
     return MQPIV_OK;
   }
-  int ss = std::max({j.is[0] + 2 * j.sm[0], j.is[1] + 2 * j.sm[1], j.is[2] + 2 * j.sm[2]});
-  if (fft_len_3d(ss) < 0)
-    return fail(c, MQPIV_ERR_UNSUPPORTED, "search size %d exceeds the largest 3-D transform", ss);
-  size_t need = scratch_3d_bytes(j, 0);
-  if (need > c->scratch_bytes) {
-    if (c->d_scratch) {
-      hipStreamSynchronize(s);
-      hipFree(c->d_scratch);
-      c->d_scratch = nullptr;
-      c->scratch_bytes = 0;
-    }
-    HIP_OR_FAIL(c, hipMalloc(&c->d_scratch, need));
-    c->scratch_bytes = need;
+  // 3-D
+  if (j.alg == ALG_MASK_NSQECC) return fail(c, MQPIV_ERR_UNSUPPORTED, "masked NSQECC is built for 2-D only");
+  const int ss = std::max({j.is[0] + 2 * j.sm[0], j.is[1] + 2 * j.sm[1], j.is[2] + 2 * j.sm[2]});
+  int fp64 = 0;
+  if (j.precision_mode == MQPIV_FP64) fp64 = 1;
+  else if (j.precision_mode == MQPIV_AUTO) fp64 = (j.alg == ALG_ZNCC && j.compute_sn) ? 1 : 0;
+  const int L = fft_len_3d(fp64, ss);
+  if (L < 0) return fail(c, MQPIV_ERR_UNSUPPORTED, "3-D search size %d exceeds the largest transform (64)", ss);
+  int rc = ensure(c, &c->d_scratch, &c->scratch_bytes, scratch_3d_bytes(fp64, L, j), s);
+  if (rc) return rc;
+  hipError_t e;
+  {
+    KernelTimer kt(c, s);
+    e = launch_piv3d(fp64, L, j, c->d_scratch, j.sat1f, s);
   }
-  hipError_t e = launch_piv3d(j, c->d_scratch, c->scratch_bytes, s);
   if (e != hipSuccess) return fail(c, MQPIV_ERR_DEVICE, "3-D launch failed: %s", hipGetErrorString(e));
   return MQPIV_OK;
 }

@@ -20,6 +20,7 @@ struct Job {
   int32_t* peaks;          // optional: N int32 per IA, 1-based integer peak (0 = none)
   const double* sat1;      // FP64 integral array of img2 (sum of pixels), or nullptr
   const double* sat2;      // FP64 integral array of img2^2, or nullptr
+  const double* sat1f;     // FP64 integral array of img1 (3-D ZNCC means), or nullptr
   int64_t dims[3];
   int32_t is[3], sm[3], st[3];
   int32_t vf_n[3];         // vector-field size per dim

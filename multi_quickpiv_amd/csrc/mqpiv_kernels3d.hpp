@@ -1,0 +1,552 @@
+// mqpiv_kernels3d.hpp -- 3-D cross-correlation PIV on gfx950.
+//
+// A 3-D window (L^3 complex, 884 KB at L = 48 in FP32) does not fit in LDS, so
+// each chunk of IAs runs four kernels through a device scratch buffer sized to
+// stay in the last-level cache:
+//   K1 slab_fwd  one block per (IA, z-slab i2): pack Z = A + iB for the slab,
+//                forward 2-D transform (dims 1, 2) in registers + LDS, store
+//                the slab; per-slab partial sum(F^2) and IA maxima.
+//   K2 zline     one block per (IA, pair of planes k1 / -k1): transform along
+//                dim 3, P = conj(A^) B^ with the partner bin from the paired
+//                plane, inverse along dim 3 keeping s2 < ns2.
+//   K3 slab_inv  one block per (IA, s2): inverse 2-D transform down to the
+//                (ns0 x ns1) window of real correlation values.
+//   K4 finalize  one block per IA: FP64 window sums from the 3-D integral
+//                arrays, ZNCC/NSQECC normalisation, argmax, 3-D Gaussian
+//                subpixel, SN, VF/SN in the reference layout.
+//
+// Reference behaviour restated (files relative to the reference repo root):
+//   coordinates/offsets  src/utils/piv_ops.jl:38-85 (incl. SR_TLF_off, :72)
+//   correlation          src/crosscorrelation_algorithms/fftcc.jl:101-126
+//   ZNCC / NSQECC        zncc_ops.jl:3-51, nsqecc_ops.jl:1-26
+//   3-D window sums      src/utils/common_ops.jl:164-180, 223-245
+//   peak                 src/utils/common_ops.jl:98-124 (z, c, r loop order)
+//   subpixel             src/utils/piv_ops.jl:282-326 (gaussian_3D)
+//   SN                   src/utils/piv_ops.jl:333-367; the reference's 3-D form
+//                        throws (4 index ranges on a 3-D array), so 3-D SN uses
+//                        the natural extension [1+o-, 2SM-o+] on every dim.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <type_traits>
+
+#include "mqpiv_kernels2d.hpp"
+
+namespace mqpiv {
+
+template <int L, typename T, int LPL>
+struct Cfg3 {
+  static constexpr int M = L / LPL;
+  static constexpr int LD = L + 1;
+  static constexpr int TPL = ((L * LPL + 63) / 64) * 64;  // threads for one plane of lines
+  static constexpr int NT2 = ((2 * L * LPL + 63) / 64) * 64;  // K2: two planes
+};
+
+// Per-chunk scratch layout (all per IA of the chunk):
+//   W    : L^3 complex T, [i2|k2|s2][k1][k0]
+//   S    : ns0*ns1*ns2 T  (correlation window, column-major)
+//   part : L doubles x 3 (sum F^2, max img1, max mask) per slab
+struct Scratch3 {
+  void* w;
+  void* s;
+  double* part;
+  int64_t w_stride;  // elements of cpx<T> per IA
+  int64_t s_stride;  // elements of T per IA
+};
+
+struct Geo3 {
+  Geo g[3];
+};
+
+__device__ __forceinline__ Geo3 geo3_of(const Job& J, int64_t ia) {
+  Geo3 r;
+  int64_t v0 = ia % J.vf_n[0];
+  int64_t rest = ia / J.vf_n[0];
+  int64_t v1 = rest % J.vf_n[1];
+  int64_t v2 = rest / J.vf_n[1];
+  r.g[0] = geo_of(v0, J.is[0], J.sm[0], J.st[0], J.dims[0]);
+  r.g[1] = geo_of(v1, J.is[1], J.sm[1], J.st[1], J.dims[1]);
+  r.g[2] = geo_of(v2, J.is[2], J.sm[2], J.st[2], J.dims[2]);
+  return r;
+}
+
+// Box sum of a 3-D FP64 integral array over [a, b) per dim.
+__device__ __forceinline__ double sat_box3(const double* __restrict__ S, int64_t D0, int64_t D01, int64_t a0,
+                                           int64_t b0, int64_t a1, int64_t b1, int64_t a2, int64_t b2) {
+  auto at = [&](int64_t i, int64_t j, int64_t k) { return S[i + D0 * j + D01 * k]; };
+  return (((at(b0, b1, b2) - at(a0, b1, b2)) - at(b0, a1, b2)) - at(b0, b1, a2)) +
+         ((at(a0, a1, b2) + at(a0, b1, a2)) + at(b0, a1, a2)) - at(a0, a1, a2);
+}
+
+// ZNCC means of one IA from the integral arrays of both images (sum over the IA
+// and over the clipped search region), rounded like the reference (Q12).
+struct Means3 {
+  float f32, g32;
+  double f64, g64;
+};
+__device__ __forceinline__ Means3 means3(const Job& J, const Geo3& G, const double* sat1_img1) {
+  const int64_t D0 = J.dims[0] + 1, D01 = D0 * (J.dims[1] + 1);
+  const Geo *g0 = &G.g[0], *g1 = &G.g[1], *g2 = &G.g[2];
+  double sf = sat_box3(sat1_img1, D0, D01, g0->tlf, g0->tlf + J.is[0], g1->tlf, g1->tlf + J.is[1], g2->tlf,
+                       g2->tlf + J.is[2]);
+  double sg = sat_box3(J.sat1, D0, D01, g0->sr, g0->sr + g0->gsz, g1->sr, g1->sr + g1->gsz, g2->sr,
+                       g2->sr + g2->gsz);
+  const double nF = (double)J.is[0] * J.is[1] * J.is[2];
+  const double nG = (double)g0->gsz * g1->gsz * g2->gsz;
+  Means3 m;
+  m.f32 = (float)sf / (float)nF;
+  m.g32 = (float)sg / (float)nG;
+  m.f64 = sf / nF;
+  m.g64 = sg / nG;
+  return m;
+}
+
+// ---------------------------------------------------------------------------
+// K1: forward 2-D transform of slab i2 of the packed window.
+template <int L, int ALG, typename T, int LPL>
+__global__ __launch_bounds__((Cfg3<L, T, LPL>::TPL)) void slab_fwd_kernel(Job J, Scratch3 sc, int64_t ia_base,
+                                                                          int64_t n_ia, const double* sat1_img1) {
+  using C = Cfg3<L, T, LPL>;
+  constexpr int M = C::M, LD = C::LD;
+  extern __shared__ __align__(16) unsigned char smem[];
+  cpx<T>* Z = reinterpret_cast<cpx<T>*>(smem);
+  double* red = reinterpret_cast<double*>(smem + (size_t)L * LD * sizeof(cpx<T>));
+  const int t = threadIdx.x;
+  const int line = t / LPL, q = t % LPL;
+  const int64_t il = blockIdx.x / L;  // IA within the chunk
+  const int i2 = blockIdx.x % L;
+  const int64_t ia = ia_base + il;
+  if (il >= n_ia) return;
+  const Geo3 G = geo3_of(J, ia);
+  const Geo &g0 = G.g[0], &g1 = G.g[1], &g2 = G.g[2];
+  const int dt = J.in_dtype;
+  const int64_t d0 = J.dims[0], d01 = J.dims[0] * J.dims[1];
+
+  Means3 mu{0.f, 0.f, 0.0, 0.0};
+  if constexpr (ALG == ALG_ZNCC) mu = means3(J, G, sat1_img1);
+
+  const int k0 = line;
+  const bool f_pl = i2 < J.is[2];
+  const bool g_pl = i2 >= g2.goff && i2 < g2.goff + g2.gsz;
+  const bool f_row = f_pl && k0 < J.is[0];
+  const bool g_row = g_pl && k0 >= g0.goff && k0 < g0.goff + g0.gsz;
+  const int64_t f_base = (g0.tlf + k0) + d0 * g1.tlf + d01 * (g2.tlf + i2);
+  const int64_t g_base = (g0.sr + (k0 - g0.goff)) + d0 * (g1.sr - g1.goff) + d01 * (g2.sr + (i2 - g2.goff));
+  const bool want_max = J.has_threshold;
+  double sumF2 = 0.0, mx1 = -INFINITY;
+  cpx<T> x[M];
+  auto load = [&](auto tin) {
+    using TIN = decltype(tin);
+    const TIN* pf = (const TIN*)J.img1 + f_base;
+    const TIN* pg = (const TIN*)J.img2 + g_base;
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      const int k1 = LPL * k + q;
+      const int64_t o = d0 * k1;
+      double a = 0.0, g = 0.0;
+      if (f_row && k1 < J.is[1]) {
+        const TIN f = pf[o];
+        if (want_max) mx1 = fmax(mx1, (double)f);
+        if constexpr (ALG == ALG_ZNCC)
+          a = std::is_same<TIN, float>::value ? (double)((float)f - mu.f32) : (double)f - mu.f64;
+        else
+          a = (double)f;
+        if constexpr (ALG != ALG_FFTCC) sumF2 += a * a;
+      }
+      if (g_row && k1 >= g1.goff && k1 < g1.goff + g1.gsz) {
+        const TIN gv = pg[o];
+        if constexpr (ALG == ALG_ZNCC)
+          g = std::is_same<TIN, float>::value ? (double)((float)gv - mu.g32) : (double)gv - mu.g64;
+        else
+          g = (double)gv;
+      }
+      x[k] = cpx<T>{(T)a, (T)g};
+    }
+  };
+  if (line < L) {
+    if (dt == DT_F32) load(float{});
+    else load(double{});
+  }
+  sumF2 = Group<C::TPL>::reduce(sumF2, OpAdd{}, red, t);
+  mx1 = Group<C::TPL>::reduce(mx1, OpMax{}, red, t);
+  if (t == 0) {
+    double* part = sc.part + il * (3 * L);
+    part[i2] = sumF2;
+    part[L + i2] = mx1;
+  }
+
+  // forward along dim 2 (registers), then dim 1 through LDS
+  const int cM = (LPL == 1 ? 0 : chunk_of<LPL>(q)) * M;
+  if (line < L) {
+    line_fwd<L, LPL, T>(x, q);
+#pragma unroll
+    for (int k = 0; k < M; ++k) Z[(cM + k) * LD + line] = x[k];
+  }
+  __syncthreads();
+  if (line < L) {
+#pragma unroll
+    for (int k = 0; k < M; ++k) x[k] = Z[line * LD + LPL * k + q];
+    line_fwd<L, LPL, T>(x, q);
+    // plane layout in scratch: [i2][k1][k0]
+    cpx<T>* W = (cpx<T>*)sc.w + il * sc.w_stride + (int64_t)i2 * L * L + (int64_t)line * L;
+#pragma unroll
+    for (int k = 0; k < M; ++k) W[cM + k] = x[k];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K2: along dim 3 for the planes k1 and -k1, conjugate product, inverse.
+template <int L, typename T, int LPL>
+__global__ __launch_bounds__((Cfg3<L, T, LPL>::NT2)) void zline_kernel(Scratch3 sc, int64_t n_ia, int ns2) {
+  using C = Cfg3<L, T, LPL>;
+  constexpr int M = C::M, LD = C::LD;
+  constexpr int NP = L / 2 + 1;  // plane pairs k1 in [0, L/2]
+  extern __shared__ __align__(16) unsigned char smem[];
+  cpx<T>* Pl = reinterpret_cast<cpx<T>*>(smem);  // [plane 0/1][k2][k0], LD stride
+  const int t = threadIdx.x;
+  const int64_t il = blockIdx.x / NP;
+  const int ka = blockIdx.x % NP;
+  if (il >= n_ia) return;
+  const int kb = (L - ka) % L;
+  const int pl = t / (L * LPL);  // which plane this lane works on
+  const int tt = t % (L * LPL);
+  const int line = tt / LPL, q = tt % LPL;  // line = k0
+  const int k1 = pl == 0 ? ka : kb;
+  const bool lane = pl < 2 && !(pl == 1 && ka == kb);
+  cpx<T>* W = (cpx<T>*)sc.w + il * sc.w_stride;
+  const int cM = (LPL == 1 ? 0 : chunk_of<LPL>(q)) * M;
+  cpx<T> x[M];
+  if (lane) {
+#pragma unroll
+    for (int k = 0; k < M; ++k) x[k] = W[(int64_t)(LPL * k + q) * L * L + (int64_t)k1 * L + line];
+    line_fwd<L, LPL, T>(x, q);
+#pragma unroll
+    for (int k = 0; k < M; ++k) Pl[(pl * L + cM + k) * LD + line] = x[k];
+  }
+  __syncthreads();
+  if (lane) {
+    // partner of (k0, k1, k2) is (-k0, -k1, -k2): plane kb for ka and vice versa
+    const int pp = (ka == kb) ? 0 : 1 - pl;
+    const int k0n = (L - line) % L;
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      const int k2 = cM + k;
+      const int k2n = (L - k2) % L;
+      cpx<T> z = x[k];
+      cpx<T> n = Pl[(pp * L + k2n) * LD + k0n];
+      T ar = T(0.5) * (z.x + n.x), ai = T(0.5) * (z.y - n.y);
+      T br = T(0.5) * (z.y + n.y), bi = T(0.5) * (n.x - z.x);
+      x[k] = cpx<T>{ar * br + ai * bi, ar * bi - ai * br};
+    }
+    line_inv<L, LPL, T>(x, q);
+  }
+  __syncthreads();  // every forward value of these planes is read before any write
+  if (lane) {
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      const int s2 = LPL * m + q;
+      if (s2 < ns2) W[(int64_t)s2 * L * L + (int64_t)k1 * L + line] = x[m];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K3: inverse 2-D transform of plane s2 down to the (ns0 x ns1) window.
+template <int L, typename T, int LPL>
+__global__ __launch_bounds__((Cfg3<L, T, LPL>::TPL)) void slab_inv_kernel(Scratch3 sc, int64_t n_ia, int ns0,
+                                                                          int ns1, int ns2) {
+  using C = Cfg3<L, T, LPL>;
+  constexpr int M = C::M, LD = C::LD;
+  extern __shared__ __align__(16) unsigned char smem[];
+  cpx<T>* Z = reinterpret_cast<cpx<T>*>(smem);
+  const int t = threadIdx.x;
+  const int line = t / LPL, q = t % LPL;
+  const int64_t il = blockIdx.x / ns2;
+  const int s2 = blockIdx.x % ns2;
+  if (il >= n_ia) return;
+  const cpx<T>* W = (const cpx<T>*)sc.w + il * sc.w_stride + (int64_t)s2 * L * L;
+  const int cM = (LPL == 1 ? 0 : chunk_of<LPL>(q)) * M;
+  cpx<T> x[M];
+  // inverse along dim 1: line = k1, values over k0 in this lane's chunk
+  if (line < L) {
+#pragma unroll
+    for (int k = 0; k < M; ++k) x[k] = W[(int64_t)line * L + cM + k];
+    line_inv<L, LPL, T>(x, q);
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      const int s0 = LPL * m + q;
+      if (s0 < ns0) Z[line * LD + s0] = x[m];
+    }
+  }
+  __syncthreads();
+  // inverse along dim 2 for the ns0 needed rows: line = s0
+  if (line < ns0) {
+#pragma unroll
+    for (int k = 0; k < M; ++k) x[k] = Z[(cM + k) * LD + line];
+    line_inv<L, LPL, T>(x, q);
+    T* S = (T*)sc.s + il * sc.s_stride + (int64_t)s2 * ns0 * ns1;
+    const T kScale = T(1) / T(L * L * L);
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      const int s1 = LPL * m + q;
+      if (s1 < ns1) S[s1 * ns0 + line] = x[m].x * kScale;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K4: normalisation, peak, subpixel, SN and output for one IA.
+template <int ALG, typename T>
+__global__ __launch_bounds__(256) void finalize3d_kernel(Job J, Scratch3 sc, int64_t ia_base, int64_t n_ia, int L,
+                                                         const double* sat1_img1) {
+  constexpr int NT = 256;
+  __shared__ double red[16];
+  const int t = threadIdx.x;
+  const int64_t il = blockIdx.x;
+  if (il >= n_ia) return;
+  const int64_t ia = ia_base + il;
+  const Geo3 G = geo3_of(J, ia);
+  const Geo &g0 = G.g[0], &g1 = G.g[1], &g2 = G.g[2];
+  const int sm0 = J.sm[0], sm1 = J.sm[1], sm2 = J.sm[2];
+  const int is0 = J.is[0], is1 = J.is[1], is2 = J.is[2];
+  const int ns0 = 2 * sm0 + 1, ns1 = 2 * sm1 + 1, ns2 = 2 * sm2 + 1;
+  const int nS = ns0 * ns1 * ns2;
+  T* S = (T*)sc.s + il * sc.s_stride;
+
+  // IA constants from the per-slab partials
+  double sumF2 = 0.0, mx1 = -INFINITY;
+  const double* part = sc.part + il * (3 * L);
+  for (int i = 0; i < L; ++i) {
+    sumF2 += part[i];
+    mx1 = fmax(mx1, part[L + i]);
+  }
+  const bool skip = J.skip ? (J.skip[ia] != 0) : (J.has_threshold && mx1 < J.threshold);
+
+  if constexpr (ALG == ALG_ZNCC || ALG == ALG_NSQECC) {
+    const int64_t D0 = J.dims[0] + 1, D01 = D0 * (J.dims[1] + 1);
+    double mu = 0.0;
+    if constexpr (ALG == ALG_ZNCC) {
+      Means3 m = means3(J, G, sat1_img1);
+      mu = (J.in_dtype == DT_F32) ? (double)m.g32 : m.g64;
+    }
+    const double sqF = sqrt(sumF2);
+    for (int e = t; e < nS; e += NT) {
+      const int s0 = e % ns0, s1 = (e / ns0) % ns1, s2 = e / (ns0 * ns1);
+      int lo[3], hi[3];
+      const int s[3] = {s0, s1, s2};
+      const int isz[3] = {is0, is1, is2};
+      bool empty = false;
+      for (int d = 0; d < 3; ++d) {
+        const Geo& g = G.g[d];
+        lo[d] = s[d] > g.goff ? s[d] : g.goff;
+        hi[d] = (s[d] + isz[d]) < (g.goff + g.gsz) ? (s[d] + isz[d]) : (g.goff + g.gsz);
+        if (hi[d] <= lo[d]) empty = true;
+      }
+      double W = 0.0;
+      if (!empty) {
+        const int64_t a0 = g0.sr + (lo[0] - g0.goff), b0 = g0.sr + (hi[0] - g0.goff);
+        const int64_t a1 = g1.sr + (lo[1] - g1.goff), b1 = g1.sr + (hi[1] - g1.goff);
+        const int64_t a2 = g2.sr + (lo[2] - g2.goff), b2 = g2.sr + (hi[2] - g2.goff);
+        W = sat_box3(J.sat2, D0, D01, a0, b0, a1, b1, a2, b2);
+        if constexpr (ALG == ALG_ZNCC) {
+          const double n = (double)((b0 - a0) * (b1 - a1) * (b2 - a2));
+          W = (W - 2.0 * mu * sat_box3(J.sat1, D0, D01, a0, b0, a1, b1, a2, b2)) + n * mu * mu;
+        }
+        W = W > 0.0 ? W : 0.0;
+      }
+      const double c = (double)S[e];
+      const double den = sqrt(W) * sqF;
+      double r;
+      if constexpr (ALG == ALG_ZNCC) {
+        r = c / den;
+      } else {
+        const double num = (sumF2 - 2.0 * c) + W;
+        r = den / (num + den);
+      }
+      S[e] = (T)r;
+    }
+    __syncthreads();
+  }
+
+  // peak over [off_lo, 2SM - off_hi] per dim, column-major first strict max
+  const int lo0 = g0.goff, hi0 = 2 * sm0 - g0.off_hi;
+  const int lo1 = g1.goff, hi1 = 2 * sm1 - g1.off_hi;
+  const int lo2 = g2.goff, hi2 = 2 * sm2 - g2.off_hi;
+  const int w0 = hi0 - lo0 + 1, w1 = hi1 - lo1 + 1, w2 = hi2 - lo2 + 1;
+  const int nwin = (w0 > 0 && w1 > 0 && w2 > 0) ? w0 * w1 * w2 : 0;
+  Best<T> bst{T(0), INT32_MAX};
+  for (int e = t; e < nwin; e += NT) {
+    const int p0 = lo0 + e % w0, p1 = lo1 + (e / w0) % w1, p2 = lo2 + e / (w0 * w1);
+    const int li = p0 + ns0 * (p1 + ns1 * p2);
+    const T v = S[li];
+    if (v > bst.v) bst = Best<T>{v, li};
+  }
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1) {
+    Best<T> o{__shfl_xor(bst.v, m), __shfl_xor(bst.i, m)};
+    bst = best_of(bst, o);
+  }
+  {
+    T* rv = reinterpret_cast<T*>(red);
+    int32_t* ri = reinterpret_cast<int32_t*>(red + 8);
+    __syncthreads();
+    if ((t & 63) == 0) {
+      rv[t >> 6] = bst.v;
+      ri[t >> 6] = bst.i;
+    }
+    __syncthreads();
+    Best<T> r{rv[0], ri[0]};
+    for (int w = 1; w < NT / 64; ++w) r = best_of(r, Best<T>{rv[w], ri[w]});
+    bst = r;
+  }
+
+  // SN over [off_lo, 2SM - 1 - off_hi] per dim (natural 3-D extension)
+  double sn = 0.0;
+  if (J.compute_sn) {
+    const int u0 = (2 * sm0 - 1 - g0.off_hi) - lo0 + 1;
+    const int u1 = (2 * sm1 - 1 - g1.off_hi) - lo1 + 1;
+    const int u2 = (2 * sm2 - 1 - g2.off_hi) - lo2 + 1;
+    const int nsn = (u0 > 0 && u1 > 0 && u2 > 0) ? u0 * u1 * u2 : 0;
+    double mx = -INFINITY, sm = 0.0;
+    for (int e = t; e < nsn; e += NT) {
+      const int p0 = lo0 + e % u0, p1 = lo1 + (e / u0) % u1, p2 = lo2 + e / (u0 * u1);
+      const double v = (double)S[p0 + ns0 * (p1 + ns1 * p2)];
+      mx = (v > mx || v != v) ? v : mx;
+      sm += v;
+    }
+    mx = Group<NT>::reduce(mx, OpMaxNaN{}, red, t);
+    sm = Group<NT>::reduce(sm, OpAdd{}, red, t);
+    sn = nsn > 0 ? mx / ((sm - mx) / (double)(nsn - 1)) : (double)NAN;
+  }
+
+  if (t == 0) {
+    double r[3];
+    const int smv[3] = {sm0, sm1, sm2};
+    if (skip) {
+      r[0] = r[1] = r[2] = 0.0;
+      sn = 0.0;
+    } else if (bst.i == INT32_MAX) {
+      for (int d = 0; d < 3; ++d) r[d] = -(double)(smv[d] + 1);
+    } else {
+      const int p[3] = {bst.i % ns0, (bst.i / ns0) % ns1, bst.i / (ns0 * ns1)};
+      const bool interior = p[0] > lo0 && p[0] < hi0 && p[1] > lo1 && p[1] < hi1 && p[2] > lo2 && p[2] < hi2;
+      for (int d = 0; d < 3; ++d) r[d] = (double)(p[d] + 1) - (double)(smv[d] + 1);
+      if (interior) {
+        auto at = [&](int a, int b, int c) { return (double)S[(p[0] + a) + ns0 * ((p[1] + b) + ns1 * (p[2] + c))]; };
+        double mn = INFINITY;
+        bool nan = false;
+        for (int c = -1; c <= 1; ++c)
+          for (int b = -1; b <= 1; ++b)
+            for (int a = -1; a <= 1; ++a) {
+              const double v = at(a, b, c);
+              if (v != v) nan = true;
+              mn = v < mn ? v : mn;
+            }
+        if (nan) mn = NAN;
+        const double lmx = log(1.0 + at(0, 0, 0) - mn);
+        const double lm[3] = {log(1.0 + at(-1, 0, 0) - mn), log(1.0 + at(0, -1, 0) - mn), log(1.0 + at(0, 0, -1) - mn)};
+        const double lp[3] = {log(1.0 + at(1, 0, 0) - mn), log(1.0 + at(0, 1, 0) - mn), log(1.0 + at(0, 0, 1) - mn)};
+        for (int d = 0; d < 3; ++d) {
+          const double qd = (lm[d] - lp[d]) / (2 * lm[d] - 4 * lmx + 2 * lp[d]);
+          r[d] = ((double)(p[d] + 1) + qd) - (double)(smv[d] + 1);
+        }
+      }
+    }
+    if (J.peaks) {
+      const bool has = !skip && bst.i != INT32_MAX;
+      J.peaks[3 * ia + 0] = has ? bst.i % ns0 + 1 : 0;
+      J.peaks[3 * ia + 1] = has ? (bst.i / ns0) % ns1 + 1 : 0;
+      J.peaks[3 * ia + 2] = has ? bst.i / (ns0 * ns1) + 1 : 0;
+    }
+    if (J.out_dtype == DT_F32) {
+      float* vf = (float*)J.vf;
+      for (int d = 0; d < 3; ++d) vf[3 * ia + d] = (float)r[d];
+      if (J.sn) ((float*)J.sn)[ia] = (float)sn;
+    } else {
+      double* vf = (double*)J.vf;
+      for (int d = 0; d < 3; ++d) vf[3 * ia + d] = r[d];
+      if (J.sn) ((double*)J.sn)[ia] = sn;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launch of one chunk of IAs [ia_base, ia_base + n).
+template <int L, typename T, int LPL>
+size_t scratch3_per_ia(int ns0, int ns1, int ns2) {
+  size_t w = (size_t)L * L * L * sizeof(cpx<T>);
+  size_t s = ((size_t)ns0 * ns1 * ns2 * sizeof(T) + 255) & ~(size_t)255;
+  size_t p = (size_t)3 * L * sizeof(double);
+  return w + s + p;
+}
+
+template <int L, typename T, int LPL>
+hipError_t launch3d_chunk(const Job& J, void* scratch, int64_t chunk_cap, int64_t ia_base, int64_t n,
+                          const double* sat1_img1, hipStream_t s) {
+  using C = Cfg3<L, T, LPL>;
+  const int ns0 = 2 * J.sm[0] + 1, ns1 = 2 * J.sm[1] + 1, ns2 = 2 * J.sm[2] + 1;
+  Scratch3 sc;
+  sc.w_stride = (int64_t)L * L * L;
+  sc.s_stride = (int64_t)ns0 * ns1 * ns2;
+  char* b = (char*)scratch;
+  sc.w = b;
+  b += (size_t)chunk_cap * L * L * L * sizeof(cpx<T>);
+  sc.s = b;
+  b += (((size_t)chunk_cap * ns0 * ns1 * ns2 * sizeof(T)) + 255) & ~(size_t)255;
+  sc.part = (double*)b;
+  const size_t lds1 = (size_t)L * C::LD * sizeof(cpx<T>) + 16 * sizeof(double);
+  const size_t lds2 = (size_t)2 * L * C::LD * sizeof(cpx<T>);
+  const size_t lds3 = (size_t)L * C::LD * sizeof(cpx<T>);
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute((const void*)slab_fwd_kernel<L, ALG_FFTCC, T, LPL>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    (void)hipFuncSetAttribute((const void*)slab_fwd_kernel<L, ALG_ZNCC, T, LPL>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    (void)hipFuncSetAttribute((const void*)slab_fwd_kernel<L, ALG_NSQECC, T, LPL>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    (void)hipFuncSetAttribute((const void*)zline_kernel<L, T, LPL>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              160 * 1024);
+    (void)hipFuncSetAttribute((const void*)slab_inv_kernel<L, T, LPL>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              160 * 1024);
+    attr = true;
+  }
+  switch (J.alg) {
+    case ALG_FFTCC:
+      hipLaunchKernelGGL((slab_fwd_kernel<L, ALG_FFTCC, T, LPL>), dim3((unsigned)(n * L)), dim3(C::TPL), lds1, s, J,
+                         sc, ia_base, n, sat1_img1);
+      break;
+    case ALG_ZNCC:
+      hipLaunchKernelGGL((slab_fwd_kernel<L, ALG_ZNCC, T, LPL>), dim3((unsigned)(n * L)), dim3(C::TPL), lds1, s, J,
+                         sc, ia_base, n, sat1_img1);
+      break;
+    case ALG_NSQECC:
+      hipLaunchKernelGGL((slab_fwd_kernel<L, ALG_NSQECC, T, LPL>), dim3((unsigned)(n * L)), dim3(C::TPL), lds1, s,
+                         J, sc, ia_base, n, sat1_img1);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  hipLaunchKernelGGL((zline_kernel<L, T, LPL>), dim3((unsigned)(n * (L / 2 + 1))), dim3(C::NT2), lds2, s, sc, n,
+                     ns2);
+  hipLaunchKernelGGL((slab_inv_kernel<L, T, LPL>), dim3((unsigned)(n * ns2)), dim3(C::TPL), lds3, s, sc, n, ns0, ns1,
+                     ns2);
+  switch (J.alg) {
+    case ALG_FFTCC:
+      hipLaunchKernelGGL((finalize3d_kernel<ALG_FFTCC, T>), dim3((unsigned)n), dim3(256), 0, s, J, sc, ia_base, n, L,
+                         sat1_img1);
+      break;
+    case ALG_ZNCC:
+      hipLaunchKernelGGL((finalize3d_kernel<ALG_ZNCC, T>), dim3((unsigned)n), dim3(256), 0, s, J, sc, ia_base, n, L,
+                         sat1_img1);
+      break;
+    case ALG_NSQECC:
+      hipLaunchKernelGGL((finalize3d_kernel<ALG_NSQECC, T>), dim3((unsigned)n), dim3(256), 0, s, J, sc, ia_base, n,
+                         L, sat1_img1);
+      break;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace mqpiv
