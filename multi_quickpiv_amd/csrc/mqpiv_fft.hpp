@@ -156,6 +156,8 @@ __host__ __device__ __forceinline__ void stockham_pass(const cpx<T> (&a)[L], cpx
 // Radix plans: the product of the listed radices is L.
 template <int L>
 struct Plan;
+template <> struct Plan<4> { static constexpr int n = 1; static constexpr int r[4] = {4, 1, 1, 1}; };
+template <> struct Plan<6> { static constexpr int n = 2; static constexpr int r[4] = {3, 2, 1, 1}; };
 template <> struct Plan<8> { static constexpr int n = 2; static constexpr int r[4] = {4, 2, 1, 1}; };
 template <> struct Plan<12> { static constexpr int n = 2; static constexpr int r[4] = {4, 3, 1, 1}; };
 template <> struct Plan<16> { static constexpr int n = 2; static constexpr int r[4] = {4, 4, 1, 1}; };
@@ -176,7 +178,10 @@ __host__ __device__ __forceinline__ void fft_line(cpx<T> (&x)[L]) {
   constexpr int r2 = Plan<L>::r[2];
   constexpr int r3 = Plan<L>::r[3];
   stockham_pass<L, r0, 1, SIGN>(x, y);
-  if constexpr (n == 2) {
+  if constexpr (n == 1) {
+#pragma unroll
+    for (int i = 0; i < L; ++i) x[i] = y[i];
+  } else if constexpr (n == 2) {
     stockham_pass<L, r1, r0, SIGN>(y, x);
   } else if constexpr (n == 3) {
     stockham_pass<L, r1, r0, SIGN>(y, x);
@@ -191,48 +196,60 @@ __host__ __device__ __forceinline__ void fft_line(cpx<T> (&x)[L]) {
 }
 
 // ---------------------------------------------------------------------------
-// Lane-group transforms: a line of length L split over P = 2 or 4 adjacent
-// lanes (an aligned pair / quad), each holding M = L/P samples.
+// Lane-group transforms: a line of length L split over P = 2, 4 or 8 adjacent
+// lanes (an aligned group), each holding M = L/P samples.
 //
 // Forward (decimation in time): on entry lane q holds x[P*k + q], k < M; each
 // lane transforms its decimated subsequence, then log2(P) radix-2 stages across
-// the group (partner = q ^ 2 first, then q ^ 1 for P = 4) combine them; on exit
-// lane q holds X[chunk(q)*M + k] with chunk(q) = bit-reverse of q.
+// the group (partners q ^ P/2, ..., q ^ 2, q ^ 1) combine them; on exit lane q
+// holds X[chunk(q)*M + k] with chunk(q) = the log2(P)-bit reversal of q.
 // Inverse direction (decimation in frequency): on entry lane q holds
-// x[chunk(q)*M + k]; radix-2 stages across the group (q ^ 1, then q ^ 2) come
-// first, then the local transform; on exit lane q holds X[P*m + q].
+// x[chunk(q)*M + k]; the radix-2 stages run first in the reverse order
+// (q ^ 1, ..., q ^ P/2), then the local transform; on exit lane q holds
+// X[P*m + q].
+//
+// Stage with partner mask 2^j (forward order j = log2(P)-1 .. 0) works on
+// sub-transforms of length L / 2^j; the twiddle offset of a lane is
+// chunk_{P/2^(j+1)}(q >> (j+1)) * M (the chunk its sub-group produced).
 //
 // The partner exchange is a callable so the same code runs on the CPU in unit
-// tests.  On the GPU it is a DPP quad permutation (one VALU move per dword).
+// tests.  On the GPU, q ^ 1 and q ^ 2 are DPP quad permutations (one VALU move
+// per dword) and q ^ 4 is a ds_swizzle xor.
 
 template <int P>
 __host__ __device__ constexpr int chunk_of(int q) {
-  return P == 2 ? q : ((q & 1) << 1) | (q >> 1);
+  int r = 0;
+  for (int b = 1, c = P >> 1; b < P; b <<= 1, c >>= 1)
+    if (q & b) r |= c;
+  return r;
 }
 
 #if defined(__HIP_DEVICE_COMPILE__)
 template <int MASK>
-__device__ __forceinline__ float dpp_xor(float v) {
-  // quad_perm selectors: lane ^ 1 -> [1,0,3,2] = 0xB1, lane ^ 2 -> [2,3,0,1] = 0x4E
-  constexpr int ctrl = MASK == 1 ? 0xB1 : 0x4E;
-  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), ctrl, 0xF, 0xF, false));
+__device__ __forceinline__ int xor_lane_i32(int v) {
+  if constexpr (MASK == 1) return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+  else if constexpr (MASK == 2) return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+  else return __builtin_amdgcn_ds_swizzle(v, (MASK << 10) | 0x1F);  // bit mode: xor MASK
 }
 template <int MASK>
-__device__ __forceinline__ double dpp_xor(double v) {
-  constexpr int ctrl = MASK == 1 ? 0xB1 : 0x4E;
+__device__ __forceinline__ float xor_lane(float v) {
+  return __builtin_bit_cast(float, xor_lane_i32<MASK>(__builtin_bit_cast(int, v)));
+}
+template <int MASK>
+__device__ __forceinline__ double xor_lane(double v) {
   long long u = __builtin_bit_cast(long long, v);
-  int lo = __builtin_amdgcn_mov_dpp((int)(u & 0xffffffffll), ctrl, 0xF, 0xF, false);
-  int hi = __builtin_amdgcn_mov_dpp((int)(u >> 32), ctrl, 0xF, 0xF, false);
+  int lo = xor_lane_i32<MASK>((int)(u & 0xffffffffll));
+  int hi = xor_lane_i32<MASK>((int)(u >> 32));
   return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
 }
 #endif
 
-// Device exchange with the lane q ^ MASK of the same quad.
+// Device exchange with lane q ^ MASK of the same group.
 struct DppXchg {
   template <int MASK, typename T>
   __device__ __forceinline__ cpx<T> get(cpx<T> v, int) const {
 #if defined(__HIP_DEVICE_COMPILE__)
-    return cpx<T>{dpp_xor<MASK>(v.x), dpp_xor<MASK>(v.y)};
+    return cpx<T>{xor_lane<MASK>(v.x), xor_lane<MASK>(v.y)};
 #else
     return v;
 #endif
@@ -272,33 +289,35 @@ __host__ __device__ __forceinline__ void group_dif_stage(cpx<T> (&v)[M], int q, 
   }
 }
 
+// Stage j of a P-lane group (partner mask 2^j).
+template <int L, int P, int J, int SIGN, bool DIT, int M, typename T, typename X>
+__host__ __device__ __forceinline__ void group_stage(cpx<T> (&v)[M], int q, const X& xchg) {
+  constexpr int MASK = 1 << J;
+  constexpr int LEN = L >> J;
+  const int off = chunk_of<(P >> (J + 1))>(q >> (J + 1)) * M;
+  if constexpr (DIT) group_dit_stage<LEN, SIGN, MASK, M>(v, q, off, xchg);
+  else group_dif_stage<LEN, SIGN, MASK, M>(v, q, off, xchg);
+}
+
 // Forward transform over a group of P lanes (see above).
 template <int L, int P, int SIGN, typename T, typename X>
 __host__ __device__ __forceinline__ void fft_group_dit(cpx<T> (&v)[L / P], int q, const X& xchg) {
   constexpr int M = L / P;
+  static_assert(P == 2 || P == 4 || P == 8, "groups of 2, 4 or 8 lanes");
   fft_line<M, SIGN, T>(v);
-  if constexpr (P == 2) {
-    group_dit_stage<L, SIGN, 1, M>(v, q, 0, xchg);
-  } else {
-    static_assert(P == 4, "groups of 2 or 4 lanes");
-    // sub-transforms of length L/2 over lanes {q, q^2}
-    group_dit_stage<L / 2, SIGN, 2, M>(v, q, 0, xchg);
-    // lanes q (even) and q^1 now hold E/O at chunk c = q >> 1 of length-L/2 halves
-    group_dit_stage<L, SIGN, 1, M>(v, q, (q >> 1) * M, xchg);
-  }
+  if constexpr (P >= 8) group_stage<L, P, 2, SIGN, true, M>(v, q, xchg);
+  if constexpr (P >= 4) group_stage<L, P, 1, SIGN, true, M>(v, q, xchg);
+  group_stage<L, P, 0, SIGN, true, M>(v, q, xchg);
 }
 
 // Inverse-direction transform over a group of P lanes (see above).
 template <int L, int P, int SIGN, typename T, typename X>
 __host__ __device__ __forceinline__ void fft_group_dif(cpx<T> (&v)[L / P], int q, const X& xchg) {
   constexpr int M = L / P;
-  if constexpr (P == 2) {
-    group_dif_stage<L, SIGN, 1, M>(v, q, 0, xchg);
-  } else {
-    static_assert(P == 4, "groups of 2 or 4 lanes");
-    group_dif_stage<L, SIGN, 1, M>(v, q, (q >> 1) * M, xchg);
-    group_dif_stage<L / 2, SIGN, 2, M>(v, q, 0, xchg);
-  }
+  static_assert(P == 2 || P == 4 || P == 8, "groups of 2, 4 or 8 lanes");
+  group_stage<L, P, 0, SIGN, false, M>(v, q, xchg);
+  if constexpr (P >= 4) group_stage<L, P, 1, SIGN, false, M>(v, q, xchg);
+  if constexpr (P >= 8) group_stage<L, P, 2, SIGN, false, M>(v, q, xchg);
   fft_line<M, SIGN, T>(v);
 }
 

@@ -28,8 +28,8 @@ extern template hipError_t launch_2d_l<64, float, 4>(int, const Job&, const Job*
 extern template size_t lds_bytes_2d<96, float, 4>(int, int, int, int);
 extern template hipError_t launch_2d_l<96, float, 4>(int, const Job&, const Job*, const int64_t*, int,
                                                     int64_t, int, int, int, hipStream_t);
-extern template size_t lds_bytes_2d<128, float, 4>(int, int, int, int);
-extern template hipError_t launch_2d_l<128, float, 4>(int, const Job&, const Job*, const int64_t*, int,
+extern template size_t lds_bytes_2d<128, float, 8>(int, int, int, int);
+extern template hipError_t launch_2d_l<128, float, 8>(int, const Job&, const Job*, const int64_t*, int,
                                                     int64_t, int, int, int, hipStream_t);
 extern template size_t lds_bytes_2d<16, double, 2>(int, int, int, int);
 extern template hipError_t launch_2d_l<16, double, 2>(int, const Job&, const Job*, const int64_t*, int,
@@ -43,8 +43,8 @@ extern template hipError_t launch_2d_l<48, double, 4>(int, const Job&, const Job
 extern template size_t lds_bytes_2d<64, double, 4>(int, int, int, int);
 extern template hipError_t launch_2d_l<64, double, 4>(int, const Job&, const Job*, const int64_t*, int,
                                                     int64_t, int, int, int, hipStream_t);
-extern template size_t lds_bytes_2d<96, double, 4>(int, int, int, int);
-extern template hipError_t launch_2d_l<96, double, 4>(int, const Job&, const Job*, const int64_t*, int,
+extern template size_t lds_bytes_2d<96, double, 8>(int, int, int, int);
+extern template hipError_t launch_2d_l<96, double, 8>(int, const Job&, const Job*, const int64_t*, int,
                                                     int64_t, int, int, int, hipStream_t);
 
 // Transform length for a search size: the smallest supported length >= ss
@@ -69,7 +69,7 @@ size_t lds_2d(int fp64, int L, int alg, int ns0, int ns1, int ss0) {
       case 32: return lds_bytes_2d<32, double, 4>(alg, ns0, ns1, ss0);
       case 48: return lds_bytes_2d<48, double, 4>(alg, ns0, ns1, ss0);
       case 64: return lds_bytes_2d<64, double, 4>(alg, ns0, ns1, ss0);
-      case 96: return lds_bytes_2d<96, double, 4>(alg, ns0, ns1, ss0);
+      case 96: return lds_bytes_2d<96, double, 8>(alg, ns0, ns1, ss0);
     }
   } else {
     switch (L) {
@@ -78,7 +78,7 @@ size_t lds_2d(int fp64, int L, int alg, int ns0, int ns1, int ss0) {
       case 48: return lds_bytes_2d<48, float, 4>(alg, ns0, ns1, ss0);
       case 64: return lds_bytes_2d<64, float, 4>(alg, ns0, ns1, ss0);
       case 96: return lds_bytes_2d<96, float, 4>(alg, ns0, ns1, ss0);
-      case 128: return lds_bytes_2d<128, float, 4>(alg, ns0, ns1, ss0);
+      case 128: return lds_bytes_2d<128, float, 8>(alg, ns0, ns1, ss0);
     }
   }
   return (size_t)-1;
@@ -93,7 +93,7 @@ hipError_t launch_piv2d(int fp64, int L, int alg, const Job& j0, const Job* d_jo
       case 32: return launch_2d_l<32, double, 4>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, s);
       case 48: return launch_2d_l<48, double, 4>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, s);
       case 64: return launch_2d_l<64, double, 4>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, s);
-      case 96: return launch_2d_l<96, double, 4>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, s);
+      case 96: return launch_2d_l<96, double, 8>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, s);
     }
   } else {
     switch (L) {
@@ -102,7 +102,7 @@ hipError_t launch_piv2d(int fp64, int L, int alg, const Job& j0, const Job* d_jo
       case 48: return launch_2d_l<48, float, 4>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, s);
       case 64: return launch_2d_l<64, float, 4>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, s);
       case 96: return launch_2d_l<96, float, 4>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, s);
-      case 128: return launch_2d_l<128, float, 4>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, s);
+      case 128: return launch_2d_l<128, float, 8>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, s);
     }
   }
   return hipErrorInvalidValue;

@@ -44,8 +44,8 @@ struct Cfg2 {
   static constexpr int NT = TPI * IPB;                    // threads per block
   static constexpr int LD = L + 1;                        // LDS row stride (complex)
   static constexpr int NW = (TPI + 63) / 64;              // waves per IA
-  static_assert(NW <= 8, "reduction slots hold 8 waves");
-  static_assert(LPL == 1 || LPL == 2 || LPL == 4, "1, 2 or 4 lanes per line");
+  static_assert(NW <= 16, "reduction slots hold 16 waves");
+  static_assert(LPL == 1 || LPL == 2 || LPL == 4 || LPL == 8, "1, 2, 4 or 8 lanes per line");
 };
 
 __device__ __forceinline__ double load_px(const void* img, int dt, int64_t i) {
@@ -163,7 +163,7 @@ __host__ __device__ inline size_t region_bytes_2d(int L, int tsize, int ns0, int
   (void)ss0;
   return ((size_t)L * (L + 1) * 2 * tsize + 15) & ~(size_t)15;
 }
-constexpr int kRedBytes = 128;  // reduction slots per IA
+constexpr int kRedBytes = 256;  // reduction slots per IA (16 values + 16 indices)
 
 // ---------------------------------------------------------------------------
 // The correlation of one IA.  LPL lanes (a "group", q = lane within it) share a
@@ -504,7 +504,7 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT)) void piv2d_kernel(Job j0, co
     if constexpr (C::TPI > 64) {
       __syncthreads();
       T* rv = reinterpret_cast<T*>(red);
-      int32_t* ri = reinterpret_cast<int32_t*>(red + 8);
+      int32_t* ri = reinterpret_cast<int32_t*>(red + 16);
       if ((t & 63) == 0) {
         rv[t >> 6] = bst.v;
         ri[t >> 6] = bst.i;

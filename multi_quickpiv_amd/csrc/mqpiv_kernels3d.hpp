@@ -301,7 +301,7 @@ template <int ALG, typename T>
 __global__ __launch_bounds__(256) void finalize3d_kernel(Job J, Scratch3 sc, int64_t ia_base, int64_t n_ia, int L,
                                                          const double* sat1_img1) {
   constexpr int NT = 256;
-  __shared__ double red[16];
+  __shared__ double red[32];
   const int t = threadIdx.x;
   const int64_t il = blockIdx.x;
   if (il >= n_ia) return;
@@ -389,7 +389,7 @@ __global__ __launch_bounds__(256) void finalize3d_kernel(Job J, Scratch3 sc, int
   }
   {
     T* rv = reinterpret_cast<T*>(red);
-    int32_t* ri = reinterpret_cast<int32_t*>(red + 8);
+    int32_t* ri = reinterpret_cast<int32_t*>(red + 16);
     __syncthreads();
     if ((t & 63) == 0) {
       rv[t >> 6] = bst.v;

@@ -1,0 +1,159 @@
+"""Host-side logic and the C ABI, without a GPU: parameter mirror, exports of
+every symbol include/mqpiv.h declares, error paths the host can force."""
+from __future__ import annotations
+
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import multi_quickpiv_amd as mq
+from multi_quickpiv_amd import _lib
+from multi_quickpiv_amd.params import (
+    FFTCC,
+    NSQECC,
+    ZNCC,
+    _isize,
+    _ovp,
+    _smarg,
+    _ssize,
+    _step,
+    get_vectorfield_size,
+    parseCorr,
+    setPIVParameters,
+    toDims3,
+)
+from multi_quickpiv_amd.piv import host_skip_mask, to_c_params
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+# ------------------------------------------------------------- parameters
+def test_defaults_match_reference():
+    """piv_parameters.jl:68-82."""
+    p = setPIVParameters()
+    assert p.corr_alg == NSQECC()
+    assert p.interSize == (32, 32, 32) and p.searchMargin == (10, 10, 10)
+    assert p.overlap == (0, 0, 0) and p.step is None
+    assert p.multipass == 1 and p.computeSN is False
+    assert p.threshold == -1.0 and p.mask_threshold == -1.0
+    assert p.ndims == 3 and p.ovp_th == 0.5
+    assert p.unpadded and p.good_pad and not p.odd_pad
+
+
+def test_parse_corr():
+    assert parseCorr("FFT") == FFTCC() and parseCorr("zncc") == ZNCC()
+    assert parseCorr("NSQECC") == NSQECC() and parseCorr("anything") == NSQECC()
+    for bad in ("znc", "fftc", "nsqec"):
+        with pytest.raises(NameError):
+            parseCorr(bad)
+
+
+def test_unpadded_good_pad_swap():
+    """The outer constructor passes good_pad into the unpadded slot (Q9)."""
+    p = setPIVParameters(unpadded=True, good_pad=False)
+    assert p.unpadded is False and p.good_pad is True
+
+
+def test_accessors_and_dims():
+    assert toDims3(5) == (5, 5, 5) and toDims3((3, 4)) == (3, 4, 0) and toDims3(None) is None
+    p = setPIVParameters(interSize=(32, 16), searchMargin=(8, 4), overlap=(16, 8))
+    p.ndims = 2
+    assert _isize(p) == (32, 16) and _smarg(p) == (8, 4) and _ovp(p) == (16, 8)
+    assert _ssize(p) == (48, 24)
+    assert _step(p) == (16, 8)  # step = interSize - overlap when step is nothing
+    q = setPIVParameters(interSize=32, step=(10, 12))
+    q.ndims = 2
+    assert _step(q) == (10, 12)
+    assert get_vectorfield_size((256, 256), setPIVParameters(interSize=32, step=16)) == (15, 15)
+
+
+def test_c_params_resolution():
+    p = setPIVParameters(corr_alg="zncc", interSize=32, searchMargin=16, overlap=16, computeSN=True,
+                         threshold=0.5)
+    p.ndims = 2
+    cp = to_c_params(p)
+    assert cp.corr_alg == 1 and cp.ndims == 2
+    assert tuple(cp.inter_size) == (32, 32, 1) and tuple(cp.step) == (16, 16, 1)
+    assert cp.compute_sn == 1 and cp.threshold == 0.5 and cp.precision_mode == _lib.MQPIV_AUTO
+    with pytest.raises(NotImplementedError):
+        bad = setPIVParameters(multipass=2)
+        bad.ndims = 2
+        to_c_params(bad)
+    with pytest.raises(NotImplementedError):
+        bad = setPIVParameters(unpadded=False)
+        bad.ndims = 2
+        to_c_params(bad)
+
+
+def test_size_mismatch_is_assertion_error_like_reference():
+    with pytest.raises(AssertionError, match="PIV inputs need to have the same size."):
+        mq.PIV(np.zeros((64, 64)), np.zeros((64, 65)))
+    with pytest.raises(AssertionError):
+        mq.PIV(np.zeros((64, 64)), np.zeros((64, 64)), np.zeros((64, 63)))
+
+
+def test_host_skip_mask_custom_filter():
+    img = np.zeros((64, 64))
+    img[:32, :32] = 1.0
+    p = setPIVParameters(interSize=32, step=32, threshold=0.5, filtFun=lambda w: float(np.mean(w)))
+    p.ndims = 2
+    sk = host_skip_mask(img, None, p, masked=False)
+    assert sk.tolist() == [0, 1, 1, 1]  # column-major IA order, only the first IA is bright
+    m = np.ones_like(img)
+    m[:, 32:] = 0
+    pm = setPIVParameters(interSize=32, step=32, mask_threshold=0.5)
+    pm.ndims = 2
+    assert host_skip_mask(img, m, pm, masked=True).tolist() == [0, 0, 1, 1]
+
+
+# ------------------------------------------------------------- C ABI
+def _declared_symbols():
+    text = (ROOT / "include" / "mqpiv.h").read_text()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(mqpiv_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    names = _declared_symbols()
+    assert len(names) >= 9
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) <= set(_lib.EXPORTS)
+
+
+def test_vf_size_and_param_errors_without_gpu():
+    lib = _lib.load()
+    cp = _lib.MqpivParams()
+    cp.corr_alg, cp.ndims = 2, 2
+    cp.inter_size[:] = [32, 32, 1]
+    cp.search_margin[:] = [16, 16, 0]
+    cp.step[:] = [16, 16, 1]
+    cp.precision_mode = _lib.MQPIV_AUTO
+    dims = (C.c_int64 * 3)(2048, 2048, 1)
+    vf = (C.c_int64 * 3)()
+    n = C.c_int64()
+    assert lib.mqpiv_vf_size(C.byref(cp), dims, vf, C.byref(n)) == _lib.MQPIV_OK
+    assert tuple(vf) == (127, 127, 1) and n.value == 16129
+    cp.ndims = 4
+    assert lib.mqpiv_vf_size(C.byref(cp), dims, vf, C.byref(n)) == _lib.MQPIV_ERR_BAD_PARAMS
+    cp.ndims, cp.corr_alg = 2, 9
+    assert lib.mqpiv_vf_size(C.byref(cp), dims, vf, C.byref(n)) == _lib.MQPIV_ERR_BAD_PARAMS
+    cp.corr_alg, cp.precision_mode = 2, 1  # the split-precision mode is not built
+    assert lib.mqpiv_vf_size(C.byref(cp), dims, vf, C.byref(n)) == _lib.MQPIV_ERR_UNSUPPORTED
+    cp.precision_mode = _lib.MQPIV_AUTO
+    cp.step[0] = 0
+    assert lib.mqpiv_vf_size(C.byref(cp), dims, vf, C.byref(n)) == _lib.MQPIV_ERR_BAD_PARAMS
+    assert lib.mqpiv_vf_size(None, dims, vf, C.byref(n)) == _lib.MQPIV_ERR_NULL
+
+
+def test_null_context_is_rejected():
+    lib = _lib.load()
+    cp = _lib.MqpivParams()
+    dims = (C.c_int64 * 3)(64, 64, 1)
+    assert lib.mqpiv_piv(None, C.byref(cp), dims, 0, None, None, None, None, 0, None, None) == _lib.MQPIV_ERR_NULL
+    assert lib.mqpiv_ctx_destroy(None) == _lib.MQPIV_ERR_NULL
+    assert lib.mqpiv_profile_begin(None) == _lib.MQPIV_ERR_NULL
+    assert lib.mqpiv_version().startswith(b"mqpiv")
