@@ -114,6 +114,37 @@ int mqpiv_piv_device(mqpiv_ctx* ctx, const mqpiv_params* p, const int64_t* dims,
                      const uint8_t* d_skip_mask, int32_t out_dtype, void* d_VF, void* d_SN,
                      void* stream);
 
+/* One frame pair of a batch (device buffers). */
+typedef struct mqpiv_request {
+  mqpiv_params params;
+  int64_t dims[3];
+  int32_t in_dtype;
+  int32_t out_dtype;
+  const void* img1;
+  const void* img2;
+  const void* mask;          /* masked NSQECC only */
+  const uint8_t* skip_mask;  /* nullable */
+  void* VF;
+  void* SN;                  /* nullable unless compute_sn */
+  int32_t* peaks;            /* nullable */
+} mqpiv_request;
+
+/* A batch of frame pairs of any sizes and parameters, device buffers (no
+ * reference counterpart: the reference processes one pair per PIV call).  2-D
+ * pairs that share algorithm, precision and transform length run in ONE kernel
+ * launch over all their IAs; 3-D pairs run one after another.  Enqueued on
+ * `stream` (NULL = the context's), not synchronised.  Results per pair are
+ * identical to separate calls: every IA is computed independently. */
+int mqpiv_piv_batch_device(mqpiv_ctx* ctx, const mqpiv_request* reqs, int32_t n, void* stream);
+
+/* The same 2-D batch captured once into a HIP graph.  The buffers named in reqs
+ * must stay valid for the graph's lifetime; every launch recomputes from their
+ * current contents (the graph owns its integral-array and job-table memory). */
+typedef struct mqpiv_graph mqpiv_graph;
+int mqpiv_graph_create(mqpiv_ctx* ctx, const mqpiv_request* reqs, int32_t n, mqpiv_graph** out);
+int mqpiv_graph_launch(mqpiv_graph* g, void* stream);
+int mqpiv_graph_destroy(mqpiv_graph* g);
+
 /* Measurement hooks (no reference counterpart).  Between profile_begin and
  * profile_end every correlation-kernel launch is bracketed by HIP events on the
  * stream it runs on; profile_end synchronises and returns the durations in

@@ -40,6 +40,10 @@ EXPORTS = (
     "mqpiv_version",
     "mqpiv_profile_begin",
     "mqpiv_profile_end",
+    "mqpiv_piv_batch_device",
+    "mqpiv_graph_create",
+    "mqpiv_graph_launch",
+    "mqpiv_graph_destroy",
 )
 
 
@@ -54,6 +58,22 @@ class MqpivParams(C.Structure):
         ("precision_mode", C.c_int32),
         ("threshold", C.c_double),
         ("mask_threshold", C.c_double),
+    ]
+
+
+class MqpivRequest(C.Structure):
+    _fields_ = [
+        ("params", MqpivParams),
+        ("dims", C.c_int64 * 3),
+        ("in_dtype", C.c_int32),
+        ("out_dtype", C.c_int32),
+        ("img1", C.c_void_p),
+        ("img2", C.c_void_p),
+        ("mask", C.c_void_p),
+        ("skip_mask", C.c_void_p),
+        ("VF", C.c_void_p),
+        ("SN", C.c_void_p),
+        ("peaks", C.c_void_p),
     ]
 
 
@@ -99,6 +119,14 @@ def load() -> C.CDLL:
         lib.mqpiv_piv_device.restype = C.c_int
         lib.mqpiv_piv_device.argtypes = [vp, C.POINTER(MqpivParams), C.POINTER(C.c_int64), C.c_int32,
                                          vp, vp, vp, vp, C.c_int32, vp, vp, vp]
+        lib.mqpiv_piv_batch_device.restype = C.c_int
+        lib.mqpiv_piv_batch_device.argtypes = [vp, C.POINTER(MqpivRequest), C.c_int32, vp]
+        lib.mqpiv_graph_create.restype = C.c_int
+        lib.mqpiv_graph_create.argtypes = [vp, C.POINTER(MqpivRequest), C.c_int32, C.POINTER(vp)]
+        lib.mqpiv_graph_launch.restype = C.c_int
+        lib.mqpiv_graph_launch.argtypes = [vp, vp]
+        lib.mqpiv_graph_destroy.restype = C.c_int
+        lib.mqpiv_graph_destroy.argtypes = [vp]
         lib.mqpiv_profile_begin.restype = C.c_int
         lib.mqpiv_profile_begin.argtypes = [vp]
         lib.mqpiv_profile_end.restype = C.c_int

@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
+#include <tuple>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -49,6 +51,14 @@ struct mqpiv_ctx {
   // FP64 integral arrays of the search image (sum, sum of squares) + scan scratch
   void* d_sat = nullptr;
   size_t sat_cap = 0;
+  // batches: integral arrays of all pairs, device job tables, pinned staging
+  void* d_bsat = nullptr;
+  size_t bsat_cap = 0;
+  void* d_table = nullptr;
+  size_t table_cap = 0;
+  void* h_table = nullptr;
+  size_t h_table_cap = 0;
+  hipEvent_t table_done = nullptr;
   // measurement hooks
   bool profiling = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
@@ -135,6 +145,10 @@ extern "C" int mqpiv_ctx_destroy(mqpiv_ctx* c) {
   if (c->d_stage) hipFree(c->d_stage);
   if (c->d_scratch) hipFree(c->d_scratch);
   if (c->d_sat) hipFree(c->d_sat);
+  if (c->d_bsat) hipFree(c->d_bsat);
+  if (c->d_table) hipFree(c->d_table);
+  if (c->h_table) hipHostFree(c->h_table);
+  if (c->table_done) hipEventDestroy(c->table_done);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return MQPIV_OK;
@@ -256,30 +270,44 @@ static int prepare_sats(mqpiv_ctx* c, Job& j, hipStream_t s) {
   return MQPIV_OK;
 }
 
+// Precision and transform length of a 2-D job (see MQPIV_AUTO in mqpiv.h).
+static int choose_2d(mqpiv_ctx* c, const Job& j, int* fp64_out, int* L_out) {
+  const int ss0 = j.is[0] + 2 * j.sm[0], ss1 = j.is[1] + 2 * j.sm[1];
+  const int ss = std::max(ss0, ss1);
+  const int ns0 = 2 * j.sm[0] + 1, ns1 = 2 * j.sm[1] + 1;
+  if ((int64_t)j.dims[0] * 256 >= ((int64_t)1 << 31))
+    return fail(c, MQPIV_ERR_UNSUPPORTED, "dims[0] = %lld is too large for 32-bit in-window offsets",
+                (long long)j.dims[0]);
+  auto fits = [&](int fp64) {
+    int L = fft_len_2d(fp64, ss);
+    return L > 0 && lds_2d(fp64, L, j.alg, ns0, ns1, ss0) <= 160 * 1024;
+  };
+  int fp64 = 0;
+  if (j.precision_mode == MQPIV_FP64) {
+    if (!fits(1))
+      return fail(c, MQPIV_ERR_UNSUPPORTED, "FP64 transforms support 2-D search sizes up to 96 (got %d)", ss);
+    fp64 = 1;
+  } else if (j.precision_mode == MQPIV_AUTO) {
+    bool want = j.alg == ALG_MASK_NSQECC || (j.alg == ALG_ZNCC && j.compute_sn);
+    fp64 = want && fits(1);
+  }
+  if (!fits(fp64))
+    return fail(c, MQPIV_ERR_UNSUPPORTED, "search size %d exceeds the largest 2-D transform (128)", ss);
+  *fp64_out = fp64;
+  *L_out = fft_len_2d(fp64, ss);
+  return MQPIV_OK;
+}
+
 static int launch_job(mqpiv_ctx* c, Job& j, hipStream_t s) {
   if (j.n_ia == 0) return MQPIV_OK;
-  int src = prepare_sats(c, j, s);
-  if (src) return src;
   if (j.ndims == 2) {
-    const int ss0 = j.is[0] + 2 * j.sm[0], ss1 = j.is[1] + 2 * j.sm[1];
-    const int ss = std::max(ss0, ss1);
+    const int ss0 = j.is[0] + 2 * j.sm[0];
     const int ns0 = 2 * j.sm[0] + 1, ns1 = 2 * j.sm[1] + 1;
-    auto fits = [&](int fp64) {
-      int L = fft_len_2d(fp64, ss);
-      return L > 0 && lds_2d(fp64, L, j.alg, ns0, ns1, ss0) <= 160 * 1024;
-    };
-    int fp64 = 0;
-    if (j.precision_mode == MQPIV_FP64) {
-      if (!fits(1))
-        return fail(c, MQPIV_ERR_UNSUPPORTED, "FP64 transforms support 2-D search sizes up to 96 (got %d)", ss);
-      fp64 = 1;
-    } else if (j.precision_mode == MQPIV_AUTO) {
-      bool want = j.alg == ALG_MASK_NSQECC || (j.alg == ALG_ZNCC && j.compute_sn);
-      fp64 = want && fits(1);
-    }
-    if (!fits(fp64))
-      return fail(c, MQPIV_ERR_UNSUPPORTED, "search size %d exceeds the largest 2-D transform (128)", ss);
-    const int L = fft_len_2d(fp64, ss);
+    int fp64 = 0, L = 0;
+    int rc2 = choose_2d(c, j, &fp64, &L);
+    if (rc2) return rc2;
+    rc2 = prepare_sats(c, j, s);
+    if (rc2) return rc2;
     hipError_t e;
     {
       KernelTimer kt(c, s);
@@ -290,6 +318,8 @@ static int launch_job(mqpiv_ctx* c, Job& j, hipStream_t s) {
   }
   // 3-D
   if (j.alg == ALG_MASK_NSQECC) return fail(c, MQPIV_ERR_UNSUPPORTED, "masked NSQECC is built for 2-D only");
+  int src = prepare_sats(c, j, s);
+  if (src) return src;
   const int ss = std::max({j.is[0] + 2 * j.sm[0], j.is[1] + 2 * j.sm[1], j.is[2] + 2 * j.sm[2]});
   int fp64 = 0;
   if (j.precision_mode == MQPIV_FP64) fp64 = 1;
@@ -420,5 +450,242 @@ extern "C" int mqpiv_profile_end(mqpiv_ctx* c, double* kernel_ms, int32_t cap, i
   }
   c->events.clear();
   if (n) *n = k;
+  return MQPIV_OK;
+}
+
+// ===========================================================================
+// Batches
+
+namespace {
+
+struct BatchGroup {
+  int fp64 = 0, L = 0, alg = 0;
+  std::vector<int> members;  // indices into BatchPlan::jobs2d
+  int64_t n_ia = 0;
+  int ns0cap = 1, ns1cap = 1, ss0cap = 1;
+  size_t table_off = 0;  // byte offset of [Job x m][int64 prefix x (m+1)]
+};
+
+struct BatchPlan {
+  std::vector<Job> jobs2d;
+  std::vector<size_t> sat_off;  // per 2-D job, offset of its integral arrays (or SIZE_MAX)
+  std::vector<int> nsat;
+  std::vector<Job> jobs3d;
+  std::vector<BatchGroup> groups;
+  size_t sat_total = 0, sat_tmp = 0, table_bytes = 0;
+};
+
+size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+}  // namespace
+
+static int plan_batch(mqpiv_ctx* c, const mqpiv_request* r, int n, BatchPlan& P) {
+  if (n < 0 || (n > 0 && !r)) return fail(c, MQPIV_ERR_NULL, "requests is NULL");
+  std::map<std::tuple<int, int, int>, int> key2g;
+  for (int i = 0; i < n; ++i) {
+    Job j;
+    int rc = build_job(c, &r[i].params, r[i].dims, r[i].in_dtype, r[i].img1, r[i].img2, r[i].mask, r[i].skip_mask,
+                       r[i].out_dtype, r[i].VF, r[i].SN, &j);
+    if (rc) return fail(c, rc, "request %d: %s", i, c->err.c_str());
+    j.peaks = r[i].peaks;
+    if (j.n_ia == 0) continue;
+    if (j.ndims == 3) {
+      P.jobs3d.push_back(j);
+      continue;
+    }
+    int fp64 = 0, L = 0;
+    rc = choose_2d(c, j, &fp64, &L);
+    if (rc) return fail(c, rc, "request %d: %s", i, c->err.c_str());
+    const int ns = j.alg == ALG_ZNCC ? 2 : (j.alg == ALG_NSQECC ? 1 : 0);
+    const int idx = (int)P.jobs2d.size();
+    P.jobs2d.push_back(j);
+    P.nsat.push_back(ns);
+    if (ns) {
+      P.sat_off.push_back(P.sat_total);
+      P.sat_total += ns * al256(sat_bytes(j.dims, 2));
+      P.sat_tmp = std::max(P.sat_tmp, al256(sat_tmp_bytes(j.dims, 2)));
+    } else {
+      P.sat_off.push_back(SIZE_MAX);
+    }
+    auto key = std::make_tuple(fp64, L, (int)j.alg);
+    auto it = key2g.find(key);
+    int g;
+    if (it == key2g.end()) {
+      g = (int)P.groups.size();
+      key2g[key] = g;
+      BatchGroup bg;
+      bg.fp64 = fp64;
+      bg.L = L;
+      bg.alg = j.alg;
+      P.groups.push_back(bg);
+    } else {
+      g = it->second;
+    }
+    BatchGroup& bg = P.groups[g];
+    bg.members.push_back(idx);
+    bg.n_ia += j.n_ia;
+    bg.ns0cap = std::max(bg.ns0cap, 2 * j.sm[0] + 1);
+    bg.ns1cap = std::max(bg.ns1cap, 2 * j.sm[1] + 1);
+    bg.ss0cap = std::max(bg.ss0cap, j.is[0] + 2 * j.sm[0]);
+  }
+  for (auto& g : P.groups) {
+    if (lds_2d(g.fp64, g.L, g.alg, g.ns0cap, g.ns1cap, g.ss0cap) > 160 * 1024)
+      return fail(c, MQPIV_ERR_UNSUPPORTED, "batch group (L=%d) does not fit in LDS", g.L);
+    g.table_off = P.table_bytes;
+    P.table_bytes += al256(g.members.size() * sizeof(Job) + (g.members.size() + 1) * sizeof(int64_t));
+  }
+  return MQPIV_OK;
+}
+
+// Fills the job tables (host image of the device tables at d_table) with the
+// integral-array pointers inside sat_base.
+static void fill_tables(BatchPlan& P, char* sat_base, char* host_table) {
+  for (size_t i = 0; i < P.jobs2d.size(); ++i) {
+    Job& j = P.jobs2d[i];
+    if (P.nsat[i]) {
+      const size_t one = al256(sat_bytes(j.dims, 2));
+      j.sat2 = (const double*)(sat_base + P.sat_off[i]);
+      j.sat1 = P.nsat[i] >= 2 ? (const double*)(sat_base + P.sat_off[i] + one) : nullptr;
+    }
+  }
+  for (auto& g : P.groups) {
+    Job* jt = (Job*)(host_table + g.table_off);
+    int64_t* pf = (int64_t*)(host_table + g.table_off + g.members.size() * sizeof(Job));
+    int64_t acc = 0;
+    for (size_t m = 0; m < g.members.size(); ++m) {
+      jt[m] = P.jobs2d[g.members[m]];
+      pf[m] = acc;
+      acc += jt[m].n_ia;
+    }
+    pf[g.members.size()] = acc;
+  }
+}
+
+// Integral arrays + one launch per group (device job tables already in place).
+static int enqueue_2d(mqpiv_ctx* c, BatchPlan& P, char* sat_base, char* d_table, hipStream_t s) {
+  double* tmp = (double*)(sat_base + P.sat_total);
+  for (size_t i = 0; i < P.jobs2d.size(); ++i) {
+    const Job& j = P.jobs2d[i];
+    if (!P.nsat[i]) continue;
+    hipError_t e = build_sat(j.img2, j.in_dtype, 2, j.dims, 2, (double*)j.sat2, tmp, s);
+    if (e == hipSuccess && j.sat1) e = build_sat(j.img2, j.in_dtype, 1, j.dims, 2, (double*)j.sat1, tmp, s);
+    if (e != hipSuccess) return fail(c, MQPIV_ERR_DEVICE, "integral arrays: %s", hipGetErrorString(e));
+  }
+  for (auto& g : P.groups) {
+    const Job* dj = (const Job*)(d_table + g.table_off);
+    const int64_t* dp = (const int64_t*)(d_table + g.table_off + g.members.size() * sizeof(Job));
+    hipError_t e;
+    {
+      KernelTimer kt(c, s);
+      e = launch_piv2d(g.fp64, g.L, g.alg, P.jobs2d[g.members[0]], dj, dp, (int)g.members.size(), g.n_ia,
+                       g.ns0cap, g.ns1cap, g.ss0cap, s);
+    }
+    if (e != hipSuccess) return fail(c, MQPIV_ERR_DEVICE, "batch launch failed: %s", hipGetErrorString(e));
+  }
+  return MQPIV_OK;
+}
+
+extern "C" int mqpiv_piv_batch_device(mqpiv_ctx* c, const mqpiv_request* reqs, int32_t n, void* stream) {
+  if (!c) return MQPIV_ERR_NULL;
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, MQPIV_ERR_DEVICE, "hipSetDevice failed");
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  BatchPlan P;
+  int rc = plan_batch(c, reqs, n, P);
+  if (rc) return rc;
+  if (!P.groups.empty()) {
+    rc = ensure(c, &c->d_bsat, &c->bsat_cap, P.sat_total + P.sat_tmp + 256, s);
+    if (rc) return rc;
+    rc = ensure(c, &c->d_table, &c->table_cap, P.table_bytes + 256, s);
+    if (rc) return rc;
+    // pinned staging for the job tables; the previous batch's copy must be done
+    if (c->table_done) hipEventSynchronize(c->table_done);
+    if (P.table_bytes > c->h_table_cap) {
+      if (c->h_table) hipHostFree(c->h_table);
+      c->h_table = nullptr;
+      c->h_table_cap = 0;
+      HIP_OR_FAIL(c, hipHostMalloc(&c->h_table, P.table_bytes, hipHostMallocDefault));
+      c->h_table_cap = P.table_bytes;
+    }
+    if (!c->table_done) HIP_OR_FAIL(c, hipEventCreateWithFlags(&c->table_done, hipEventDisableTiming));
+    fill_tables(P, (char*)c->d_bsat, (char*)c->h_table);
+    HIP_OR_FAIL(c, hipMemcpyAsync(c->d_table, c->h_table, P.table_bytes, hipMemcpyHostToDevice, s));
+    HIP_OR_FAIL(c, hipEventRecord(c->table_done, s));
+    rc = enqueue_2d(c, P, (char*)c->d_bsat, (char*)c->d_table, s);
+    if (rc) return rc;
+  }
+  for (auto& j : P.jobs3d) {
+    rc = launch_job(c, j, s);
+    if (rc) return rc;
+  }
+  return MQPIV_OK;
+}
+
+struct mqpiv_graph {
+  mqpiv_ctx* ctx = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  void* d_mem = nullptr;  // integral arrays + job tables owned by the graph
+};
+
+extern "C" int mqpiv_graph_create(mqpiv_ctx* c, const mqpiv_request* reqs, int32_t n, mqpiv_graph** out) {
+  if (!c || !out) return MQPIV_ERR_NULL;
+  *out = nullptr;
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, MQPIV_ERR_DEVICE, "hipSetDevice failed");
+  BatchPlan P;
+  int rc = plan_batch(c, reqs, n, P);
+  if (rc) return rc;
+  if (!P.jobs3d.empty()) return fail(c, MQPIV_ERR_UNSUPPORTED, "graph capture covers 2-D pairs only");
+  mqpiv_graph* g = new mqpiv_graph();
+  g->ctx = c;
+  const size_t sat_part = al256(P.sat_total + P.sat_tmp + 256);
+  hipError_t e = hipMalloc(&g->d_mem, sat_part + P.table_bytes + 256);
+  if (e != hipSuccess) {
+    delete g;
+    return fail(c, MQPIV_ERR_OOM, "graph memory: %s", hipGetErrorString(e));
+  }
+  char* sat_base = (char*)g->d_mem;
+  char* d_table = sat_base + sat_part;
+  std::vector<char> host(P.table_bytes + 1);
+  fill_tables(P, sat_base, host.data());
+  e = hipMemcpy(d_table, host.data(), P.table_bytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) {
+    hipFree(g->d_mem);
+    delete g;
+    return fail(c, MQPIV_ERR_DEVICE, "graph setup: %s", hipGetErrorString(e));
+  }
+  const bool prof = c->profiling;
+  c->profiling = false;
+  rc = enqueue_2d(c, P, sat_base, d_table, c->stream);
+  c->profiling = prof;
+  e = hipStreamEndCapture(c->stream, &g->graph);
+  if (rc == MQPIV_OK && e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+  if (rc != MQPIV_OK || e != hipSuccess) {
+    if (g->graph) hipGraphDestroy(g->graph);
+    hipFree(g->d_mem);
+    delete g;
+    return rc ? rc : fail(c, MQPIV_ERR_DEVICE, "graph capture: %s", hipGetErrorString(e));
+  }
+  *out = g;
+  return MQPIV_OK;
+}
+
+extern "C" int mqpiv_graph_launch(mqpiv_graph* g, void* stream) {
+  if (!g) return MQPIV_ERR_NULL;
+  hipStream_t s = stream ? (hipStream_t)stream : g->ctx->stream;
+  hipError_t e = hipGraphLaunch(g->exec, s);
+  if (e != hipSuccess) return fail(g->ctx, MQPIV_ERR_DEVICE, "hipGraphLaunch: %s", hipGetErrorString(e));
+  return MQPIV_OK;
+}
+
+extern "C" int mqpiv_graph_destroy(mqpiv_graph* g) {
+  if (!g) return MQPIV_ERR_NULL;
+  hipDeviceSynchronize();
+  if (g->exec) hipGraphExecDestroy(g->exec);
+  if (g->graph) hipGraphDestroy(g->graph);
+  if (g->d_mem) hipFree(g->d_mem);
+  delete g;
   return MQPIV_OK;
 }

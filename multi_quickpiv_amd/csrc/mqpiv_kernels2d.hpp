@@ -265,7 +265,8 @@ __device__ __forceinline__ double sat_box(const double* __restrict__ S, int64_t 
 template <int L, int ALG, typename T, int LPL>
 __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT)) void piv2d_kernel(Job j0, const Job* __restrict__ jobs,
                                                                       const int64_t* __restrict__ prefix,
-                                                                      int n_jobs, int64_t n_ia_total) {
+                                                                      int n_jobs, int64_t n_ia_total, int ns0_cap,
+                                                                      int ns1_cap) {
   using C = Cfg2<L, T, LPL>;
   constexpr int H = C::H;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -291,8 +292,10 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT)) void piv2d_kernel(Job j0, co
   const int ns0 = 2 * sm0 + 1, ns1 = 2 * sm1 + 1;
   const int ss0 = is0 + 2 * sm0;
 
-  const size_t zbytes = region_bytes_2d(L, sizeof(T), ns0, ns1, ss0);
-  const size_t c1bytes = (ALG == ALG_MASK_NSQECC) ? sim_bytes(ns0, ns1, sizeof(T)) : 0;
+  // LDS layout is uniform across the launch (jobs of a batch may differ in
+  // their margins): the host passes the largest ns0, ns1 of the launch
+  const size_t zbytes = region_bytes_2d(L, sizeof(T), ns0_cap, ns1_cap, ss0);
+  const size_t c1bytes = (ALG == ALG_MASK_NSQECC) ? sim_bytes(ns0_cap, ns1_cap, sizeof(T)) : 0;
   const size_t per_ia = zbytes + c1bytes;
   unsigned char* base = smem + (size_t)sub * per_ia;
   cpx<T>* Z = reinterpret_cast<cpx<T>*>(base);
@@ -614,7 +617,7 @@ hipError_t launch_2d_t(const Job& j0, const Job* d_jobs, const int64_t* d_prefix
   }
   int64_t nb = (n_ia_total + C::IPB - 1) / C::IPB;
   hipLaunchKernelGGL((piv2d_kernel<L, ALG, T, LPL>), dim3((unsigned)nb), dim3(C::NT), lds, s, j0,
-                     d_jobs, d_prefix, n_jobs, n_ia_total);
+                     d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1);
   return hipGetLastError();
 }
 

@@ -171,3 +171,96 @@ def piv_device(ctx: "_lib.Context", pivparams: PIVParameters, shape: Tuple[int, 
     rc = ctx.lib.mqpiv_piv_device(ctx.handle, C.byref(cp), dims, in_dtype, d_img1, d_img2, d_mask,
                                   None, out_dtype, d_vf, d_sn, stream)
     ctx.check(rc)
+
+
+class DeviceRequest:
+    """One frame pair resident on the device for batched / graph execution.
+
+    ``img1``, ``img2`` (and ``mask``) are torch tensors of the frame's shape in
+    the reference's axis order; they are stored column-major (dim 1 fastest)
+    once, here.  Outputs ``VF`` (N x nVF..., FP64) and ``SN`` are device tensors
+    in the reference layout, filled by ``piv_batch`` / graph launches.
+    """
+
+    def __init__(self, img1, img2, pivparams: PIVParameters, mask=None, mode: str = "auto",
+                 want_peaks: bool = False):
+        import torch
+
+        if img1.shape != img2.shape or (mask is not None and mask.shape != img1.shape):
+            raise AssertionError("PIV inputs need to have the same size.")
+        N = img1.dim()
+        pivparams.ndims = N
+        self.shape = tuple(img1.shape)
+        dt = torch.float32 if (img1.dtype == torch.float32 and img2.dtype == torch.float32) else torch.float64
+        perm = tuple(reversed(range(N)))
+        self.a = img1.to(dt).permute(*perm).contiguous()
+        self.b = img2.to(dt).permute(*perm).contiguous()
+        self.m = mask.to(dt).permute(*perm).contiguous() if mask is not None else None
+        self.in_code = _lib.MQPIV_F32 if dt == torch.float32 else _lib.MQPIV_F64
+        self.vf_size = get_vectorfield_size(self.shape, pivparams)
+        n = int(np.prod(self.vf_size))
+        dev = img1.device
+        self.VF = torch.zeros(n * N, dtype=torch.float64, device=dev)
+        self.SN = torch.zeros(n, dtype=torch.float64, device=dev) if pivparams.computeSN else None
+        self.PK = torch.zeros(n * N, dtype=torch.int32, device=dev) if want_peaks else None
+        self.params = to_c_params(pivparams, mask_NSQECC.code if mask is not None else None, mode)
+
+    def c_request(self) -> "_lib.MqpivRequest":
+        r = _lib.MqpivRequest()
+        r.params = self.params
+        for d in range(3):
+            r.dims[d] = self.shape[d] if d < len(self.shape) else 1
+        r.in_dtype = self.in_code
+        r.out_dtype = _lib.MQPIV_F64
+        r.img1, r.img2 = self.a.data_ptr(), self.b.data_ptr()
+        r.mask = self.m.data_ptr() if self.m is not None else None
+        r.skip_mask = None
+        r.VF = self.VF.data_ptr()
+        r.SN = self.SN.data_ptr() if self.SN is not None else None
+        r.peaks = self.PK.data_ptr() if self.PK is not None else None
+        return r
+
+    def results(self):
+        """(VF, SN[, peaks]) as numpy arrays in the reference layout."""
+        N = len(self.shape)
+        vf = tuple(self.vf_size)
+        VF = self.VF.cpu().numpy().reshape((N, *vf), order="F")
+        SN = self.SN.cpu().numpy().reshape(vf, order="F") if self.SN is not None else None
+        if self.PK is not None:
+            return VF, SN, self.PK.cpu().numpy().reshape((N, *vf), order="F")
+        return VF, SN
+
+
+def piv_batch(reqs, ctx=None, stream: Optional[int] = None) -> None:
+    """All requests in one batched call (one kernel launch per algorithm /
+    precision / transform-length group); asynchronous on ``stream``."""
+    ctx = ctx or _lib.context(reqs[0].a.device.index or 0)
+    arr = (_lib.MqpivRequest * len(reqs))(*[r.c_request() for r in reqs])
+    ctx.check(ctx.lib.mqpiv_piv_batch_device(ctx.handle, arr, len(reqs), stream))
+
+
+class BatchGraph:
+    """A batch captured once into a HIP graph; ``launch`` replays it on the
+    requests' current buffer contents."""
+
+    def __init__(self, reqs, ctx=None):
+        self.ctx = ctx or _lib.context(reqs[0].a.device.index or 0)
+        self.reqs = list(reqs)  # keep the buffers alive
+        arr = (_lib.MqpivRequest * len(reqs))(*[r.c_request() for r in reqs])
+        h = C.c_void_p()
+        self.ctx.check(self.ctx.lib.mqpiv_graph_create(self.ctx.handle, arr, len(reqs), C.byref(h)))
+        self.handle = h
+
+    def launch(self, stream: Optional[int] = None) -> None:
+        self.ctx.check(self.ctx.lib.mqpiv_graph_launch(self.handle, stream))
+
+    def close(self) -> None:
+        if self.handle:
+            self.ctx.lib.mqpiv_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
