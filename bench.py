@@ -15,7 +15,14 @@ time.  The CPU baseline is the Float64 oracle's FFT mode (a restatement of the
 reference's PIV_CPU loop with per-IA transforms) over the same four runs,
 OpenMP over IAs on the host cores.
 
-    python bench.py [--steps K] [--warmup W] [--mode auto|fp32|fp64] [--no-cpu]
+A serving leg follows ("serving"): a Poisson stream (200 pairs/s) of mixed 2-D
+and 3-D requests over frames resident in HBM, served by the continuous-batching
+server and by the plain per-request path on the same schedule, then the same
+requests all at once (saturation); p50/p99 latency and throughput of both, and
+a bit-for-bit comparison of their outputs.
+
+    python bench.py [--steps K] [--warmup W] [--mode auto|fp32|fp64] [--no-cpu] [--no-serve]
+    python bench.py --serve [--serve-requests N] [--serve-rate R]    (serving leg only)
 """
 from __future__ import annotations
 
@@ -59,6 +66,176 @@ def uses_fp64(mode: str, alg: str, compute_sn: bool, ss: int) -> bool:
     return False
 
 
+class _ResidentRequest:
+    """A request over frames shared in HBM (column-major), with its own outputs."""
+
+    def __init__(self, frame, p, mode, out_sets=1):
+        import torch
+
+        from multi_quickpiv_amd import _lib
+        from multi_quickpiv_amd.params import get_vectorfield_size
+        from multi_quickpiv_amd.piv import to_c_params
+
+        self.frame, self.p = frame, p
+        shape = frame["shape"]
+        p.ndims = len(shape)
+        self.vf = get_vectorfield_size(shape, p)
+        self.n = int(np.prod(self.vf))
+        self.params = to_c_params(p, None, mode)
+        self.outs = [(torch.zeros(self.n * len(shape), dtype=torch.float64, device="cuda"),
+                      torch.zeros(self.n, dtype=torch.float64, device="cuda") if p.computeSN else None)
+                     for _ in range(out_sets)]
+        self._lib = _lib
+
+    def c_request(self, k=0):
+        L = self._lib
+        r = L.MqpivRequest()
+        r.params = self.params
+        shape = self.frame["shape"]
+        for d in range(3):
+            r.dims[d] = shape[d] if d < len(shape) else 1
+        r.in_dtype, r.out_dtype = L.MQPIV_F32, L.MQPIV_F64
+        r.img1, r.img2 = self.frame["A"].data_ptr(), self.frame["B"].data_ptr()
+        r.mask = r.skip_mask = r.peaks = None
+        VF, SN = self.outs[k]
+        r.VF = VF.data_ptr()
+        r.SN = SN.data_ptr() if SN is not None else None
+        return r
+
+
+def serve_bench(n_requests: int, rates, seed: int, n_slots: int = 4, max_batch: int = 32,
+                budget_gib: int = 64, mode: str = "auto", log=None) -> dict:
+    """Poisson stream of mixed 2-D / 3-D requests over frames resident in HBM,
+    served by the continuous-batching server and, on the same arrival schedule,
+    by the plain per-request path (one pair at a time, in arrival order, each
+    call synchronised), for each arrival rate in ``rates`` (0 = every request
+    at once).  Returns latency percentiles (arrival to results in HBM) and
+    throughput of both, and whether every served output equals the plain
+    path's bit for bit."""
+    import threading
+
+    import torch
+
+    from multi_quickpiv_amd import _lib, setPIVParameters, synth
+    from multi_quickpiv_amd.piv import Server
+
+    rng = np.random.default_rng(seed)
+    frames = []
+    # 2-D: 512^2 .. 4096^2 at 0.05 particles/px, Lamb-Oseen (core radius and
+    # peak scaled with the frame) + drift; 3-D: 128^3 .. 512x512x128 at 0.01
+    # particles/voxel, shear + drift.  Two frame pairs per size, seeded.
+    for k, n in enumerate([512, 1024, 2048, 4096] * 2):
+        sh = (n, n)
+        a, b = synth.particle_pair(sh, synth.n_particles(sh, 0.05), 1.2,
+                                   synth.lamb_oseen(sh, core_radius=300.0 * n / 2048, peak=8.0), 7000 + k,
+                                   device="cuda", dtype=torch.float32)
+        frames.append({"shape": sh, "A": a.t().contiguous(), "B": b.t().contiguous(), "nd": 2})
+    for k, sh in enumerate([(128, 128, 128), (256, 256, 128), (512, 512, 128)] * 2):
+        a, b = synth.particle_pair(sh, synth.n_particles(sh, 0.01), 1.5, synth.shear3d(), 8000 + k,
+                                   device="cuda", dtype=torch.float32)
+        frames.append({"shape": sh, "A": a.permute(2, 1, 0).contiguous(), "B": b.permute(2, 1, 0).contiguous(),
+                       "nd": 3})
+    del a, b
+    f2 = [f for f in frames if f["nd"] == 2]
+    f3 = [f for f in frames if f["nd"] == 3]
+    algs = ["fft", "zncc", "nsqecc"]
+    reqs = []
+    for i in range(n_requests):
+        if rng.random() < 0.7:
+            fr = f2[rng.integers(len(f2))]
+            isz = int(rng.choice([16, 32, 64]))
+            p = setPIVParameters(corr_alg=algs[rng.integers(3)], interSize=isz, searchMargin=isz // 2,
+                                 step=isz // 2)
+        else:
+            fr = f3[rng.integers(len(f3))]
+            p = setPIVParameters(corr_alg=algs[rng.integers(3)], interSize=32, searchMargin=8, step=16)
+        reqs.append(_ResidentRequest(fr, p, mode, out_sets=2))
+    gaps = rng.exponential(1.0, n_requests)
+    torch.cuda.synchronize()
+
+    class _Sub:  # submit_device wants an object with c_request()
+        def __init__(self, r, k):
+            self.r, self.k = r, k
+
+        def c_request(self):
+            return self.r.c_request(self.k)
+
+    big3 = max((r for r in reqs if len(r.frame["shape"]) == 3), key=lambda r: r.n)
+    big2 = max((r for r in reqs if len(r.frame["shape"]) == 2), key=lambda r: r.n)
+    warm = [big3] * n_slots + [big2] * n_slots  # admitted round-robin: one of each per slot
+    sv = Server(0, budget_gib << 30, n_slots, max_batch)
+    for r in warm:  # warm up every slot (3-D transform scratch, job tables, code objects)
+        sv.wait(sv.submit_device(_Sub(r, 0)))
+    ctx = _lib.context(0)
+    s = torch.cuda.Stream()
+
+    def plain_call(r):
+        ctx.check(ctx.lib.mqpiv_piv_batch_device(ctx.handle, (_lib.MqpivRequest * 1)(r.c_request(1)), 1,
+                                                 s.cuda_stream))
+
+    for r in warm:
+        plain_call(r)
+    s.synchronize()
+    n_ia = sum(r.n for r in reqs)
+    out = {"requests": n_requests, "ia_total": n_ia, "slots": n_slots, "max_batch": max_batch,
+           "mix": "70% 2-D (512^2-4096^2, FFTCC/ZNCC/NSQECC, IA 16/32/64, margin = step = IA/2), "
+                  "30% 3-D (128^3-512x512x128, IA 32^3, margin 8, step 16); frames resident in HBM "
+                  "(8 2-D and 6 3-D seeded pairs), parameters drawn per request"}
+    for rate in rates:
+        arrivals = np.cumsum(gaps) / rate if rate > 0 else np.zeros(n_requests)
+        # ---- served: a submitter thread follows the arrival schedule
+        sv.reset_stats()
+        tickets = [None] * n_requests
+        t0 = time.perf_counter() + 0.05
+
+        def submitter():
+            for i, r in enumerate(reqs):
+                dt = t0 + arrivals[i] - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                tickets[i] = sv.submit_device(_Sub(r, 0))
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        th.join()
+        for t in tickets:
+            sv.wait(t)
+        st = sv.stats()
+        # ---- plain per-request path on the same schedule
+        lat = []
+        t0 = time.perf_counter() + 0.05
+        t_end = t0
+        for i, r in enumerate(reqs):
+            dt = t0 + arrivals[i] - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            plain_call(r)
+            s.synchronize()
+            t_end = time.perf_counter()
+            lat.append(t_end - (t0 + arrivals[i]))
+        lat = np.array(lat) * 1e3
+        plain_span = t_end - t0
+        same = all(torch.equal(r.outs[0][0], r.outs[1][0]) and
+                   (r.outs[0][1] is None or torch.equal(r.outs[0][1], r.outs[1][1])) for r in reqs)
+        key = f"rate_{int(rate)}" if rate > 0 else "burst"
+        out[key] = {
+            "arrivals": f"Poisson, {rate:g} pairs/s" if rate > 0 else "all at t=0 (saturation)",
+            "served": {"p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
+                       "max_ms": round(st["max_ms"], 3), "pairs_per_s": round(st["pairs_per_s"], 1),
+                       "ia_per_s": round(st["ia_per_s"], 1)},
+            "plain": {"p50_ms": round(float(np.percentile(lat, 50)), 3),
+                      "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                      "max_ms": round(float(lat.max()), 3),
+                      "pairs_per_s": round(n_requests / plain_span, 1),
+                      "ia_per_s": round(n_ia / plain_span, 1)},
+            "identical_outputs": bool(same),
+        }
+        if log:
+            log(f"{key}: {out[key]}")
+    sv.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,6 +247,13 @@ def main() -> int:
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one 256 MiB device-to-device copy (scale reference for the "
                          "FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--serve", action="store_true",
+                    help="run only the serving benchmark (Poisson stream of mixed requests)")
+    ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg of the default run")
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-batch", type=int, default=32)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
@@ -81,6 +265,16 @@ def main() -> int:
     from multi_quickpiv_amd import _lib, setPIVParameters, synth
     from multi_quickpiv_amd.params import get_vectorfield_size
     from multi_quickpiv_amd.piv import piv_device
+
+    if args.serve:
+        torch.cuda.set_device(0)
+        res = serve_bench(args.serve_requests, [args.serve_rate, 0.0], 11, args.serve_slots, args.serve_batch,
+                          mode=args.mode, log=lambda m: print(m, file=sys.stderr, flush=True))
+        print(json.dumps({"metric": "served frame pairs/s with p50/p99 latency",
+                          "value": res[f"rate_{int(args.serve_rate)}"]["served"]["pairs_per_s"],
+                          "unit": "pairs/s", "n_gpus": 1, "higher_is_better": True, "data": "synthetic",
+                          "config": {"workload": "serving"}, "serving": res}))
+        return 0
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
@@ -175,6 +369,11 @@ def main() -> int:
                           "call_ms": round(call_ms[k], 4),
                           "ia_per_s": round(j["n"] / (kern_ms[k] * 1e-3), 1)})
 
+    serving = None
+    if not args.no_serve:
+        serving = serve_bench(args.serve_requests, [args.serve_rate, 0.0], 11, args.serve_slots,
+                              args.serve_batch, mode=args.mode)
+
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as O
@@ -226,6 +425,7 @@ def main() -> int:
                      "kernel_ms": round(kern_ms[kd], 4)},
         "breakdown": breakdown,
         "outputs_finite": finite,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

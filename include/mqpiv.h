@@ -152,6 +152,46 @@ int mqpiv_graph_destroy(mqpiv_graph* g);
 int mqpiv_profile_begin(mqpiv_ctx* ctx);
 int mqpiv_profile_end(mqpiv_ctx* ctx, double* kernel_ms, int32_t cap, int32_t* n);
 
+/* Continuous-batching request server (no reference counterpart: the reference
+ * runs one PIV call at a time on the caller's thread).  A worker thread admits
+ * submitted pairs, in arrival order, into one of n_slots execution slots as soon
+ * as a slot is free and the pair's device footprint fits in the
+ * hbm_budget_bytes pool (up to max_batch pairs per slot).  The footprint is the
+ * pair's FP64 integral arrays plus, for host requests, its staged inputs and
+ * outputs, each rounded to 256 B; every slot also keeps its own context (3-D
+ * transform scratch, at most MQPIV_3D_CHUNK_BYTES, 256 MiB by default) outside
+ * the pool.  Each slot copies its host pairs in, runs all its pairs as one
+ * batched call (as mqpiv_piv_batch_device) on its own stream, and copies
+ * VF/SN/peaks back, so batches of different slots overlap on the device.
+ *   mqpiv_server_submit: HOST pointers, valid until mqpiv_server_wait reports
+ *     completion (pinned memory lets copies overlap kernels);
+ *   mqpiv_server_submit_device: DEVICE pointers (frames resident in HBM); the
+ *     outputs are in device memory at completion.
+ * skip_mask is not accepted (MQPIV_ERR_UNSUPPORTED); a pair larger than the
+ * whole pool is rejected with MQPIV_ERR_OOM at submit.  Results are identical
+ * to a lone mqpiv_piv call: every IA is computed independently. */
+typedef struct mqpiv_server mqpiv_server;
+typedef struct {
+  uint64_t completed; /* pairs finished OK since create / reset_stats */
+  int64_t ia_done;
+  uint64_t pending;   /* submitted, not yet admitted */
+  double p50_ms, p99_ms, max_ms; /* submit -> results on host */
+  double pairs_per_s, ia_per_s;  /* over first submit .. last completion */
+} mqpiv_server_stats;
+int mqpiv_server_create(int device, uint64_t hbm_budget_bytes, int32_t n_slots, int32_t max_batch,
+                        mqpiv_server** out);
+int mqpiv_server_submit(mqpiv_server* sv, const mqpiv_request* req, uint64_t* ticket);
+int mqpiv_server_submit_device(mqpiv_server* sv, const mqpiv_request* req, uint64_t* ticket);
+/* Waits up to timeout_s (< 0: forever); *status = the pair's MQPIV_* code, or -1
+ * if still running (the ticket then stays valid). */
+int mqpiv_server_wait(mqpiv_server* sv, uint64_t ticket, double timeout_s, int32_t* status);
+int mqpiv_server_stats_get(mqpiv_server* sv, mqpiv_server_stats* st);
+/* Message of the last failed batch (a batch fails as a whole: every pair in it
+ * reports the same status).  Valid until the next call on this thread. */
+const char* mqpiv_server_last_error(mqpiv_server* sv);
+int mqpiv_server_reset_stats(mqpiv_server* sv);
+int mqpiv_server_destroy(mqpiv_server* sv);
+
 /* Library build identification (for logs). */
 const char* mqpiv_version(void);
 

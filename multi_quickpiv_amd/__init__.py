@@ -27,11 +27,15 @@ from .params import (  # noqa: F401
     setPIVParameters,
     toDims3,
 )
-from .piv import PIV, piv_device  # noqa: F401
+from .piv import PIV, BatchGraph, DeviceRequest, Server, piv_batch, piv_device  # noqa: F401
 
 __all__ = [
     "PIV",
     "piv_device",
+    "piv_batch",
+    "DeviceRequest",
+    "BatchGraph",
+    "Server",
     "setPIVParameters",
     "PIVParameters",
     "FFTCC",

@@ -44,6 +44,14 @@ EXPORTS = (
     "mqpiv_graph_create",
     "mqpiv_graph_launch",
     "mqpiv_graph_destroy",
+    "mqpiv_server_create",
+    "mqpiv_server_submit",
+    "mqpiv_server_submit_device",
+    "mqpiv_server_wait",
+    "mqpiv_server_stats_get",
+    "mqpiv_server_reset_stats",
+    "mqpiv_server_last_error",
+    "mqpiv_server_destroy",
 )
 
 
@@ -74,6 +82,19 @@ class MqpivRequest(C.Structure):
         ("VF", C.c_void_p),
         ("SN", C.c_void_p),
         ("peaks", C.c_void_p),
+    ]
+
+
+class MqpivServerStats(C.Structure):
+    _fields_ = [
+        ("completed", C.c_uint64),
+        ("ia_done", C.c_int64),
+        ("pending", C.c_uint64),
+        ("p50_ms", C.c_double),
+        ("p99_ms", C.c_double),
+        ("max_ms", C.c_double),
+        ("pairs_per_s", C.c_double),
+        ("ia_per_s", C.c_double),
     ]
 
 
@@ -131,6 +152,22 @@ def load() -> C.CDLL:
         lib.mqpiv_profile_begin.argtypes = [vp]
         lib.mqpiv_profile_end.restype = C.c_int
         lib.mqpiv_profile_end.argtypes = [vp, C.POINTER(C.c_double), C.c_int32, C.POINTER(C.c_int32)]
+        lib.mqpiv_server_create.restype = C.c_int
+        lib.mqpiv_server_create.argtypes = [C.c_int, C.c_uint64, C.c_int32, C.c_int32, C.POINTER(vp)]
+        lib.mqpiv_server_submit.restype = C.c_int
+        lib.mqpiv_server_submit.argtypes = [vp, C.POINTER(MqpivRequest), C.POINTER(C.c_uint64)]
+        lib.mqpiv_server_submit_device.restype = C.c_int
+        lib.mqpiv_server_submit_device.argtypes = [vp, C.POINTER(MqpivRequest), C.POINTER(C.c_uint64)]
+        lib.mqpiv_server_wait.restype = C.c_int
+        lib.mqpiv_server_wait.argtypes = [vp, C.c_uint64, C.c_double, C.POINTER(C.c_int32)]
+        lib.mqpiv_server_stats_get.restype = C.c_int
+        lib.mqpiv_server_stats_get.argtypes = [vp, C.POINTER(MqpivServerStats)]
+        lib.mqpiv_server_last_error.restype = C.c_char_p
+        lib.mqpiv_server_last_error.argtypes = [vp]
+        lib.mqpiv_server_reset_stats.restype = C.c_int
+        lib.mqpiv_server_reset_stats.argtypes = [vp]
+        lib.mqpiv_server_destroy.restype = C.c_int
+        lib.mqpiv_server_destroy.argtypes = [vp]
         _lib = lib
         return lib
 

@@ -58,6 +58,7 @@ SOURCES = [
     "mqpiv_kernels3d.hip",
     "mqpiv_sat.hip",
     "mqpiv_api.cpp",
+    "mqpiv_server.cpp",
 ]
 
 

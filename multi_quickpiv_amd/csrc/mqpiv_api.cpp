@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../../include/mqpiv.h"
+#include "mqpiv_batch.h"
 #include "mqpiv_internal.h"
 
 namespace mqpiv {
@@ -249,15 +250,34 @@ static int build_job(mqpiv_ctx* c, const mqpiv_params* p, const int64_t* dims, i
 
 // Integral arrays the normalisation reads: NSQECC needs sum(G^2), ZNCC also sum(G).
 // (3-D ZNCC also reads the means from integral arrays: sum(F) of image 1.)
-static int prepare_sats(mqpiv_ctx* c, Job& j, hipStream_t s) {
+// Integral arrays a job needs: count, bytes of one (256-aligned) and of the
+// prefix-scan temporary.
+static int sat_count(int alg, int ndims) {
+  return alg == ALG_ZNCC ? (ndims == 3 ? 3 : 2) : (alg == ALG_NSQECC ? 1 : 0);
+}
+
+size_t request_ws_bytes(const mqpiv_request& r) {
+  const int nd = r.params.ndims;
+  const int nsat = sat_count(r.params.corr_alg, nd);
+  if (!nsat) return 0;
+  const size_t one = (sat_bytes(r.dims, nd) + 255) & ~(size_t)255;
+  const size_t tmp = (sat_tmp_bytes(r.dims, nd) + 255) & ~(size_t)255;
+  return nsat * one + tmp;
+}
+
+// Builds the job's integral arrays in ws (request_ws_bytes of it) or, when ws is
+// NULL, in the context's grow-only buffer.
+static int prepare_sats(mqpiv_ctx* c, Job& j, hipStream_t s, char* ws = nullptr) {
   j.sat1 = j.sat2 = j.sat1f = nullptr;
   if (j.alg != ALG_ZNCC && j.alg != ALG_NSQECC) return MQPIV_OK;
   const size_t one = (sat_bytes(j.dims, j.ndims) + 255) & ~(size_t)255;
   const size_t tmp = (sat_tmp_bytes(j.dims, j.ndims) + 255) & ~(size_t)255;
-  const int nsat = j.alg == ALG_ZNCC ? (j.ndims == 3 ? 3 : 2) : 1;
-  int rc = ensure(c, &c->d_sat, &c->sat_cap, nsat * one + tmp, s);
-  if (rc) return rc;
-  char* b = (char*)c->d_sat;
+  const int nsat = sat_count(j.alg, j.ndims);
+  if (!ws) {
+    int rc = ensure(c, &c->d_sat, &c->sat_cap, nsat * one + tmp, s);
+    if (rc) return rc;
+  }
+  char* b = ws ? ws : (char*)c->d_sat;
   double* t = (double*)(b + nsat * one);
   hipError_t e = build_sat(j.img2, j.in_dtype, 2, j.dims, j.ndims, (double*)b, t, s);
   if (e == hipSuccess && nsat >= 2) e = build_sat(j.img2, j.in_dtype, 1, j.dims, j.ndims, (double*)(b + one), t, s);
@@ -298,7 +318,7 @@ static int choose_2d(mqpiv_ctx* c, const Job& j, int* fp64_out, int* L_out) {
   return MQPIV_OK;
 }
 
-static int launch_job(mqpiv_ctx* c, Job& j, hipStream_t s) {
+static int launch_job(mqpiv_ctx* c, Job& j, hipStream_t s, char* ws = nullptr) {
   if (j.n_ia == 0) return MQPIV_OK;
   if (j.ndims == 2) {
     const int ss0 = j.is[0] + 2 * j.sm[0];
@@ -306,7 +326,7 @@ static int launch_job(mqpiv_ctx* c, Job& j, hipStream_t s) {
     int fp64 = 0, L = 0;
     int rc2 = choose_2d(c, j, &fp64, &L);
     if (rc2) return rc2;
-    rc2 = prepare_sats(c, j, s);
+    rc2 = prepare_sats(c, j, s, ws);
     if (rc2) return rc2;
     hipError_t e;
     {
@@ -318,7 +338,7 @@ static int launch_job(mqpiv_ctx* c, Job& j, hipStream_t s) {
   }
   // 3-D
   if (j.alg == ALG_MASK_NSQECC) return fail(c, MQPIV_ERR_UNSUPPORTED, "masked NSQECC is built for 2-D only");
-  int src = prepare_sats(c, j, s);
+  int src = prepare_sats(c, j, s, ws);
   if (src) return src;
   const int ss = std::max({j.is[0] + 2 * j.sm[0], j.is[1] + 2 * j.sm[1], j.is[2] + 2 * j.sm[2]});
   int fp64 = 0;
@@ -470,6 +490,7 @@ struct BatchPlan {
   std::vector<Job> jobs2d;
   std::vector<size_t> sat_off;  // per 2-D job, offset of its integral arrays (or SIZE_MAX)
   std::vector<int> nsat;
+  std::vector<int> req2d, req3d;  // request index of each 2-D / 3-D job
   std::vector<Job> jobs3d;
   std::vector<BatchGroup> groups;
   size_t sat_total = 0, sat_tmp = 0, table_bytes = 0;
@@ -491,6 +512,7 @@ static int plan_batch(mqpiv_ctx* c, const mqpiv_request* r, int n, BatchPlan& P)
     if (j.n_ia == 0) continue;
     if (j.ndims == 3) {
       P.jobs3d.push_back(j);
+      P.req3d.push_back(i);
       continue;
     }
     int fp64 = 0, L = 0;
@@ -499,6 +521,7 @@ static int plan_batch(mqpiv_ctx* c, const mqpiv_request* r, int n, BatchPlan& P)
     const int ns = j.alg == ALG_ZNCC ? 2 : (j.alg == ALG_NSQECC ? 1 : 0);
     const int idx = (int)P.jobs2d.size();
     P.jobs2d.push_back(j);
+    P.req2d.push_back(i);
     P.nsat.push_back(ns);
     if (ns) {
       P.sat_off.push_back(P.sat_total);
@@ -538,14 +561,16 @@ static int plan_batch(mqpiv_ctx* c, const mqpiv_request* r, int n, BatchPlan& P)
 }
 
 // Fills the job tables (host image of the device tables at d_table) with the
-// integral-array pointers inside sat_base.
-static void fill_tables(BatchPlan& P, char* sat_base, char* host_table) {
+// integral-array pointers: inside sat_base, or in each request's own workspace
+// ws[request] when ws is given.
+static void fill_tables(BatchPlan& P, char* sat_base, char* host_table, void* const* ws = nullptr) {
   for (size_t i = 0; i < P.jobs2d.size(); ++i) {
     Job& j = P.jobs2d[i];
     if (P.nsat[i]) {
       const size_t one = al256(sat_bytes(j.dims, 2));
-      j.sat2 = (const double*)(sat_base + P.sat_off[i]);
-      j.sat1 = P.nsat[i] >= 2 ? (const double*)(sat_base + P.sat_off[i] + one) : nullptr;
+      char* b = ws ? (char*)ws[P.req2d[i]] : sat_base + P.sat_off[i];
+      j.sat2 = (const double*)b;
+      j.sat1 = P.nsat[i] >= 2 ? (const double*)(b + one) : nullptr;
     }
   }
   for (auto& g : P.groups) {
@@ -562,11 +587,15 @@ static void fill_tables(BatchPlan& P, char* sat_base, char* host_table) {
 }
 
 // Integral arrays + one launch per group (device job tables already in place).
-static int enqueue_2d(mqpiv_ctx* c, BatchPlan& P, char* sat_base, char* d_table, hipStream_t s) {
-  double* tmp = (double*)(sat_base + P.sat_total);
+// The prefix-scan temporary follows the arrays: at sat_base + sat_total, or in
+// each request's workspace after its own arrays.
+static int enqueue_2d(mqpiv_ctx* c, BatchPlan& P, char* sat_base, char* d_table, hipStream_t s,
+                      bool own_ws = false) {
   for (size_t i = 0; i < P.jobs2d.size(); ++i) {
     const Job& j = P.jobs2d[i];
     if (!P.nsat[i]) continue;
+    double* tmp = own_ws ? (double*)((char*)j.sat2 + P.nsat[i] * al256(sat_bytes(j.dims, 2)))
+                         : (double*)(sat_base + P.sat_total);
     hipError_t e = build_sat(j.img2, j.in_dtype, 2, j.dims, 2, (double*)j.sat2, tmp, s);
     if (e == hipSuccess && j.sat1) e = build_sat(j.img2, j.in_dtype, 1, j.dims, 2, (double*)j.sat1, tmp, s);
     if (e != hipSuccess) return fail(c, MQPIV_ERR_DEVICE, "integral arrays: %s", hipGetErrorString(e));
@@ -585,7 +614,7 @@ static int enqueue_2d(mqpiv_ctx* c, BatchPlan& P, char* sat_base, char* d_table,
   return MQPIV_OK;
 }
 
-extern "C" int mqpiv_piv_batch_device(mqpiv_ctx* c, const mqpiv_request* reqs, int32_t n, void* stream) {
+int batch_device_ws(mqpiv_ctx* c, const mqpiv_request* reqs, int32_t n, void* stream, void* const* ws) {
   if (!c) return MQPIV_ERR_NULL;
   std::lock_guard<std::mutex> lk(c->mu);
   if (hipSetDevice(c->device) != hipSuccess) return fail(c, MQPIV_ERR_DEVICE, "hipSetDevice failed");
@@ -594,8 +623,10 @@ extern "C" int mqpiv_piv_batch_device(mqpiv_ctx* c, const mqpiv_request* reqs, i
   int rc = plan_batch(c, reqs, n, P);
   if (rc) return rc;
   if (!P.groups.empty()) {
-    rc = ensure(c, &c->d_bsat, &c->bsat_cap, P.sat_total + P.sat_tmp + 256, s);
-    if (rc) return rc;
+    if (!ws) {
+      rc = ensure(c, &c->d_bsat, &c->bsat_cap, P.sat_total + P.sat_tmp + 256, s);
+      if (rc) return rc;
+    }
     rc = ensure(c, &c->d_table, &c->table_cap, P.table_bytes + 256, s);
     if (rc) return rc;
     // pinned staging for the job tables; the previous batch's copy must be done
@@ -608,17 +639,21 @@ extern "C" int mqpiv_piv_batch_device(mqpiv_ctx* c, const mqpiv_request* reqs, i
       c->h_table_cap = P.table_bytes;
     }
     if (!c->table_done) HIP_OR_FAIL(c, hipEventCreateWithFlags(&c->table_done, hipEventDisableTiming));
-    fill_tables(P, (char*)c->d_bsat, (char*)c->h_table);
+    fill_tables(P, (char*)c->d_bsat, (char*)c->h_table, ws);
     HIP_OR_FAIL(c, hipMemcpyAsync(c->d_table, c->h_table, P.table_bytes, hipMemcpyHostToDevice, s));
     HIP_OR_FAIL(c, hipEventRecord(c->table_done, s));
-    rc = enqueue_2d(c, P, (char*)c->d_bsat, (char*)c->d_table, s);
+    rc = enqueue_2d(c, P, (char*)c->d_bsat, (char*)c->d_table, s, ws != nullptr);
     if (rc) return rc;
   }
-  for (auto& j : P.jobs3d) {
-    rc = launch_job(c, j, s);
+  for (size_t k = 0; k < P.jobs3d.size(); ++k) {
+    rc = launch_job(c, P.jobs3d[k], s, ws ? (char*)ws[P.req3d[k]] : nullptr);
     if (rc) return rc;
   }
   return MQPIV_OK;
+}
+
+extern "C" int mqpiv_piv_batch_device(mqpiv_ctx* c, const mqpiv_request* reqs, int32_t n, void* stream) {
+  return batch_device_ws(c, reqs, n, stream, nullptr);
 }
 
 struct mqpiv_graph {

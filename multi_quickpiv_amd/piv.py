@@ -264,3 +264,116 @@ class BatchGraph:
             self.close()
         except Exception:
             pass
+
+
+class Server:
+    """Continuous-batching request server (``mqpiv_server_*``): a native worker
+    thread admits submitted pairs into execution slots by device footprint,
+    runs each admitted set as one batched call and copies results back.
+
+    ``submit`` takes host arrays in the reference's axis order; arrays that are
+    already Fortran-ordered (for example views of pinned memory) are used in
+    place and must not be modified until ``wait`` returns.  Outputs are
+    allocated per request unless ``out=(VF, SN)`` supplies them."""
+
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 8 << 30, n_slots: int = 4,
+                 max_batch: int = 64):
+        self.lib = _lib.load()
+        h = C.c_void_p()
+        rc = self.lib.mqpiv_server_create(device, hbm_budget_bytes, n_slots, max_batch, C.byref(h))
+        if rc != _lib.MQPIV_OK:
+            raise _lib.MqpivError(rc, f"mqpiv_server_create(device={device}) failed")
+        self.handle = h
+        self._live: dict = {}
+
+    def submit(self, img1: Any, img2: Any, pivparams: PIVParameters, mask: Any = None, mode: str = "auto",
+               precision: Any = np.float64, out: Optional[Tuple[np.ndarray, Optional[np.ndarray]]] = None,
+               return_peaks: bool = False) -> int:
+        a1, a2 = _as_input(img1), _as_input(img2)
+        if a1.shape != a2.shape or (mask is not None and np.shape(mask) != a1.shape):
+            raise AssertionError("PIV inputs need to have the same size.")
+        N = a1.ndim
+        pivparams.ndims = N
+        dt = _common_dtype(a1, a2, *([] if mask is None else [_as_input(mask)]))
+        a1 = np.asfortranarray(a1, dtype=dt)
+        a2 = np.asfortranarray(a2, dtype=dt)
+        m = np.asfortranarray(_as_input(mask), dtype=dt) if mask is not None else None
+        out_dt = np.dtype(precision)
+        vf = get_vectorfield_size(a1.shape, pivparams)
+        if out is None:
+            VF = np.zeros((N, *vf), dtype=out_dt, order="F")
+            SN = np.zeros(vf, dtype=out_dt, order="F") if pivparams.computeSN else None
+        else:
+            VF, SN = out
+            out_dt = VF.dtype
+        PK = np.zeros((N, *vf), dtype=np.int32, order="F") if return_peaks else None
+        r = _lib.MqpivRequest()
+        r.params = to_c_params(pivparams, mask_NSQECC.code if mask is not None else None, mode)
+        for d in range(3):
+            r.dims[d] = a1.shape[d] if d < N else 1
+        r.in_dtype = _lib.MQPIV_F32 if dt == np.float32 else _lib.MQPIV_F64
+        r.out_dtype = _lib.MQPIV_F32 if out_dt == np.float32 else _lib.MQPIV_F64
+        r.img1, r.img2 = a1.ctypes.data, a2.ctypes.data
+        r.mask = m.ctypes.data if m is not None else None
+        r.skip_mask = None
+        r.VF = VF.ctypes.data
+        r.SN = SN.ctypes.data if SN is not None else None
+        r.peaks = PK.ctypes.data if PK is not None else None
+        t = C.c_uint64()
+        rc = self.lib.mqpiv_server_submit(self.handle, C.byref(r), C.byref(t))
+        if rc != _lib.MQPIV_OK:
+            raise _lib.MqpivError(rc, "mqpiv_server_submit rejected the request")
+        self._live[t.value] = (a1, a2, m, VF, SN, PK)
+        return t.value
+
+    def submit_device(self, dreq: "DeviceRequest") -> int:
+        """Submit a pair resident in device memory (a DeviceRequest, or any
+        object whose ``c_request()`` names device buffers); its outputs stay on
+        the device."""
+        r = dreq.c_request()
+        t = C.c_uint64()
+        rc = self.lib.mqpiv_server_submit_device(self.handle, C.byref(r), C.byref(t))
+        if rc != _lib.MQPIV_OK:
+            raise _lib.MqpivError(rc, "mqpiv_server_submit_device rejected the request")
+        self._live[t.value] = dreq
+        return t.value
+
+    def wait(self, ticket: int, timeout: float = -1.0):
+        """(VF, SN[, peaks]) of a finished host request, or the DeviceRequest of
+        a device one; None if ``timeout`` expired."""
+        st = C.c_int32()
+        rc = self.lib.mqpiv_server_wait(self.handle, ticket, timeout, C.byref(st))
+        if rc != _lib.MQPIV_OK:
+            raise _lib.MqpivError(rc, f"unknown ticket {ticket}")
+        if st.value == -1:
+            return None
+        live = self._live.pop(ticket)
+        if st.value != _lib.MQPIV_OK:
+            msg = self.lib.mqpiv_server_last_error(self.handle).decode(errors="replace")
+            raise _lib.MqpivError(st.value, f"request {ticket} failed: {msg}")
+        if not isinstance(live, tuple):  # a device request: outputs stay on the device
+            return live
+        _, _, _, VF, SN, PK = live
+        return (VF, SN, PK) if PK is not None else (VF, SN)
+
+    def stats(self) -> dict:
+        s = _lib.MqpivServerStats()
+        rc = self.lib.mqpiv_server_stats_get(self.handle, C.byref(s))
+        if rc != _lib.MQPIV_OK:
+            raise _lib.MqpivError(rc, "mqpiv_server_stats_get failed")
+        return {k: getattr(s, k) for k, _ in s._fields_}
+
+    def reset_stats(self) -> None:
+        self.lib.mqpiv_server_reset_stats(self.handle)
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.mqpiv_server_destroy(self.handle)
+            self.handle = None
+            self._live.clear()
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass

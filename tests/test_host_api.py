@@ -157,3 +157,18 @@ def test_null_context_is_rejected():
     assert lib.mqpiv_ctx_destroy(None) == _lib.MQPIV_ERR_NULL
     assert lib.mqpiv_profile_begin(None) == _lib.MQPIV_ERR_NULL
     assert lib.mqpiv_version().startswith(b"mqpiv")
+
+
+def test_server_argument_errors_without_gpu():
+    lib = _lib.load()
+    h = C.c_void_p()
+    assert lib.mqpiv_server_create(0, 1 << 30, 0, 8, C.byref(h)) == _lib.MQPIV_ERR_BAD_PARAMS
+    assert lib.mqpiv_server_create(0, 1 << 30, 2, 0, C.byref(h)) == _lib.MQPIV_ERR_BAD_PARAMS
+    assert lib.mqpiv_server_create(0, 1024, 2, 8, C.byref(h)) == _lib.MQPIV_ERR_BAD_PARAMS
+    assert lib.mqpiv_server_create(0, 1 << 30, 2, 8, None) == _lib.MQPIV_ERR_NULL
+    t = C.c_uint64()
+    st = C.c_int32()
+    assert lib.mqpiv_server_submit(None, None, C.byref(t)) == _lib.MQPIV_ERR_NULL
+    assert lib.mqpiv_server_wait(None, 1, 0.0, C.byref(st)) == _lib.MQPIV_ERR_NULL
+    assert lib.mqpiv_server_stats_get(None, None) == _lib.MQPIV_ERR_NULL
+    assert lib.mqpiv_server_destroy(None) == _lib.MQPIV_ERR_NULL
