@@ -192,6 +192,60 @@ const char* mqpiv_server_last_error(mqpiv_server* sv);
 int mqpiv_server_reset_stats(mqpiv_server* sv);
 int mqpiv_server_destroy(mqpiv_server* sv);
 
+/* ---------------------------------------------------------------------------
+ * Vector-field post-processing on the device (src/postprocessing/postprocessing.jl).
+ * All buffers are device memory in the reference layout: VF is ndims x nVF...
+ * (component fastest), per-vector maps (magnitudes, SN, masks, collectiveness)
+ * are nVF... with dim 1 fastest.  VF and maps have the element type desc->dtype
+ * (the PIV output precision); masks are one byte per vector (0/1).  Radii are
+ * per dim (the reference's toDims3(radius)[1:N]); boxes are clipped at the
+ * field border (localOp, postprocessing.jl:56-101).  Enqueued on `stream` (NULL
+ * = the context's), not synchronised; they share the context's scratch, so one
+ * stream at a time per context.  Outputs must not alias inputs. */
+typedef struct mqpiv_vf_desc {
+  int32_t ndims;   /* 2 or 3: components per vector = grid dims */
+  int32_t dtype;   /* MQPIV_F32 / MQPIV_F64 */
+  int64_t size[3]; /* nVF per dim (size[2] ignored in 2-D) */
+} mqpiv_vf_desc;
+
+#define MQPIV_CMP_GT 0       /* mask = map > th (TH_masking's default cmp)   */
+#define MQPIV_CMP_LT 1       /* mask = map < th (remove_low_SN)              */
+#define MQPIV_REPLACE_ZERO 0 /* zeroReplace                                  */
+#define MQPIV_REPLACE_MEDIAN 1 /* medianReplace, and meanReplace (its body is the median) */
+
+/* magnitudes(VF): M = sqrt(sum(VF[:,i].^2))   (postprocessing.jl:178-194) */
+int mqpiv_pp_magnitudes(mqpiv_ctx* ctx, const mqpiv_vf_desc* desc, const void* VF, void* M, void* stream);
+/* STD_masking(n, map): |map - mean| > n * std, corrected std (:598-610) */
+int mqpiv_pp_std_mask(mqpiv_ctx* ctx, const mqpiv_vf_desc* desc, const void* map, double n_std, uint8_t* mask,
+                      void* stream);
+/* TH_masking(th, map; cmp) (:640-663) */
+int mqpiv_pp_th_mask(mqpiv_ctx* ctx, const mqpiv_vf_desc* desc, const void* map, double th, int32_t cmp,
+                     uint8_t* mask, void* stream);
+/* The replacement step of STD_filtering / TH_filtering / remove_low_SN
+ * (:539-573, 618-634, 672-703, 712-732): out = VF, with every flagged vector
+ * replaced by the zero vector or by the per-component median of the clipped
+ * box (radius) in VF.  Median: NaN if the box holds a NaN; even counts give
+ * a/2 + b/2 of the central pair (Statistics.median). */
+int mqpiv_pp_replace(mqpiv_ctx* ctx, const mqpiv_vf_desc* desc, const void* VF, const uint8_t* mask, int32_t how,
+                     const int64_t* radius, void* out, void* stream);
+/* average(radius, VF): per-component box mean (:359-381) */
+int mqpiv_pp_average(mqpiv_ctx* ctx, const mqpiv_vf_desc* desc, const void* VF, const int64_t* radius, void* out,
+                     void* stream);
+/* collectiveness(radius, VF; max_angle) (:294-331): Float32 fraction of the box
+ * whose normalised dot product with the vector exceeds min_dot = cosd(max_angle). */
+int mqpiv_pp_collectiveness(mqpiv_ctx* ctx, const mqpiv_vf_desc* desc, const void* VF, const int64_t* radius,
+                            double min_dot, float* out, void* stream);
+/* similarityAveraging(radius, VF; max_angle, normalize, scale_by_similarity,
+ * similarity_exponent) (:478-523), min_dot = cosd(max_angle); exponent >= 0. */
+int mqpiv_pp_similarity_averaging(mqpiv_ctx* ctx, const mqpiv_vf_desc* desc, const void* VF, const int64_t* radius,
+                                  double min_dot, int32_t normalize, int32_t scale_by_similarity,
+                                  int32_t similarity_exponent, void* out, void* stream);
+/* local_magnitude_mask(ratio, VF; radius) (:202-218): out = VF with vectors whose
+ * magnitude / mean magnitude of the rest of the box is not < ratio set to
+ * (signed) zero; removed = 1 for those. */
+int mqpiv_pp_local_magnitude_mask(mqpiv_ctx* ctx, const mqpiv_vf_desc* desc, const void* VF, double ratio,
+                                  const int64_t* radius, void* out, uint8_t* removed, void* stream);
+
 /* Library build identification (for logs). */
 const char* mqpiv_version(void);
 

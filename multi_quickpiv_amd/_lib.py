@@ -23,6 +23,11 @@ MQPIV_ERR_NULL = 6
 MQPIV_F32 = 0
 MQPIV_F64 = 1
 
+MQPIV_CMP_GT = 0
+MQPIV_CMP_LT = 1
+MQPIV_REPLACE_ZERO = 0
+MQPIV_REPLACE_MEDIAN = 1
+
 MQPIV_PLAIN_FP32 = 0
 MQPIV_FP64 = 2
 MQPIV_AUTO = 3
@@ -52,6 +57,14 @@ EXPORTS = (
     "mqpiv_server_reset_stats",
     "mqpiv_server_last_error",
     "mqpiv_server_destroy",
+    "mqpiv_pp_magnitudes",
+    "mqpiv_pp_std_mask",
+    "mqpiv_pp_th_mask",
+    "mqpiv_pp_replace",
+    "mqpiv_pp_average",
+    "mqpiv_pp_collectiveness",
+    "mqpiv_pp_similarity_averaging",
+    "mqpiv_pp_local_magnitude_mask",
 )
 
 
@@ -83,6 +96,10 @@ class MqpivRequest(C.Structure):
         ("SN", C.c_void_p),
         ("peaks", C.c_void_p),
     ]
+
+
+class MqpivVfDesc(C.Structure):
+    _fields_ = [("ndims", C.c_int32), ("dtype", C.c_int32), ("size", C.c_int64 * 3)]
 
 
 class MqpivServerStats(C.Structure):
@@ -168,6 +185,22 @@ def load() -> C.CDLL:
         lib.mqpiv_server_reset_stats.argtypes = [vp]
         lib.mqpiv_server_destroy.restype = C.c_int
         lib.mqpiv_server_destroy.argtypes = [vp]
+        D = C.POINTER(MqpivVfDesc)
+        R = C.POINTER(C.c_int64)
+        for name, args in (
+            ("mqpiv_pp_magnitudes", [vp, D, vp, vp, vp]),
+            ("mqpiv_pp_std_mask", [vp, D, vp, C.c_double, vp, vp]),
+            ("mqpiv_pp_th_mask", [vp, D, vp, C.c_double, C.c_int32, vp, vp]),
+            ("mqpiv_pp_replace", [vp, D, vp, vp, C.c_int32, R, vp, vp]),
+            ("mqpiv_pp_average", [vp, D, vp, R, vp, vp]),
+            ("mqpiv_pp_collectiveness", [vp, D, vp, R, C.c_double, vp, vp]),
+            ("mqpiv_pp_similarity_averaging", [vp, D, vp, R, C.c_double, C.c_int32, C.c_int32, C.c_int32, vp,
+                                               vp]),
+            ("mqpiv_pp_local_magnitude_mask", [vp, D, vp, C.c_double, R, vp, vp, vp]),
+        ):
+            f = getattr(lib, name)
+            f.restype = C.c_int
+            f.argtypes = args
         _lib = lib
         return lib
 

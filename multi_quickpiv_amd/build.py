@@ -59,6 +59,8 @@ SOURCES = [
     "mqpiv_sat.hip",
     "mqpiv_api.cpp",
     "mqpiv_server.cpp",
+    "mqpiv_post.hip",
+    "mqpiv_post_api.cpp",
 ]
 
 

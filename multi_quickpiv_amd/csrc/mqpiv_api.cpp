@@ -19,6 +19,7 @@
 
 #include "../../include/mqpiv.h"
 #include "mqpiv_batch.h"
+#include "mqpiv_ctx.h"
 #include "mqpiv_internal.h"
 
 namespace mqpiv {
@@ -37,55 +38,8 @@ hipError_t launch_piv3d(int fp64, int L, const Job& j, void* scratch, const doub
 
 using namespace mqpiv;
 
-struct mqpiv_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  uint64_t budget = 0;
-  std::string err;
-  std::mutex mu;
-  // staging for mqpiv_piv (host buffers)
-  void* d_stage = nullptr;
-  size_t stage_bytes = 0;
-  // scratch for multi-kernel (3-D) paths
-  void* d_scratch = nullptr;
-  size_t scratch_bytes = 0;
-  // FP64 integral arrays of the search image (sum, sum of squares) + scan scratch
-  void* d_sat = nullptr;
-  size_t sat_cap = 0;
-  // batches: integral arrays of all pairs, device job tables, pinned staging
-  void* d_bsat = nullptr;
-  size_t bsat_cap = 0;
-  void* d_table = nullptr;
-  size_t table_cap = 0;
-  void* h_table = nullptr;
-  size_t h_table_cap = 0;
-  hipEvent_t table_done = nullptr;
-  // measurement hooks
-  bool profiling = false;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
-};
-
-// Brackets one correlation-kernel launch with events when profiling is on.
-struct KernelTimer {
-  mqpiv_ctx* c;
-  hipStream_t s;
-  hipEvent_t a = nullptr, b = nullptr;
-  KernelTimer(mqpiv_ctx* c_, hipStream_t s_) : c(c_), s(s_) {
-    if (c->profiling && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess)
-      hipEventRecord(a, s);
-  }
-  ~KernelTimer() {
-    if (a && b) {
-      hipEventRecord(b, s);
-      c->events.emplace_back(a, b);
-    }
-  }
-};
-
-static int fail(mqpiv_ctx* c, int code, const char* fmt, ...) __attribute__((format(printf, 3, 4)));
-
 // Grow-only device buffer owned by the context.
-static int ensure(mqpiv_ctx* c, void** buf, size_t* cap, size_t need, hipStream_t s) {
+int ensure(mqpiv_ctx* c, void** buf, size_t* cap, size_t need, hipStream_t s) {
   if (need <= *cap) return MQPIV_OK;
   if (*buf) {
     hipStreamSynchronize(s);
@@ -100,8 +54,7 @@ static int ensure(mqpiv_ctx* c, void** buf, size_t* cap, size_t need, hipStream_
   return MQPIV_OK;
 }
 
-static int fail(mqpiv_ctx* c, int code, const char* fmt, ...) __attribute__((format(printf, 3, 4)));
-static int fail(mqpiv_ctx* c, int code, const char* fmt, ...) {
+int fail(mqpiv_ctx* c, int code, const char* fmt, ...) {
   char buf[512];
   va_list ap;
   va_start(ap, fmt);
@@ -110,14 +63,6 @@ static int fail(mqpiv_ctx* c, int code, const char* fmt, ...) {
   if (c) c->err = buf;
   return code;
 }
-
-#define HIP_OR_FAIL(ctx, expr)                                                              \
-  do {                                                                                      \
-    hipError_t e_ = (expr);                                                                 \
-    if (e_ != hipSuccess)                                                                   \
-      return fail(ctx, e_ == hipErrorOutOfMemory ? MQPIV_ERR_OOM : MQPIV_ERR_DEVICE,        \
-                  "%s failed: %s", #expr, hipGetErrorString(e_));                           \
-  } while (0)
 
 extern "C" const char* mqpiv_version(void) { return "mqpiv 0.1.0 (gfx950)"; }
 
@@ -149,6 +94,7 @@ extern "C" int mqpiv_ctx_destroy(mqpiv_ctx* c) {
   if (c->d_bsat) hipFree(c->d_bsat);
   if (c->d_table) hipFree(c->d_table);
   if (c->h_table) hipHostFree(c->h_table);
+  if (c->d_pp) hipFree(c->d_pp);
   if (c->table_done) hipEventDestroy(c->table_done);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;

@@ -221,7 +221,11 @@ class DeviceRequest:
         return r
 
     def results(self):
-        """(VF, SN[, peaks]) as numpy arrays in the reference layout."""
+        """(VF, SN[, peaks]) as numpy arrays in the reference layout (waits for
+        all work on the device: the batch may run on a library-owned stream)."""
+        import torch
+
+        torch.cuda.synchronize(self.VF.device)
         N = len(self.shape)
         vf = tuple(self.vf_size)
         VF = self.VF.cpu().numpy().reshape((N, *vf), order="F")

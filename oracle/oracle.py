@@ -190,3 +190,108 @@ def peak_subpixel_sn(S: np.ndarray, sm: Sequence[int], off_lo: Sequence[int], of
     lib.orc_subpixel(S.ctypes.data, ns, nd, sm3, lo3, hi3, pk, val.value, disp)
     sn = lib.orc_sn(S.ctypes.data, ns, nd, sm3, lo3, hi3)
     return tuple(pk[d] for d in range(nd)), val.value, tuple(disp[d] for d in range(nd)), sn
+
+
+# ---------------------------------------------------------------- post-processing
+# Restatement of src/postprocessing/postprocessing.jl (oracle/mqpiv_oracle_pp.c).
+def _sfx(a: np.ndarray) -> str:
+    return "_f32" if a.dtype == np.float32 else "_f64"
+
+
+def _vf_in(VF) -> Tuple[np.ndarray, int, "C.Array"]:
+    VF = np.asarray(VF)
+    if VF.dtype not in (np.float32, np.float64):
+        VF = VF.astype(np.float64)
+    VF = np.asfortranarray(VF)
+    nd = VF.shape[0]
+    assert VF.ndim == nd + 1 and nd in (2, 3), "VF must be (N, nVF...) with N = 2 or 3"
+    return VF, nd, _i3(VF.shape[1:], 1)
+
+
+def _r3(radius, nd: int) -> "C.Array":
+    r = (radius,) * nd if isinstance(radius, (int, np.integer)) else tuple(radius)[:nd]
+    return _i3(r, 0)
+
+
+def _fn(name: str, a: np.ndarray):
+    return getattr(load(), f"orc_pp_{name}{_sfx(a)}")
+
+
+def pp_magnitudes(VF) -> np.ndarray:
+    VF, nd, sz = _vf_in(VF)
+    M = np.zeros(VF.shape[1:], dtype=VF.dtype, order="F")
+    f = _fn("magnitudes", VF)
+    f.restype = None
+    f(C.c_int(nd), sz, C.c_void_p(VF.ctypes.data), C.c_void_p(M.ctypes.data))
+    return M
+
+
+def pp_th_mask(x, th: float, cmp: int = 0) -> np.ndarray:
+    x = np.asfortranarray(x)
+    mask = np.zeros(x.shape, dtype=np.uint8, order="F")
+    f = _fn("th_mask", x)
+    f.restype = None
+    f(C.c_int64(x.size), C.c_void_p(x.ctypes.data), C.c_double(th), C.c_int(cmp), C.c_void_p(mask.ctypes.data))
+    return mask
+
+
+def pp_std_mask(x, nsd: float):
+    """(mask, mean, std) of STD_masking on a per-vector map."""
+    x = np.asfortranarray(x)
+    mask = np.zeros(x.shape, dtype=np.uint8, order="F")
+    stats = (C.c_double * 2)()
+    f = _fn("std_mask", x)
+    f.restype = None
+    f(C.c_int64(x.size), C.c_void_p(x.ctypes.data), C.c_double(nsd), C.c_void_p(mask.ctypes.data), stats)
+    return mask, stats[0], stats[1]
+
+
+def pp_replace(VF, mask, how: int, radius) -> np.ndarray:
+    VF, nd, sz = _vf_in(VF)
+    mask = np.asfortranarray(mask, dtype=np.uint8)
+    out = np.zeros_like(VF, order="F")
+    f = _fn("replace", VF)
+    f.restype = None
+    f(C.c_int(nd), sz, C.c_void_p(VF.ctypes.data), C.c_void_p(mask.ctypes.data), C.c_int(how),
+      _r3(radius, nd), C.c_void_p(out.ctypes.data))
+    return out
+
+
+def pp_average(VF, radius) -> np.ndarray:
+    VF, nd, sz = _vf_in(VF)
+    out = np.zeros_like(VF, order="F")
+    f = _fn("average", VF)
+    f.restype = None
+    f(C.c_int(nd), sz, C.c_void_p(VF.ctypes.data), _r3(radius, nd), C.c_void_p(out.ctypes.data))
+    return out
+
+
+def pp_collectiveness(VF, radius, min_dot: float) -> np.ndarray:
+    VF, nd, sz = _vf_in(VF)
+    out = np.zeros(VF.shape[1:], dtype=np.float32, order="F")
+    f = _fn("collectiveness", VF)
+    f.restype = None
+    f(C.c_int(nd), sz, C.c_void_p(VF.ctypes.data), _r3(radius, nd), C.c_double(min_dot),
+      C.c_void_p(out.ctypes.data))
+    return out
+
+
+def pp_simavg(VF, radius, min_dot: float, normalize: bool = True, scale: bool = True, expo: int = 2):
+    VF, nd, sz = _vf_in(VF)
+    out = np.zeros_like(VF, order="F")
+    f = _fn("simavg", VF)
+    f.restype = None
+    f(C.c_int(nd), sz, C.c_void_p(VF.ctypes.data), _r3(radius, nd), C.c_double(min_dot),
+      C.c_int(int(normalize)), C.c_int(int(scale)), C.c_int(expo), C.c_void_p(out.ctypes.data))
+    return out
+
+
+def pp_lmm(VF, ratio: float, radius):
+    VF, nd, sz = _vf_in(VF)
+    out = np.zeros_like(VF, order="F")
+    removed = np.zeros(VF.shape[1:], dtype=np.uint8, order="F")
+    f = _fn("lmm", VF)
+    f.restype = None
+    f(C.c_int(nd), sz, C.c_void_p(VF.ctypes.data), C.c_double(ratio), _r3(radius, nd),
+      C.c_void_p(out.ctypes.data), C.c_void_p(removed.ctypes.data))
+    return out, removed
