@@ -227,8 +227,8 @@ __host__ __device__ constexpr int chunk_of(int q) {
 #if defined(__HIP_DEVICE_COMPILE__)
 template <int MASK>
 __device__ __forceinline__ int xor_lane_i32(int v) {
-  if constexpr (MASK == 1) return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
-  else if constexpr (MASK == 2) return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+  if constexpr (MASK == 1) return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, true);  // quad_perm [1,0,3,2]
+  else if constexpr (MASK == 2) return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, true);  // quad_perm [2,3,0,1]
   else return __builtin_amdgcn_ds_swizzle(v, (MASK << 10) | 0x1F);  // bit mode: xor MASK
 }
 template <int MASK>
@@ -256,37 +256,98 @@ struct DppXchg {
   }
 };
 
-// One radix-2 stage across lanes q and q ^ MASK (DIT form): the member with
-// (q & MASK) != 0 first multiplies by W_len^(k + off), then both exchange;
-// the low member keeps E + W*O, the high member E - W*O.
-template <int LEN, int SIGN, int MASK, int M, typename T, typename X>
-__host__ __device__ __forceinline__ void group_dit_stage(cpx<T> (&v)[M], int q, int off, const X& xchg) {
-  const bool hi = (q & MASK) != 0;
-  if (hi) {
-#pragma unroll
-    for (int k = 0; k < M; ++k) v[k] = twiddle_mul<LEN, SIGN>(v[k], k + off);
-  }
-#pragma unroll
-  for (int k = 0; k < M; ++k) {
-    cpx<T> u = xchg.template get<MASK>(v[k], k);
-    v[k] = hi ? csub(u, v[k]) : cadd(v[k], u);
+// Fused multiply-add in the element type (host and device).
+template <typename T>
+__host__ __device__ __forceinline__ T fma_t(T a, T b, T c) {
+  if constexpr (sizeof(T) == 4) return __builtin_fmaf(a, b, c);
+  else return __builtin_fma(a, b, c);
+}
+
+// u * -exp(SIGN * 2*pi*i * K / L) for a compile-time K: the negated twiddle
+// that the high member of a cross-lane stage applies.
+template <int L, int SIGN, int K, typename T>
+__host__ __device__ __forceinline__ cpx<T> twiddle_mul_neg(cpx<T> u) {
+  constexpr int k = K % L;
+  if constexpr (k == 0) return cpx<T>{-u.x, -u.y};
+  else if constexpr (4 * k == L) return SIGN > 0 ? cpx<T>{u.y, -u.x} : cpx<T>{-u.y, u.x};
+  else if constexpr (2 * k == L) return u;
+  else if constexpr (4 * k == 3 * L) return SIGN > 0 ? cpx<T>{-u.y, u.x} : cpx<T>{u.y, -u.x};
+  else {
+    constexpr Twiddles<L, T> W{};
+    constexpr T c = -W.c[k];
+    constexpr T s = SIGN > 0 ? -W.s[k] : W.s[k];
+    return cpx<T>{u.x * c - u.y * s, u.x * s + u.y * c};
   }
 }
 
-// One radix-2 stage across lanes q and q ^ MASK (DIF form): exchange, the low
-// member keeps a + b, the high member (a - b) * W_len^(k + off).
-template <int LEN, int SIGN, int MASK, int M, typename T, typename X>
-__host__ __device__ __forceinline__ void group_dif_stage(cpx<T> (&v)[M], int q, int off, const X& xchg) {
+// The lane-dependent factor of a cross-lane stage.  A stage's twiddle is
+// W_LEN^(k + c*M) with k < M compile-time and c in [0, NC) the lane's chunk;
+// since LEN = 2*NC*M it factors as W_LEN^k * exp(SIGN*i*pi*c/NC): NC = 1 needs
+// nothing, NC = 2 a rotation by SIGN*i (selects), NC = 4 one runtime complex
+// product with a factor picked once per stage.  No runtime-indexed tables.
+template <int NC, int SIGN, typename T>
+__host__ __device__ __forceinline__ cpx<T> lane_factor(int c) {
+  if constexpr (NC == 4) {
+    constexpr T h = T(0.70710678118654752440084436210485);
+    const bool b0 = (c & 1) != 0, b1 = (c & 2) != 0;
+    const T re = b1 ? (b0 ? -h : T(0)) : (b0 ? h : T(1));
+    const T im = b1 ? (b0 ? h : T(1)) : (b0 ? h : T(0));
+    return cpx<T>{re, SIGN > 0 ? im : -im};
+  } else {
+    return cpx<T>{T(1), T(0)};
+  }
+}
+template <int NC, int SIGN, typename T>
+__host__ __device__ __forceinline__ cpx<T> apply_lane_factor(cpx<T> u, int c, cpx<T> f) {
+  if constexpr (NC == 1) {
+    return u;
+  } else if constexpr (NC == 2) {
+    // u * (SIGN*i)^c
+    const cpx<T> r = SIGN > 0 ? cpx<T>{-u.y, u.x} : cpx<T>{u.y, -u.x};
+    return c ? r : u;
+  } else {
+    return cpx<T>{u.x * f.x - u.y * f.y, u.x * f.y + u.y * f.x};
+  }
+}
+
+template <int LEN, int SIGN, int NC, int M, int K, typename T>
+__host__ __device__ __forceinline__ void hi_twiddle(cpx<T> (&v)[M], int c, cpx<T> f) {
+  if constexpr (K < M) {
+    v[K] = apply_lane_factor<NC, SIGN>(twiddle_mul_neg<LEN, SIGN, K>(v[K]), c, f);
+    hi_twiddle<LEN, SIGN, NC, M, K + 1>(v, c, f);
+  }
+}
+
+// One radix-2 stage across lanes q and q ^ MASK (DIT form).  The high member
+// (q & MASK) holds the odd half and replaces it by -O*W (negated twiddle); the
+// exchange then gives both members the same form, out = self + s * partner
+// with s = -1 (low: E + O*W) or +1 (high: E - O*W), a single fused multiply-add
+// per component with the partner value straight from the cross-lane move.
+template <int LEN, int SIGN, int MASK, int NC, int M, typename T, typename X>
+__host__ __device__ __forceinline__ void group_dit_stage(cpx<T> (&v)[M], int q, int c, const X& xchg) {
   const bool hi = (q & MASK) != 0;
+  if (hi) hi_twiddle<LEN, SIGN, NC, M, 0>(v, c, lane_factor<NC, SIGN, T>(c));
+  const T s = hi ? T(1) : T(-1);
 #pragma unroll
   for (int k = 0; k < M; ++k) {
-    cpx<T> u = xchg.template get<MASK>(v[k], k);
-    v[k] = hi ? csub(u, v[k]) : cadd(v[k], u);
+    const cpx<T> u = xchg.template get<MASK>(v[k], k);
+    v[k] = cpx<T>{fma_t(u.x, s, v[k].x), fma_t(u.y, s, v[k].y)};
   }
-  if (hi) {
+}
+
+// One radix-2 stage across lanes q and q ^ MASK (DIF form): out = self +
+// s * partner with s = +1 (low: a + b) or -1 (high: b - a), then the high
+// member multiplies by the negated twiddle, giving (a - b) * W.
+template <int LEN, int SIGN, int MASK, int NC, int M, typename T, typename X>
+__host__ __device__ __forceinline__ void group_dif_stage(cpx<T> (&v)[M], int q, int c, const X& xchg) {
+  const bool hi = (q & MASK) != 0;
+  const T s = hi ? T(-1) : T(1);
 #pragma unroll
-    for (int k = 0; k < M; ++k) v[k] = twiddle_mul<LEN, SIGN>(v[k], k + off);
+  for (int k = 0; k < M; ++k) {
+    const cpx<T> u = xchg.template get<MASK>(v[k], k);
+    v[k] = cpx<T>{fma_t(u.x, s, v[k].x), fma_t(u.y, s, v[k].y)};
   }
+  if (hi) hi_twiddle<LEN, SIGN, NC, M, 0>(v, c, lane_factor<NC, SIGN, T>(c));
 }
 
 // Stage j of a P-lane group (partner mask 2^j).
@@ -294,9 +355,10 @@ template <int L, int P, int J, int SIGN, bool DIT, int M, typename T, typename X
 __host__ __device__ __forceinline__ void group_stage(cpx<T> (&v)[M], int q, const X& xchg) {
   constexpr int MASK = 1 << J;
   constexpr int LEN = L >> J;
-  const int off = chunk_of<(P >> (J + 1))>(q >> (J + 1)) * M;
-  if constexpr (DIT) group_dit_stage<LEN, SIGN, MASK, M>(v, q, off, xchg);
-  else group_dif_stage<LEN, SIGN, MASK, M>(v, q, off, xchg);
+  constexpr int NC = P >> (J + 1);
+  const int c = chunk_of<NC>(q >> (J + 1));
+  if constexpr (DIT) group_dit_stage<LEN, SIGN, MASK, NC, M>(v, q, c, xchg);
+  else group_dif_stage<LEN, SIGN, MASK, NC, M>(v, q, c, xchg);
 }
 
 // Forward transform over a group of P lanes (see above).

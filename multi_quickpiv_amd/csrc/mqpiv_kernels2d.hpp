@@ -48,6 +48,16 @@ struct Cfg2 {
   static_assert(LPL == 1 || LPL == 2 || LPL == 4 || LPL == 8, "1, 2, 4 or 8 lanes per line");
 };
 
+// Global-address-space view of a device pointer: loads through it are
+// global_load (vector memory only) rather than flat loads, which also count
+// against the LDS counter and force waits on both.
+template <typename T>
+using gptr = const T __attribute__((address_space(1)))*;
+template <typename T>
+__device__ __forceinline__ gptr<T> as_global(const void* p) {
+  return (gptr<T>)p;
+}
+
 __device__ __forceinline__ double load_px(const void* img, int dt, int64_t i) {
   return dt == DT_F32 ? (double)((const float*)img)[i] : ((const double*)img)[i];
 }
@@ -321,6 +331,15 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT)) void piv2d_kernel(Job j0, co
   auto k1_of = [&](int k) { return LPL * k + h; };
   auto f_at = [&](int k1) -> bool { return f_row && k1 < is1; };
   auto g_at = [&](int k1) -> bool { return g_row && k1 >= g1.goff && k1 < g1.goff + g1.gsz; };
+  // Every lane issues all its loads before using any (one wait for the batch):
+  // positions outside the IA / clipped search region read an in-image pixel
+  // instead (the IA origin, or the search region's first row / column) and
+  // their values are discarded.  g_base addresses column k1 of the padded
+  // buffer, whose image columns start at k1 = goff1.
+  const int64_t f_lane = f_row ? f_base : (g0.tlf + d0 * g1.tlf);
+  const int64_t g_lane = g_row ? g_base : (g0.sr + d0 * (g1.sr - g1.goff));
+  auto f_off = [&](int k1) -> uint32_t { return f_at(k1) ? ud0 * (uint32_t)k1 : 0u; };
+  auto g_off = [&](int k1) -> uint32_t { return ud0 * (uint32_t)(g_at(k1) ? k1 : g1.goff); };
 
   // ---- ZNCC means over the IA and the clipped search region (zncc_ops.jl:10,22;
   // computed in the input element type, quirk Q12)
@@ -330,13 +349,19 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT)) void piv2d_kernel(Job j0, co
     double sf = 0.0, sg = 0.0;
     auto sums = [&](auto tin) {
       using TIN = decltype(tin);
-      const TIN* pf = (const TIN*)J.img1 + f_base;
-      const TIN* pg = (const TIN*)J.img2 + g_base;
+      const gptr<TIN> pf = as_global<TIN>(J.img1) + f_lane;
+      const gptr<TIN> pg = as_global<TIN>(J.img2) + g_lane;
+      TIN fv[H], gv[H];
+#pragma unroll
+      for (int k = 0; k < H; ++k) {
+        fv[k] = pf[f_off(k1_of(k))];
+        gv[k] = pg[g_off(k1_of(k))];
+      }
 #pragma unroll
       for (int k = 0; k < H; ++k) {
         const int k1 = k1_of(k);
-        if (f_at(k1)) sf += (double)pf[ud0 * (uint32_t)k1];
-        if (g_at(k1)) sg += (double)pg[ud0 * (uint32_t)k1];
+        if (f_at(k1)) sf += (double)fv[k];
+        if (g_at(k1)) sg += (double)gv[k];
       }
     };
     if (dt == DT_F32) sums(float{});
@@ -357,22 +382,31 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT)) void piv2d_kernel(Job j0, co
   const bool want_max = J.has_threshold || J.has_mask_threshold;
   auto load = [&](auto tin) {
     using TIN = decltype(tin);
-    const TIN* pf = (const TIN*)J.img1 + f_base;
-    const TIN* pm = (const TIN*)J.mask + f_base;
-    const TIN* pg = (const TIN*)J.img2 + g_base;
+    const gptr<TIN> pf = as_global<TIN>(J.img1) + f_lane;
+    const gptr<TIN> pg = as_global<TIN>(J.img2) + g_lane;
+    TIN fv[H], gv[H], mv[H];
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+      fv[k] = pf[f_off(k1_of(k))];
+      gv[k] = pg[g_off(k1_of(k))];
+    }
+    if constexpr (ALG == ALG_MASK_NSQECC) {
+      const gptr<TIN> pm = as_global<TIN>(J.mask) + f_lane;
+#pragma unroll
+      for (int k = 0; k < H; ++k) mv[k] = pm[f_off(k1_of(k))];
+    }
 #pragma unroll
     for (int k = 0; k < H; ++k) {
       const int k1 = k1_of(k);
-      const uint32_t o = ud0 * (uint32_t)k1;
       double a = 0.0, g = 0.0;
       if (f_at(k1)) {
-        const TIN f = pf[o];
+        const TIN f = fv[k];
         if (want_max) mx1 = fmax(mx1, (double)f);
         if constexpr (ALG == ALG_ZNCC) {
           a = std::is_same<TIN, float>::value ? (double)((float)f - meanF32) : (double)f - meanF64;
           sumF2 += a * a;
         } else if constexpr (ALG == ALG_MASK_NSQECC) {
-          const double m = (double)pm[o];
+          const double m = (double)mv[k];
           if (want_max) mx2 = fmax(mx2, m);
           a = (double)f * m;  // masked_nsqecc_ops.jl:19-25
           const double wi = m > 0 ? m : 1.0;
@@ -383,11 +417,10 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT)) void piv2d_kernel(Job j0, co
         }
       }
       if (g_at(k1)) {
-        const TIN gv = pg[o];
         if constexpr (ALG == ALG_ZNCC)
-          g = std::is_same<TIN, float>::value ? (double)((float)gv - meanG32) : (double)gv - meanG64;
+          g = std::is_same<TIN, float>::value ? (double)((float)gv[k] - meanG32) : (double)gv[k] - meanG64;
         else
-          g = (double)gv;
+          g = (double)gv[k];
       }
       x[k] = cpx<T>{(T)a, (T)g};
     }
@@ -416,16 +449,21 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT)) void piv2d_kernel(Job j0, co
     // second correlation: mask (x) G^2 (mask_nsqecc.jl:155)
     auto load2 = [&](auto tin) {
       using TIN = decltype(tin);
-      const TIN* pm = (const TIN*)J.mask + f_base;
-      const TIN* pg = (const TIN*)J.img2 + g_base;
+      const gptr<TIN> pm = as_global<TIN>(J.mask) + f_lane;
+      const gptr<TIN> pg = as_global<TIN>(J.img2) + g_lane;
+      TIN mv[H], gv[H];
+#pragma unroll
+      for (int k = 0; k < H; ++k) {
+        mv[k] = pm[f_off(k1_of(k))];
+        gv[k] = pg[g_off(k1_of(k))];
+      }
 #pragma unroll
       for (int k = 0; k < H; ++k) {
         const int k1 = k1_of(k);
-        const uint32_t o = ud0 * (uint32_t)k1;
         double m = 0.0, g2 = 0.0;
-        if (f_at(k1)) m = (double)pm[o];
+        if (f_at(k1)) m = (double)mv[k];
         if (g_at(k1)) {
-          const double g = (double)pg[o];
+          const double g = (double)gv[k];
           g2 = g * g;
         }
         x[k] = cpx<T>{(T)m, (T)g2};
@@ -442,36 +480,68 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT)) void piv2d_kernel(Job j0, co
     // W(s) = sum of G'^2 over the search-buffer window [s, s + is) (the SAT
     // window sums of common_ops.jl:203-245), from the image's FP64 integral
     // arrays after clipping the window to the part that holds image data.
+    // Each lane takes NB elements per round and issues all their corner loads
+    // before using any.  Corner indices are clamped so that an empty window
+    // (no image data) reads four equal corners and sums to exactly 0, and
+    // elements past the end read element 0's corners; no load is predicated.
     const int64_t D0 = d0 + 1;
     const double sqF = sqrt(sumF2);
     const double mu = (dt == DT_F32) ? (double)meanG32 : meanG64;
-    for (int e = t; e < ns0 * ns1; e += C::TPI) {
-      const int s0 = e % ns0, s1 = e / ns0;
-      const int lo0 = s0 > g0.goff ? s0 : g0.goff;
-      const int hi0 = (s0 + is0) < (g0.goff + g0.gsz) ? (s0 + is0) : (g0.goff + g0.gsz);
-      const int lo1 = s1 > g1.goff ? s1 : g1.goff;
-      const int hi1 = (s1 + is1) < (g1.goff + g1.gsz) ? (s1 + is1) : (g1.goff + g1.gsz);
-      double W = 0.0;
-      if (hi0 > lo0 && hi1 > lo1) {
+    const gptr<double> sat2 = as_global<double>(J.sat2);
+    const gptr<double> sat1 = as_global<double>(J.sat1);
+    constexpr int NB = 4;
+    constexpr int NQ = (ALG == ALG_ZNCC) ? 2 : 1;
+    const int nE = ns0 * ns1;
+    for (int e0 = t; e0 < nE; e0 += NB * C::TPI) {
+      double q[NB][NQ][4];
+      double nbox[NB];
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        const int e = e0 + j * C::TPI;
+        const int ec = e < nE ? e : 0;
+        const int s0 = ec % ns0, s1 = ec / ns0;
+        const int lo0 = s0 > g0.goff ? s0 : g0.goff;
+        int hi0 = (s0 + is0) < (g0.goff + g0.gsz) ? (s0 + is0) : (g0.goff + g0.gsz);
+        hi0 = hi0 > lo0 ? hi0 : lo0;
+        const int lo1 = s1 > g1.goff ? s1 : g1.goff;
+        int hi1 = (s1 + is1) < (g1.goff + g1.gsz) ? (s1 + is1) : (g1.goff + g1.gsz);
+        hi1 = hi1 > lo1 ? hi1 : lo1;
         const int64_t a0 = g0.sr + (lo0 - g0.goff), b0 = g0.sr + (hi0 - g0.goff);
         const int64_t a1 = g1.sr + (lo1 - g1.goff), b1 = g1.sr + (hi1 - g1.goff);
-        W = sat_box(J.sat2, D0, a0, b0, a1, b1);
+        const int64_t c11 = b0 + D0 * b1, c01 = a0 + D0 * b1, c10 = b0 + D0 * a1, c00 = a0 + D0 * a1;
+        q[j][0][0] = sat2[c11];
+        q[j][0][1] = sat2[c01];
+        q[j][0][2] = sat2[c10];
+        q[j][0][3] = sat2[c00];
         if constexpr (ALG == ALG_ZNCC) {
-          const double n = (double)((b0 - a0) * (b1 - a1));
-          W = (W - 2.0 * mu * sat_box(J.sat1, D0, a0, b0, a1, b1)) + n * mu * mu;
+          q[j][NQ - 1][0] = sat1[c11];
+          q[j][NQ - 1][1] = sat1[c01];
+          q[j][NQ - 1][2] = sat1[c10];
+          q[j][NQ - 1][3] = sat1[c00];
+        }
+        nbox[j] = (double)((b0 - a0) * (b1 - a1));
+      }
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        const int e = e0 + j * C::TPI;
+        if (e >= nE) break;
+        double W = ((q[j][0][0] - q[j][0][1]) - q[j][0][2]) + q[j][0][3];
+        if constexpr (ALG == ALG_ZNCC) {
+          const double S1 = ((q[j][NQ - 1][0] - q[j][NQ - 1][1]) - q[j][NQ - 1][2]) + q[j][NQ - 1][3];
+          W = (W - 2.0 * mu * S1) + nbox[j] * mu * mu;
         }
         W = W > 0.0 ? W : 0.0;
+        const double c = (double)S[e];
+        const double den = sqrt(W) * sqF;
+        double r;
+        if constexpr (ALG == ALG_ZNCC) {
+          r = c / den;  // zncc_ops.jl:39-48
+        } else {
+          const double num = (sumF2 - 2.0 * c) + W;  // nsqecc_ops.jl:58-60
+          r = den / (num + den);                     // nsqecc_ops.jl:72-73
+        }
+        S[e] = (T)r;
       }
-      const double c = (double)S[e];
-      const double den = sqrt(W) * sqF;
-      double r;
-      if constexpr (ALG == ALG_ZNCC) {
-        r = c / den;  // zncc_ops.jl:39-48
-      } else {
-        const double num = (sumF2 - 2.0 * c) + W;  // nsqecc_ops.jl:58-60
-        r = den / (num + den);                     // nsqecc_ops.jl:72-73
-      }
-      S[e] = (T)r;
     }
     __syncthreads();
   } else if constexpr (ALG == ALG_MASK_NSQECC) {
