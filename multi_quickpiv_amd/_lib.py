@@ -7,10 +7,13 @@ silently take another path.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import threading
 from pathlib import Path
 
-LIB_PATH = Path(__file__).resolve().parent / "libmqpiv.so"
+# The in-tree build; MQPIV_LIB names another build of the same library (for
+# example one compiled with different tuning flags).
+LIB_PATH = Path(os.environ.get("MQPIV_LIB") or (Path(__file__).resolve().parent / "libmqpiv.so"))
 
 MQPIV_OK = 0
 MQPIV_ERR_SIZE_MISMATCH = 1

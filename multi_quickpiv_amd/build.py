@@ -71,27 +71,31 @@ def _run(cmd: list[str]) -> None:
         raise RuntimeError(f"build step failed: {' '.join(cmd)}")
 
 
-def _compile(src: str, incremental: bool = False) -> Path:
-    obj = BUILD / (Path(src).stem + ".o")
+def _compile(src: str, incremental: bool = False, build: Path = BUILD, extra=()) -> Path:
+    obj = build / (Path(src).stem + ".o")
     if incremental and obj.exists():
         deps = [CSRC / src, *CSRC.glob("*.hpp"), *CSRC.glob("*.h"), ROOT / "include" / "mqpiv.h"]
         if obj.stat().st_mtime > max(d.stat().st_mtime for d in deps):
             return obj
-    cmd = [HIPCC, *DEVICE_FLAGS, "-c", str(CSRC / src), "-o", str(obj)]
+    cmd = [HIPCC, *DEVICE_FLAGS, *extra, "-c", str(CSRC / src), "-o", str(obj)]
     if src.endswith(".cpp"):
-        cmd = [HIPCC, *DEVICE_FLAGS, "-x", "hip", "-c", str(CSRC / src), "-o", str(obj)]
+        cmd = [HIPCC, *DEVICE_FLAGS, *extra, "-x", "hip", "-c", str(CSRC / src), "-o", str(obj)]
     _run(cmd)
     return obj
 
 
-def build_lib(jobs: int = 8, incremental: bool = False) -> Path:
-    BUILD.mkdir(exist_ok=True)
+def build_lib(jobs: int = 8, incremental: bool = False, out: Path = LIB, extra=()) -> Path:
+    """Compiles every source into ``out``.  ``extra`` adds compiler flags (a
+    tuning build, e.g. ``-DMQPIV_OCC_CAP=3``); such builds use their own
+    object directory next to ``out``."""
+    build = BUILD if out == LIB and not extra else out.parent / (out.stem + "_build")
+    build.mkdir(exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(lambda src: _compile(src, incremental), SOURCES))
-    tmp = LIB.with_suffix(".so.tmp")
+        objs = list(ex.map(lambda src: _compile(src, incremental, build, tuple(extra)), SOURCES))
+    tmp = out.with_suffix(".so.tmp")
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)])
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
 
 
 def build_oracle() -> Path:

@@ -272,8 +272,26 @@ __device__ __forceinline__ double sat_box(const double* __restrict__ S, int64_t 
   return ((S[b0 + D0 * b1] - S[a0 + D0 * b1]) - S[b0 + D0 * a1]) + S[a0 + D0 * a1];
 }
 
+// Waves per SIMD that the LDS footprint of one block allows (at most 4): the
+// occupancy the register allocation is asked to fit, so that VGPRs do not
+// become the tighter limit.  The masked measure's extra correlation window is
+// sized for the largest margin a length-L transform can hold (ns <= L/2 + 1).
+#ifndef MQPIV_OCC_CAP
+#define MQPIV_OCC_CAP 4
+#endif
 template <int L, int ALG, typename T, int LPL>
-__global__ __launch_bounds__((Cfg2<L, T, LPL>::NT)) void piv2d_kernel(Job j0, const Job* __restrict__ jobs,
+constexpr int min_waves_2d() {
+  using C = Cfg2<L, T, LPL>;
+  constexpr size_t z = ((size_t)L * (L + 1) * 2 * sizeof(T) + 15) & ~(size_t)15;
+  constexpr size_t c1 = ALG == ALG_MASK_NSQECC ? (((size_t)(L / 2 + 1) * (L / 2 + 1) * sizeof(T) + 15) & ~(size_t)15) : 0;
+  constexpr size_t per_block = (z + c1 + kRedBytes) * C::IPB;
+  constexpr int blocks = (int)((160 * 1024) / per_block);
+  constexpr int waves = blocks * (C::NT / 64) / 4;
+  return waves < 1 ? 1 : (waves > MQPIV_OCC_CAP ? MQPIV_OCC_CAP : waves);
+}
+
+template <int L, int ALG, typename T, int LPL>
+__global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL>())) void piv2d_kernel(Job j0, const Job* __restrict__ jobs,
                                                                       const int64_t* __restrict__ prefix,
                                                                       int n_jobs, int64_t n_ia_total, int ns0_cap,
                                                                       int ns1_cap) {
