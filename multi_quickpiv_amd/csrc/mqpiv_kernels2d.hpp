@@ -507,7 +507,7 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
     const double mu = (dt == DT_F32) ? (double)meanG32 : meanG64;
     const gptr<double> sat2 = as_global<double>(J.sat2);
     const gptr<double> sat1 = as_global<double>(J.sat1);
-    constexpr int NB = 4;
+    constexpr int NB = C::TPI >= 384 ? 2 : 4;  // wide IAs: fewer elements per lane, fewer live registers
     constexpr int NQ = (ALG == ALG_ZNCC) ? 2 : 1;
     const int nE = ns0 * ns1;
     for (int e0 = t; e0 < nE; e0 += NB * C::TPI) {
@@ -625,46 +625,54 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
     sn = nsn > 0 ? mx / ((sm - mx) / (double)(nsn - 1)) : (double)NAN;
   }
 
-  // ---- Gaussian subpixel + write (piv_ops.jl:249-326, 236-243)
+  // ---- Gaussian subpixel + write (piv_ops.jl:249-326, 236-243).  The 3x3
+  // neighbourhood is spread over the IA's first 9 lanes (lane 3*(b1+1)+(b0+1)
+  // holds offset (b0, b1)); its minimum is a shuffle reduction and each lane
+  // takes the log of its own value, so the five logs run side by side; lane 0
+  // combines them.  Same operations as a serial evaluation, bit for bit.
+  const bool has_peak = !skip && bst.i != INT32_MAX;
+  const int pk0 = has_peak ? bst.i % ns0 : 0, pk1 = has_peak ? bst.i / ns0 : 0;  // 0-based
+  const bool interior = has_peak && (pk0 > lo0) && (pk0 < hi0) && (pk1 > lo1) && (pk1 < hi1);
+  double lfit[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // lane 0: log(1 + v - mn) at centre, up, down, left, right
+  if (interior && t < 64) {  // uniform over the IA's first wave; other waves skip
+    const int tt = t < 9 ? t : 4;  // lanes past 8 repeat the centre (neutral for the minimum)
+    const double v = (double)S[(pk0 + tt % 3 - 1) + ns0 * (pk1 + tt / 3 - 1)];
+    double mn = v;
+#pragma unroll
+    for (int m = 1; m < 16; m <<= 1) {
+      const double o = __shfl_xor(mn, m);
+      mn = (mn != mn || o != o) ? (double)NAN : (o < mn ? o : mn);
+    }
+    const double lv = log(1.0 + v - mn);
+    const int lane0 = (int)(threadIdx.x & 63) - (t & 63);  // lane of t = 0 in this wave
+    lfit[0] = __shfl(lv, lane0 + 4);
+    lfit[1] = __shfl(lv, lane0 + 3);
+    lfit[2] = __shfl(lv, lane0 + 5);
+    lfit[3] = __shfl(lv, lane0 + 1);
+    lfit[4] = __shfl(lv, lane0 + 7);
+  }
   if (valid && t == 0) {
     double r0, r1;
     if (skip) {
       r0 = r1 = 0.0;
       sn = 0.0;
-    } else if (bst.i == INT32_MAX) {
+    } else if (!has_peak) {
       r0 = -(double)(sm0 + 1);
       r1 = -(double)(sm1 + 1);
     } else {
-      const int p0 = bst.i % ns0, p1 = bst.i / ns0;  // 0-based; reference peak = p + 1
-      const bool interior = (p0 > lo0) && (p0 < hi0) && (p1 > lo1) && (p1 < hi1);
-      r0 = (double)(p0 + 1) - (double)(sm0 + 1);
-      r1 = (double)(p1 + 1) - (double)(sm1 + 1);
+      r0 = (double)(pk0 + 1) - (double)(sm0 + 1);
+      r1 = (double)(pk1 + 1) - (double)(sm1 + 1);
       if (interior) {
-        double mn = INFINITY;
-        bool nan = false;
-        for (int b1 = -1; b1 <= 1; ++b1)
-          for (int b0 = -1; b0 <= 1; ++b0) {
-            const double v = (double)S[(p0 + b0) + ns0 * (p1 + b1)];
-            if (v != v) nan = true;
-            mn = v < mn ? v : mn;
-          }
-        if (nan) mn = NAN;
-        const double mxv = (double)S[p0 + ns0 * p1];
-        const double lmx = log(1.0 + mxv - mn);
-        const double up = log(1.0 + (double)S[(p0 - 1) + ns0 * p1] - mn);
-        const double dn = log(1.0 + (double)S[(p0 + 1) + ns0 * p1] - mn);
-        const double lf = log(1.0 + (double)S[p0 + ns0 * (p1 - 1)] - mn);
-        const double rt = log(1.0 + (double)S[p0 + ns0 * (p1 + 1)] - mn);
+        const double lmx = lfit[0], up = lfit[1], dn = lfit[2], lf = lfit[3], rt = lfit[4];
         const double q0 = (up - dn) / (2 * up - 4 * lmx + 2 * dn);
         const double q1 = (lf - rt) / (2 * lf - 4 * lmx + 2 * rt);
-        r0 = ((double)(p0 + 1) + q0) - (double)(sm0 + 1);
-        r1 = ((double)(p1 + 1) + q1) - (double)(sm1 + 1);
+        r0 = ((double)(pk0 + 1) + q0) - (double)(sm0 + 1);
+        r1 = ((double)(pk1 + 1) + q1) - (double)(sm1 + 1);
       }
     }
     if (J.peaks) {
-      const bool has = !skip && bst.i != INT32_MAX;
-      J.peaks[2 * ia + 0] = has ? bst.i % ns0 + 1 : 0;
-      J.peaks[2 * ia + 1] = has ? bst.i / ns0 + 1 : 0;
+      J.peaks[2 * ia + 0] = has_peak ? pk0 + 1 : 0;
+      J.peaks[2 * ia + 1] = has_peak ? pk1 + 1 : 0;
     }
     if (J.out_dtype == DT_F32) {
       float* vf = (float*)J.vf;
