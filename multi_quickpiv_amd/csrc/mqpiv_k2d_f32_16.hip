@@ -1,9 +1,10 @@
 // mqpiv_k2d_f32_16.hip -- instances of the fused 2-D kernel:
-// float transforms of length 16, 1 lane(s) per line.
+// float transforms of length 16 (lanes per line: mqpiv_lpl2d.h).
 #include "mqpiv_kernels2d.hpp"
+#include "mqpiv_lpl2d.h"
 
 namespace mqpiv {
-template size_t lds_bytes_2d<16, float, 1>(int, int, int, int);
-template hipError_t launch_2d_l<16, float, 1>(int, const Job&, const Job*, const int64_t*, int, int64_t,
+template size_t lds_bytes_2d<16, float, Lpl2<float, 16>::v>(int, int, int, int);
+template hipError_t launch_2d_l<16, float, Lpl2<float, 16>::v>(int, const Job&, const Job*, const int64_t*, int, int64_t,
                                                 int, int, int, hipStream_t);
 }  // namespace mqpiv

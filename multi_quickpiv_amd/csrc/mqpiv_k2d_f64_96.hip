@@ -1,9 +1,10 @@
 // mqpiv_k2d_f64_96.hip -- instances of the fused 2-D kernel:
-// double transforms of length 96, 8 lane(s) per line.
+// double transforms of length 96 (lanes per line: mqpiv_lpl2d.h).
 #include "mqpiv_kernels2d.hpp"
+#include "mqpiv_lpl2d.h"
 
 namespace mqpiv {
-template size_t lds_bytes_2d<96, double, 8>(int, int, int, int);
-template hipError_t launch_2d_l<96, double, 8>(int, const Job&, const Job*, const int64_t*, int, int64_t,
+template size_t lds_bytes_2d<96, double, Lpl2<double, 96>::v>(int, int, int, int);
+template hipError_t launch_2d_l<96, double, Lpl2<double, 96>::v>(int, const Job&, const Job*, const int64_t*, int, int64_t,
                                                 int, int, int, hipStream_t);
 }  // namespace mqpiv

@@ -221,7 +221,13 @@ __device__ __forceinline__ void correlate_2d(cpx<T> (&x)[L / LPL], cpx<T>* __res
     for (int k = 0; k < M; ++k) Z[line * LD + cM + k] = x[k];
   }
   __syncthreads();
-  if (lane) {
+  // The correlation is real, so P is Hermitian: the dim-1 inverse of column
+  // L - k1 is the conjugate of column k1's.  Only columns k1 <= L/2 are
+  // multiplied and inverted; the dim-2 pass then rebuilds each row's full
+  // spectrum from that half and inverts two real rows per complex transform.
+  constexpr int KH = L / 2 + 1;  // columns kept
+  const bool half = line < KH;
+  if (half) {
     // --- P(k) = conj(A^(k)) B^(k) with Z^ = A^ + i B^ (fft_ops.jl:95-105)
     const int k1n = (L - line) % L;
 #pragma unroll
@@ -238,7 +244,7 @@ __device__ __forceinline__ void correlate_2d(cpx<T> (&x)[L / LPL], cpx<T>* __res
     line_inv<L, LPL, T>(x, q);
   }
   __syncthreads();
-  if (lane) {
+  if (half) {
 #pragma unroll
     for (int m = 0; m < M; ++m) {
       const int s0 = LPL * m + q;
@@ -246,21 +252,38 @@ __device__ __forceinline__ void correlate_2d(cpx<T> (&x)[L / LPL], cpx<T>* __res
     }
   }
   __syncthreads();
-  // --- inverse along dim 2 for the ns0 needed rows: line = s0
-  const bool act = line < ns0;
+  // --- inverse along dim 2, rows s0 = 2r and 2r + 1 packed as Ya + i Yb:
+  // line = r, lane q holds k1 = cM + k of the packed spectrum
+  const int npair = (ns0 + 1) / 2;
+  const bool act = line < npair;
   if (act) {
+    const int sa = 2 * line, sb = 2 * line + 1;
+    const bool has_b = sb < ns0;
 #pragma unroll
-    for (int k = 0; k < M; ++k) x[k] = Z[(cM + k) * LD + line];
+    for (int k = 0; k < M; ++k) {
+      const int k1 = cM + k;
+      const bool mirror = k1 >= KH;
+      const int kc = mirror ? L - k1 : k1;
+      const cpx<T> ya = Z[kc * LD + sa];
+      const cpx<T> yb = has_b ? Z[kc * LD + sb] : cpx<T>{T(0), T(0)};
+      // Y(s, k1) = conj(Y(s, L - k1)) above L/2
+      const T ayi = mirror ? -ya.y : ya.y, byi = mirror ? -yb.y : yb.y;
+      x[k] = cpx<T>{ya.x - byi, ayi + yb.x};
+    }
     line_inv<L, LPL, T>(x, q);
   }
   __syncthreads();
   if (act) {
     T* S = reinterpret_cast<T*>(Z);
     const T kScale = T(1) / T(L * L);
+    const int sa = 2 * line, sb = 2 * line + 1;
 #pragma unroll
     for (int m = 0; m < M; ++m) {
       const int s1 = LPL * m + q;
-      if (s1 < ns1) S[s1 * ns0 + line] = x[m].x * kScale;
+      if (s1 < ns1) {
+        S[s1 * ns0 + sa] = x[m].x * kScale;
+        if (sb < ns0) S[s1 * ns0 + sb] = x[m].y * kScale;
+      }
     }
   }
   __syncthreads();
