@@ -31,6 +31,8 @@ size_t sat_bytes(const int64_t* dims, int nd);
 size_t sat_tmp_bytes(const int64_t* dims, int nd);
 hipError_t build_sat(const void* img, int dt, int power, const int64_t* dims, int nd, double* S, double* tmp,
                      hipStream_t s);
+hipError_t build_sat_pair(const void* img, int dt, const int64_t* dims, int nd, double* S2, double* S1,
+                          double* tmp, hipStream_t s);
 int fft_len_3d(int fp64, int ss);
 size_t scratch_3d_bytes(int fp64, int L, const Job& j);
 hipError_t launch_piv3d(int fp64, int L, const Job& j, void* scratch, const double* sat1_img1, hipStream_t s);
@@ -225,8 +227,8 @@ static int prepare_sats(mqpiv_ctx* c, Job& j, hipStream_t s, char* ws = nullptr)
   }
   char* b = ws ? ws : (char*)c->d_sat;
   double* t = (double*)(b + nsat * one);
-  hipError_t e = build_sat(j.img2, j.in_dtype, 2, j.dims, j.ndims, (double*)b, t, s);
-  if (e == hipSuccess && nsat >= 2) e = build_sat(j.img2, j.in_dtype, 1, j.dims, j.ndims, (double*)(b + one), t, s);
+  hipError_t e = nsat >= 2 ? build_sat_pair(j.img2, j.in_dtype, j.dims, j.ndims, (double*)b, (double*)(b + one), t, s)
+                           : build_sat(j.img2, j.in_dtype, 2, j.dims, j.ndims, (double*)b, t, s);
   if (e == hipSuccess && nsat >= 3)
     e = build_sat(j.img1, j.in_dtype, 1, j.dims, j.ndims, (double*)(b + 2 * one), t, s);
   if (e != hipSuccess) return fail(c, MQPIV_ERR_DEVICE, "integral array build failed: %s", hipGetErrorString(e));
@@ -542,8 +544,8 @@ static int enqueue_2d(mqpiv_ctx* c, BatchPlan& P, char* sat_base, char* d_table,
     if (!P.nsat[i]) continue;
     double* tmp = own_ws ? (double*)((char*)j.sat2 + P.nsat[i] * al256(sat_bytes(j.dims, 2)))
                          : (double*)(sat_base + P.sat_total);
-    hipError_t e = build_sat(j.img2, j.in_dtype, 2, j.dims, 2, (double*)j.sat2, tmp, s);
-    if (e == hipSuccess && j.sat1) e = build_sat(j.img2, j.in_dtype, 1, j.dims, 2, (double*)j.sat1, tmp, s);
+    hipError_t e = j.sat1 ? build_sat_pair(j.img2, j.in_dtype, j.dims, 2, (double*)j.sat2, (double*)j.sat1, tmp, s)
+                          : build_sat(j.img2, j.in_dtype, 2, j.dims, 2, (double*)j.sat2, tmp, s);
     if (e != hipSuccess) return fail(c, MQPIV_ERR_DEVICE, "integral arrays: %s", hipGetErrorString(e));
   }
   for (auto& g : P.groups) {
