@@ -74,7 +74,8 @@ __device__ __forceinline__ Geo3 geo3_of(const Job& J, int64_t ia) {
 // Box sum of a 3-D FP64 integral array over [a, b) per dim.
 __device__ __forceinline__ double sat_box3(const double* __restrict__ S, int64_t D0, int64_t D01, int64_t a0,
                                            int64_t b0, int64_t a1, int64_t b1, int64_t a2, int64_t b2) {
-  auto at = [&](int64_t i, int64_t j, int64_t k) { return S[i + D0 * j + D01 * k]; };
+  const gptr<double> g = as_global<double>(S);
+  auto at = [&](int64_t i, int64_t j, int64_t k) { return g[i + D0 * j + D01 * k]; };
   return (((at(b0, b1, b2) - at(a0, b1, b2)) - at(b0, a1, b2)) - at(b0, b1, a2)) +
          ((at(a0, a1, b2) + at(a0, b1, a2)) + at(b0, a1, a2)) - at(a0, a1, a2);
 }
@@ -136,17 +137,31 @@ __global__ __launch_bounds__((Cfg3<L, T, LPL>::TPL)) void slab_fwd_kernel(Job J,
   const bool want_max = J.has_threshold;
   double sumF2 = 0.0, mx1 = -INFINITY;
   cpx<T> x[M];
+  // Every lane issues all its loads before using any: positions outside the
+  // IA / clipped search region read an in-volume voxel instead (the IA origin,
+  // or the search region's first column in this lane's row and plane) and
+  // their values are discarded, so no load is predicated.
+  auto f_at = [&](int k1) { return f_row && k1 < J.is[1]; };
+  auto g_at = [&](int k1) { return g_row && k1 >= g1.goff && k1 < g1.goff + g1.gsz; };
+  const int64_t f_lane = f_row ? f_base : (g0.tlf + d0 * g1.tlf + d01 * g2.tlf);
+  const int64_t g_lane = g_row ? g_base : (g0.sr + d0 * (g1.sr - g1.goff) + d01 * g2.sr);
   auto load = [&](auto tin) {
     using TIN = decltype(tin);
-    const TIN* pf = (const TIN*)J.img1 + f_base;
-    const TIN* pg = (const TIN*)J.img2 + g_base;
+    const gptr<TIN> pf = as_global<TIN>(J.img1) + f_lane;
+    const gptr<TIN> pg = as_global<TIN>(J.img2) + g_lane;
+    TIN fv[M], gv[M];
 #pragma unroll
     for (int k = 0; k < M; ++k) {
       const int k1 = LPL * k + q;
-      const int64_t o = d0 * k1;
+      fv[k] = pf[f_at(k1) ? d0 * k1 : 0];
+      gv[k] = pg[d0 * (g_at(k1) ? k1 : g1.goff)];
+    }
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      const int k1 = LPL * k + q;
       double a = 0.0, g = 0.0;
-      if (f_row && k1 < J.is[1]) {
-        const TIN f = pf[o];
+      if (f_at(k1)) {
+        const TIN f = fv[k];
         if (want_max) mx1 = fmax(mx1, (double)f);
         if constexpr (ALG == ALG_ZNCC)
           a = std::is_same<TIN, float>::value ? (double)((float)f - mu.f32) : (double)f - mu.f64;
@@ -154,12 +169,11 @@ __global__ __launch_bounds__((Cfg3<L, T, LPL>::TPL)) void slab_fwd_kernel(Job J,
           a = (double)f;
         if constexpr (ALG != ALG_FFTCC) sumF2 += a * a;
       }
-      if (g_row && k1 >= g1.goff && k1 < g1.goff + g1.gsz) {
-        const TIN gv = pg[o];
+      if (g_at(k1)) {
         if constexpr (ALG == ALG_ZNCC)
-          g = std::is_same<TIN, float>::value ? (double)((float)gv - mu.g32) : (double)gv - mu.g64;
+          g = std::is_same<TIN, float>::value ? (double)((float)gv[k] - mu.g32) : (double)gv[k] - mu.g64;
         else
-          g = (double)gv;
+          g = (double)gv[k];
       }
       x[k] = cpx<T>{(T)a, (T)g};
     }
@@ -225,9 +239,13 @@ __global__ __launch_bounds__((Cfg3<L, T, LPL>::NT2)) void zline_kernel(Scratch3 
     for (int k = 0; k < M; ++k) Pl[(pl * L + cM + k) * LD + line] = x[k];
   }
   __syncthreads();
-  if (lane) {
-    // partner of (k0, k1, k2) is (-k0, -k1, -k2): plane kb for ka and vice versa
-    const int pp = (ka == kb) ? 0 : 1 - pl;
+  // The correlation is real: plane L - ka of the product is the reflected
+  // conjugate of plane ka, so only plane ka (<= L/2) is multiplied and
+  // inverted; plane kb was transformed above as its partner.
+  const bool keep = lane && pl == 0;
+  if (keep) {
+    // partner of (k0, k1, k2) is (-k0, -k1, -k2): plane kb
+    const int pp = (ka == kb) ? 0 : 1;
     const int k0n = (L - line) % L;
 #pragma unroll
     for (int k = 0; k < M; ++k) {
@@ -242,7 +260,7 @@ __global__ __launch_bounds__((Cfg3<L, T, LPL>::NT2)) void zline_kernel(Scratch3 
     line_inv<L, LPL, T>(x, q);
   }
   __syncthreads();  // every forward value of these planes is read before any write
-  if (lane) {
+  if (keep) {
 #pragma unroll
     for (int m = 0; m < M; ++m) {
       const int s2 = LPL * m + q;
@@ -268,8 +286,12 @@ __global__ __launch_bounds__((Cfg3<L, T, LPL>::TPL)) void slab_inv_kernel(Scratc
   const cpx<T>* W = (const cpx<T>*)sc.w + il * sc.w_stride + (int64_t)s2 * L * L;
   const int cM = (LPL == 1 ? 0 : chunk_of<LPL>(q)) * M;
   cpx<T> x[M];
-  // inverse along dim 1: line = k1, values over k0 in this lane's chunk
-  if (line < L) {
+  // Each plane's spectrum is Hermitian in (k0, k1) (the correlation is real)
+  // and only columns k1 <= L/2 were written: inverse along dim 1 for those,
+  // then along dim 2 with two real rows packed per complex transform and the
+  // columns above L/2 rebuilt as conjugates.
+  constexpr int KH = L / 2 + 1;
+  if (line < KH) {
 #pragma unroll
     for (int k = 0; k < M; ++k) x[k] = W[(int64_t)line * L + cM + k];
     line_inv<L, LPL, T>(x, q);
@@ -280,17 +302,30 @@ __global__ __launch_bounds__((Cfg3<L, T, LPL>::TPL)) void slab_inv_kernel(Scratc
     }
   }
   __syncthreads();
-  // inverse along dim 2 for the ns0 needed rows: line = s0
-  if (line < ns0) {
+  const int npair = (ns0 + 1) / 2;
+  if (line < npair) {
+    const int sa = 2 * line, sb = 2 * line + 1;
+    const bool has_b = sb < ns0;
 #pragma unroll
-    for (int k = 0; k < M; ++k) x[k] = Z[(cM + k) * LD + line];
+    for (int k = 0; k < M; ++k) {
+      const int k1 = cM + k;
+      const bool mirror = k1 >= KH;
+      const int kc = mirror ? L - k1 : k1;
+      const cpx<T> ya = Z[kc * LD + sa];
+      const cpx<T> yb = has_b ? Z[kc * LD + sb] : cpx<T>{T(0), T(0)};
+      const T ayi = mirror ? -ya.y : ya.y, byi = mirror ? -yb.y : yb.y;
+      x[k] = cpx<T>{ya.x - byi, ayi + yb.x};
+    }
     line_inv<L, LPL, T>(x, q);
     T* S = (T*)sc.s + il * sc.s_stride + (int64_t)s2 * ns0 * ns1;
     const T kScale = T(1) / T(L * L * L);
 #pragma unroll
     for (int m = 0; m < M; ++m) {
       const int s1 = LPL * m + q;
-      if (s1 < ns1) S[s1 * ns0 + line] = x[m].x * kScale;
+      if (s1 < ns1) {
+        S[s1 * ns0 + sa] = x[m].x * kScale;
+        if (has_b) S[s1 * ns0 + sb] = x[m].y * kScale;
+      }
     }
   }
 }
@@ -331,40 +366,72 @@ __global__ __launch_bounds__(256) void finalize3d_kernel(Job J, Scratch3 sc, int
       mu = (J.in_dtype == DT_F32) ? (double)m.g32 : m.g64;
     }
     const double sqF = sqrt(sumF2);
-    for (int e = t; e < nS; e += NT) {
-      const int s0 = e % ns0, s1 = (e / ns0) % ns1, s2 = e / (ns0 * ns1);
-      int lo[3], hi[3];
-      const int s[3] = {s0, s1, s2};
-      const int isz[3] = {is0, is1, is2};
-      bool empty = false;
-      for (int d = 0; d < 3; ++d) {
-        const Geo& g = G.g[d];
-        lo[d] = s[d] > g.goff ? s[d] : g.goff;
-        hi[d] = (s[d] + isz[d]) < (g.goff + g.gsz) ? (s[d] + isz[d]) : (g.goff + g.gsz);
-        if (hi[d] <= lo[d]) empty = true;
-      }
-      double W = 0.0;
-      if (!empty) {
-        const int64_t a0 = g0.sr + (lo[0] - g0.goff), b0 = g0.sr + (hi[0] - g0.goff);
-        const int64_t a1 = g1.sr + (lo[1] - g1.goff), b1 = g1.sr + (hi[1] - g1.goff);
-        const int64_t a2 = g2.sr + (lo[2] - g2.goff), b2 = g2.sr + (hi[2] - g2.goff);
-        W = sat_box3(J.sat2, D0, D01, a0, b0, a1, b1, a2, b2);
-        if constexpr (ALG == ALG_ZNCC) {
-          const double n = (double)((b0 - a0) * (b1 - a1) * (b2 - a2));
-          W = (W - 2.0 * mu * sat_box3(J.sat1, D0, D01, a0, b0, a1, b1, a2, b2)) + n * mu * mu;
+    // NB window positions per lane per round, all 8 (16 for ZNCC) corner loads
+    // issued before any use; corner indices are clamped into the search region
+    // and an empty window's sum is replaced by 0 afterwards.
+    constexpr int NB = 2;
+    const gptr<double> sat2 = as_global<double>(J.sat2);
+    const gptr<double> sat1 = as_global<double>(J.sat1);
+    for (int e0 = t; e0 < nS; e0 += NB * NT) {
+      double q2[NB][8], q1[NB][8], nbox[NB];
+      bool empty[NB];
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        const int e = e0 + j * NT;
+        const int ec = e < nS ? e : 0;
+        const int s[3] = {ec % ns0, (ec / ns0) % ns1, ec / (ns0 * ns1)};
+        const int isz[3] = {is0, is1, is2};
+        int64_t a[3], bb[3];
+        empty[j] = false;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const Geo& g = G.g[d];
+          const int lo = s[d] > g.goff ? s[d] : g.goff;
+          int hi = (s[d] + isz[d]) < (g.goff + g.gsz) ? (s[d] + isz[d]) : (g.goff + g.gsz);
+          if (hi <= lo) {
+            empty[j] = true;
+            hi = lo;
+          }
+          a[d] = g.sr + (lo - g.goff);
+          bb[d] = g.sr + (hi - g.goff);
         }
-        W = W > 0.0 ? W : 0.0;
+        // corner order of sat_box3: (b,b,b) (a,b,b) (b,a,b) (b,b,a) (a,a,b) (a,b,a) (b,a,a) (a,a,a)
+        const int64_t idx[8] = {bb[0] + D0 * bb[1] + D01 * bb[2], a[0] + D0 * bb[1] + D01 * bb[2],
+                                bb[0] + D0 * a[1] + D01 * bb[2],  bb[0] + D0 * bb[1] + D01 * a[2],
+                                a[0] + D0 * a[1] + D01 * bb[2],   a[0] + D0 * bb[1] + D01 * a[2],
+                                bb[0] + D0 * a[1] + D01 * a[2],   a[0] + D0 * a[1] + D01 * a[2]};
+#pragma unroll
+        for (int c = 0; c < 8; ++c) q2[j][c] = sat2[idx[c]];
+        if constexpr (ALG == ALG_ZNCC) {
+#pragma unroll
+          for (int c = 0; c < 8; ++c) q1[j][c] = sat1[idx[c]];
+        }
+        nbox[j] = (double)((bb[0] - a[0]) * (bb[1] - a[1]) * (bb[2] - a[2]));
       }
-      const double c = (double)S[e];
-      const double den = sqrt(W) * sqF;
-      double r;
-      if constexpr (ALG == ALG_ZNCC) {
-        r = c / den;
-      } else {
-        const double num = (sumF2 - 2.0 * c) + W;
-        r = den / (num + den);
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        const int e = e0 + j * NT;
+        if (e >= nS) break;
+        auto box = [&](const double (&v)[8]) {
+          return (((v[0] - v[1]) - v[2]) - v[3]) + ((v[4] + v[5]) + v[6]) - v[7];
+        };
+        double W = 0.0;
+        if (!empty[j]) {
+          W = box(q2[j]);
+          if constexpr (ALG == ALG_ZNCC) W = (W - 2.0 * mu * box(q1[j])) + nbox[j] * mu * mu;
+          W = W > 0.0 ? W : 0.0;
+        }
+        const double c = (double)S[e];
+        const double den = sqrt(W) * sqF;
+        double r;
+        if constexpr (ALG == ALG_ZNCC) {
+          r = c / den;
+        } else {
+          const double num = (sumF2 - 2.0 * c) + W;
+          r = den / (num + den);
+        }
+        S[e] = (T)r;
       }
-      S[e] = (T)r;
     }
     __syncthreads();
   }
