@@ -15,13 +15,15 @@ time.  The CPU baseline is the Float64 oracle's FFT mode (a restatement of the
 reference's PIV_CPU loop with per-IA transforms) over the same four runs,
 OpenMP over IAs on the host cores.
 
+A 3-D leg ("volume_3d") times FFTCC and ZNCC on FP32 512 x 512 x 256 particle
+volumes (interSize 32^3, searchMargin 8, step 16) with its own CPU sample.
 A serving leg follows ("serving"): a Poisson stream (200 pairs/s) of mixed 2-D
 and 3-D requests over frames resident in HBM, served by the continuous-batching
 server and by the plain per-request path on the same schedule, then the same
 requests all at once (saturation); p50/p99 latency and throughput of both, and
 a bit-for-bit comparison of their outputs.
 
-    python bench.py [--steps K] [--warmup W] [--mode auto|fp32|fp64] [--no-cpu] [--no-serve]
+    python bench.py [--steps K] [--warmup W] [--mode auto|fp32|fp64] [--no-cpu] [--no-serve] [--no-3d]
     python bench.py --serve [--serve-requests N] [--serve-rate R]    (serving leg only)
 """
 from __future__ import annotations
@@ -236,6 +238,66 @@ def serve_bench(n_requests: int, rates, seed: int, n_slots: int = 4, max_batch: 
     return out
 
 
+def volume_bench(mode: str, steps: int, cpu: bool, nthreads: int) -> dict:
+    """The 3-D workload: FP32 512 x 512 x 256 particle volumes at 0.01
+    particles/voxel (sigma 1.5), shear u = 0.01 z plus drift (2.3, -1.1, 0.7);
+    FFTCC and ZNCC at interSize 32^3, searchMargin 8, step 16 (14,415 IAs, the
+    reference transform size 48^3).  IA/s over `steps` timed calls with the
+    volumes resident in HBM, kernel time from the library's events, and the
+    Float64 oracle (FFT mode) on a sample of the same IAs."""
+    import torch
+
+    from multi_quickpiv_amd import _lib, setPIVParameters, synth
+    from multi_quickpiv_amd.params import get_vectorfield_size
+    from multi_quickpiv_amd.piv import piv_device
+
+    a, b = synth.config_c3(device="cuda")
+    shape = tuple(a.shape)
+    A = a.permute(2, 1, 0).contiguous()
+    B = b.permute(2, 1, 0).contiguous()
+    ctx = _lib.context(0)
+    st = torch.cuda.Stream()
+    out = {"workload": "3D FP32 512x512x256 particle volumes, shear + drift, interSize 32^3, searchMargin 8, "
+                       "step 16 (C3)"}
+    flop = flop_fft_per_ia(48, ndims=3)
+    for alg in ("fft", "zncc"):
+        p = setPIVParameters(corr_alg=alg, interSize=32, searchMargin=8, step=16)
+        p.ndims = 3
+        vf = get_vectorfield_size(shape, p)
+        n = int(np.prod(vf))
+        VF = torch.zeros(3 * n, dtype=torch.float64, device="cuda")
+
+        def run():
+            piv_device(ctx, p, shape, _lib.MQPIV_F32, A.data_ptr(), B.data_ptr(), None, _lib.MQPIV_F64,
+                       VF.data_ptr(), None, st.cuda_stream, mode=mode)
+
+        run()
+        torch.cuda.synchronize()
+        ctx.profile_begin()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            run()
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / steps
+        kms = float(np.mean(ctx.profile_end()))
+        out[alg] = {"n_ia": n, "ms_per_call": round(wall * 1e3, 3), "ia_per_s": round(n / wall, 1),
+                    "kernel_ms": round(kms, 3), "achieved_tflops": round(n * flop / (kms * 1e-3) / 1e12, 3),
+                    "flop_per_ia": round(flop)}
+    if cpu:
+        from oracle import oracle as O
+
+        ha, hb = a.cpu().numpy(), b.cpu().numpy()
+        n_s = 1500
+        op = O.make_params("fft", 3, 32, 8, 16, False, np.float32)
+        t0 = time.perf_counter()
+        O.piv(ha, hb, op, mode=O.FFT, nthreads=nthreads, ia_begin=0, ia_end=n_s)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(n_s / dt, 1), "unit": "IA/s", "cores": nthreads, "kind": "port",
+                               "sample": f"FFTCC, the first {n_s} IAs of the workload, Float64 oracle FFT mode "
+                                         f"(48^3 transforms), OpenMP over IAs, {dt:.1f} s"}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -252,6 +314,7 @@ def main() -> int:
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg of the default run")
+    ap.add_argument("--no-3d", action="store_true", help="skip the 3-D leg of the default run")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-batch", type=int, default=32)
     args = ap.parse_args()
@@ -369,6 +432,11 @@ def main() -> int:
                           "call_ms": round(call_ms[k], 4),
                           "ia_per_s": round(j["n"] / (kern_ms[k] * 1e-3), 1)})
 
+    nthreads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    volume = None
+    if not args.no_3d:
+        volume = volume_bench(args.mode, 3, not args.no_cpu, nthreads)
+
     serving = None
     if not args.no_serve:
         serving = serve_bench(args.serve_requests, [args.serve_rate, 0.0], 11, args.serve_slots,
@@ -378,7 +446,6 @@ def main() -> int:
     if not args.no_cpu:
         from oracle import oracle as O
 
-        nthreads = args.cpu_threads or min(16, os.cpu_count() or 1)
         ha = a.cpu().numpy()
         hb = b.cpu().numpy()
         t_cpu = 0.0
@@ -425,6 +492,7 @@ def main() -> int:
                      "kernel_ms": round(kern_ms[kd], 4)},
         "breakdown": breakdown,
         "outputs_finite": finite,
+        "volume_3d": volume,
         "serving": serving,
         "cpu_baseline": cpu,
     }
