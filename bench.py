@@ -315,6 +315,7 @@ def main() -> int:
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg of the default run")
     ap.add_argument("--no-3d", action="store_true", help="skip the 3-D leg of the default run")
+    ap.add_argument("--only-3d", action="store_true", help="run only the 3-D leg (no CPU sample)")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-batch", type=int, default=32)
     args = ap.parse_args()
@@ -328,6 +329,11 @@ def main() -> int:
     from multi_quickpiv_amd import _lib, setPIVParameters, synth
     from multi_quickpiv_amd.params import get_vectorfield_size
     from multi_quickpiv_amd.piv import piv_device
+
+    if args.only_3d:
+        torch.cuda.set_device(0)
+        print(json.dumps({"volume_3d": volume_bench(args.mode, args.steps, False, 1)}))
+        return 0
 
     if args.serve:
         torch.cuda.set_device(0)
@@ -413,7 +419,7 @@ def main() -> int:
     achieved = flop / (kern_ms[kd] * 1e-3) / 1e12
     peak = PEAK_FP64_TFLOPS if jd["fp64"] else PEAK_FP32_TFLOPS
     traffic = None
-    tfile = ROOT / "profiles" / "hbm_traffic.md"
+    tfile = ROOT / "profiles" / "piv2d_2048_zncc_nsqecc_is32_is64.md"
     if tfile.exists():
         for line in tfile.read_text().splitlines():
             if line.startswith(f"traffic_bytes[{jd['alg']}{jd['isz']}_{'f64' if jd['fp64'] else 'f32'}]"):

@@ -295,12 +295,12 @@ __device__ __forceinline__ double sat_box(const double* __restrict__ S, int64_t 
   return ((S[b0 + D0 * b1] - S[a0 + D0 * b1]) - S[b0 + D0 * a1]) + S[a0 + D0 * a1];
 }
 
-// Waves per SIMD that the LDS footprint of one block allows (at most 4): the
+// Waves per SIMD that the LDS footprint of one block allows (at most 8): the
 // occupancy the register allocation is asked to fit, so that VGPRs do not
 // become the tighter limit.  The masked measure's extra correlation window is
 // sized for the largest margin a length-L transform can hold (ns <= L/2 + 1).
 #ifndef MQPIV_OCC_CAP
-#define MQPIV_OCC_CAP 4
+#define MQPIV_OCC_CAP 8
 #endif
 template <int L, int ALG, typename T, int LPL>
 constexpr int min_waves_2d() {
@@ -321,8 +321,10 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
   using C = Cfg2<L, T, LPL>;
   constexpr int H = C::H;
   extern __shared__ __align__(16) unsigned char smem[];
-  const int t = threadIdx.x % C::TPI;
-  const int sub = threadIdx.x / C::TPI;
+  // one IA per block: every IA quantity below is uniform over the block, which
+  // lets the compiler keep it in scalar registers
+  const int t = C::IPB == 1 ? (int)threadIdx.x : (int)(threadIdx.x % C::TPI);
+  const int sub = C::IPB == 1 ? 0 : (int)(threadIdx.x / C::TPI);
   const int line = t / LPL;
   const int h = t % LPL;  // lane within the line's group
 
