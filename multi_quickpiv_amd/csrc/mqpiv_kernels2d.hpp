@@ -85,6 +85,32 @@ struct Group {
     }
     return v;
   }
+
+  // Two reductions sharing one exchange through LDS (slots 0-15 and 16-31).
+  template <typename OpA, typename OpB>
+  __device__ __forceinline__ static void reduce2(double& a, double& b, OpA opa, OpB opb, double* red, int t) {
+#pragma unroll
+    for (int m = 1; m < kSpan; m <<= 1) {
+      a = opa(a, __shfl_xor(a, m));
+      b = opb(b, __shfl_xor(b, m));
+    }
+    if constexpr (TPI > 64) {
+      __syncthreads();
+      if ((t & 63) == 0) {
+        red[t >> 6] = a;
+        red[16 + (t >> 6)] = b;
+      }
+      __syncthreads();
+      double ra = red[0], rb = red[16];
+#pragma unroll
+      for (int w = 1; w < (TPI + 63) / 64; ++w) {
+        ra = opa(ra, red[w]);
+        rb = opb(rb, red[16 + w]);
+      }
+      a = ra;
+      b = rb;
+    }
+  }
 };
 
 struct OpAdd {
@@ -386,38 +412,10 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
 
   // ---- ZNCC means over the IA and the clipped search region (zncc_ops.jl:10,22;
   // computed in the input element type, quirk Q12)
+  // ZNCC means over the IA and the clipped search region (zncc_ops.jl:10,22),
+  // computed in the input element type (quirk Q12) from the loaded window
   float meanF32 = 0.f, meanG32 = 0.f;
   double meanF64 = 0.0, meanG64 = 0.0;
-  if constexpr (ALG == ALG_ZNCC) {
-    double sf = 0.0, sg = 0.0;
-    auto sums = [&](auto tin) {
-      using TIN = decltype(tin);
-      const gptr<TIN> pf = as_global<TIN>(J.img1) + f_lane;
-      const gptr<TIN> pg = as_global<TIN>(J.img2) + g_lane;
-      TIN fv[H], gv[H];
-#pragma unroll
-      for (int k = 0; k < H; ++k) {
-        fv[k] = pf[f_off(k1_of(k))];
-        gv[k] = pg[g_off(k1_of(k))];
-      }
-#pragma unroll
-      for (int k = 0; k < H; ++k) {
-        const int k1 = k1_of(k);
-        if (f_at(k1)) sf += (double)fv[k];
-        if (g_at(k1)) sg += (double)gv[k];
-      }
-    };
-    if (dt == DT_F32) sums(float{});
-    else sums(double{});
-    sf = Group<C::TPI>::reduce(sf, OpAdd{}, red, t);
-    sg = Group<C::TPI>::reduce(sg, OpAdd{}, red, t);
-    const int64_t nF = (int64_t)is0 * is1;
-    const int64_t nG = (int64_t)g0.gsz * g1.gsz;
-    meanF32 = (float)sf / (float)nF;
-    meanG32 = (float)sg / (float)nG;
-    meanF64 = sf / (double)nF;
-    meanG64 = sg / (double)nG;
-  }
 
   // ---- load the packed window into registers; sum(F^2) and the IA maxima on the way
   cpx<T> x[H];
@@ -437,6 +435,22 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
       const gptr<TIN> pm = as_global<TIN>(J.mask) + f_lane;
 #pragma unroll
       for (int k = 0; k < H; ++k) mv[k] = pm[f_off(k1_of(k))];
+    }
+    if constexpr (ALG == ALG_ZNCC) {
+      double sf = 0.0, sg = 0.0;
+#pragma unroll
+      for (int k = 0; k < H; ++k) {
+        const int k1 = k1_of(k);
+        if (f_at(k1)) sf += (double)fv[k];
+        if (g_at(k1)) sg += (double)gv[k];
+      }
+      Group<C::TPI>::reduce2(sf, sg, OpAdd{}, OpAdd{}, red, t);
+      const int64_t nF = (int64_t)is0 * is1;
+      const int64_t nG = (int64_t)g0.gsz * g1.gsz;
+      meanF32 = (float)sf / (float)nF;
+      meanG32 = (float)sg / (float)nG;
+      meanF64 = sf / (double)nF;
+      meanG64 = sg / (double)nG;
     }
 #pragma unroll
     for (int k = 0; k < H; ++k) {
@@ -479,8 +493,7 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
   } else if (want_max) {
     // (mx2, the mask maximum, is only gathered by the masked measure, which is
     // the only one the host gives a mask threshold)
-    mx1 = Group<C::TPI>::reduce(mx1, OpMax{}, red, t);
-    mx2 = Group<C::TPI>::reduce(mx2, OpMax{}, red, t);
+    Group<C::TPI>::reduce2(mx1, mx2, OpMax{}, OpMax{}, red, t);
     if (J.has_threshold && mx1 < J.threshold) skip = 1;
     if (J.has_mask_threshold && mx2 < J.mask_threshold) skip = 1;
   }
@@ -645,8 +658,7 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
       mx = (v > mx || v != v) ? v : mx;
       sm += v;
     }
-    mx = Group<C::TPI>::reduce(mx, OpMaxNaN{}, red, t);
-    sm = Group<C::TPI>::reduce(sm, OpAdd{}, red, t);
+    Group<C::TPI>::reduce2(mx, sm, OpMaxNaN{}, OpAdd{}, red, t);
     sn = nsn > 0 ? mx / ((sm - mx) / (double)(nsn - 1)) : (double)NAN;
   }
 
