@@ -79,7 +79,7 @@ struct Group {
       if ((t & 63) == 0) rv[t >> 6] = v;
       __syncthreads();
       V r = rv[0];
-#pragma unroll
+#pragma unroll 4
       for (int w = 1; w < (TPI + 63) / 64; ++w) r = op(r, rv[w]);
       v = r;
     }
@@ -102,7 +102,7 @@ struct Group {
       }
       __syncthreads();
       double ra = red[0], rb = red[16];
-#pragma unroll
+#pragma unroll 4
       for (int w = 1; w < (TPI + 63) / 64; ++w) {
         ra = opa(ra, red[w]);
         rb = opb(rb, red[16 + w]);
@@ -199,7 +199,7 @@ __host__ __device__ inline size_t region_bytes_2d(int L, int tsize, int ns0, int
   (void)ss0;
   return ((size_t)L * (L + 1) * 2 * tsize + 15) & ~(size_t)15;
 }
-constexpr int kRedBytes = 256;  // reduction slots per IA (16 values + 16 indices)
+constexpr int kRedBytes = 512;  // reduction slots per IA (4 x 16 doubles)
 
 // ---------------------------------------------------------------------------
 // The correlation of one IA.  LPL lanes (a "group", q = lane within it) share a
@@ -531,7 +531,28 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
     correlate_2d<L, LPL, T>(x, Z, t, ns0, ns1);
   }
 
-  // ---- normalisation on the fully-overlapping window
+  // ---- normalisation, peak and SN in one pass.  Each lane normalises its
+  // window elements e = t + j*TPI, writes them back for the Gaussian fit and
+  // folds them into its candidates: the peak over [off_lo, 2SM - off_hi] per
+  // dim (common_ops.jl:98-124; first strict maximum in column-major order,
+  // starting from 0) and the SN maximum and sum over [off_lo, 2SM-1-off_hi]
+  // (piv_ops.jl:336-367).  One combined reduction follows.
+  const int lo0 = g0.goff, hi0 = 2 * sm0 - g0.off_hi;
+  const int lo1 = g1.goff, hi1 = 2 * sm1 - g1.off_hi;
+  const int shi0 = 2 * sm0 - 1 - g0.off_hi, shi1 = 2 * sm1 - 1 - g1.off_hi;
+  const bool want_sn = J.compute_sn != 0;
+  const int nE = ns0 * ns1;
+  Best<T> bst{T(0), INT32_MAX};
+  double mx = -INFINITY, smv = 0.0;
+  auto visit = [&](int e, T v) {
+    const int p0 = e % ns0, p1 = e / ns0;
+    if (p0 >= lo0 && p0 <= hi0 && p1 >= lo1 && p1 <= hi1 && v > bst.v) bst = Best<T>{v, e};
+    if (want_sn && p0 >= lo0 && p0 <= shi0 && p1 >= lo1 && p1 <= shi1) {
+      const double dv = (double)v;
+      mx = (dv > mx || dv != dv) ? dv : mx;
+      smv += dv;
+    }
+  };
   if constexpr (ALG == ALG_ZNCC || ALG == ALG_NSQECC) {
     // W(s) = sum of G'^2 over the search-buffer window [s, s + is) (the SAT
     // window sums of common_ops.jl:203-245), from the image's FP64 integral
@@ -547,7 +568,6 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
     const gptr<double> sat1 = as_global<double>(J.sat1);
     constexpr int NB = C::TPI >= 384 ? 2 : 4;  // wide IAs: fewer elements per lane, fewer live registers
     constexpr int NQ = (ALG == ALG_ZNCC) ? 2 : 1;
-    const int nE = ns0 * ns1;
     for (int e0 = t; e0 < nE; e0 += NB * C::TPI) {
       double q[NB][NQ][4];
       double nbox[NB];
@@ -556,14 +576,14 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
         const int e = e0 + j * C::TPI;
         const int ec = e < nE ? e : 0;
         const int s0 = ec % ns0, s1 = ec / ns0;
-        const int lo0 = s0 > g0.goff ? s0 : g0.goff;
-        int hi0 = (s0 + is0) < (g0.goff + g0.gsz) ? (s0 + is0) : (g0.goff + g0.gsz);
-        hi0 = hi0 > lo0 ? hi0 : lo0;
-        const int lo1 = s1 > g1.goff ? s1 : g1.goff;
-        int hi1 = (s1 + is1) < (g1.goff + g1.gsz) ? (s1 + is1) : (g1.goff + g1.gsz);
-        hi1 = hi1 > lo1 ? hi1 : lo1;
-        const int64_t a0 = g0.sr + (lo0 - g0.goff), b0 = g0.sr + (hi0 - g0.goff);
-        const int64_t a1 = g1.sr + (lo1 - g1.goff), b1 = g1.sr + (hi1 - g1.goff);
+        const int lo0w = s0 > g0.goff ? s0 : g0.goff;
+        int hi0w = (s0 + is0) < (g0.goff + g0.gsz) ? (s0 + is0) : (g0.goff + g0.gsz);
+        hi0w = hi0w > lo0w ? hi0w : lo0w;
+        const int lo1w = s1 > g1.goff ? s1 : g1.goff;
+        int hi1w = (s1 + is1) < (g1.goff + g1.gsz) ? (s1 + is1) : (g1.goff + g1.gsz);
+        hi1w = hi1w > lo1w ? hi1w : lo1w;
+        const int64_t a0 = g0.sr + (lo0w - g0.goff), b0 = g0.sr + (hi0w - g0.goff);
+        const int64_t a1 = g1.sr + (lo1w - g1.goff), b1 = g1.sr + (hi1w - g1.goff);
         const int64_t c11 = b0 + D0 * b1, c01 = a0 + D0 * b1, c10 = b0 + D0 * a1, c00 = a0 + D0 * a1;
         q[j][0][0] = sat2[c11];
         q[j][0][1] = sat2[c01];
@@ -597,69 +617,60 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
           r = den / (num + den);                     // nsqecc_ops.jl:72-73
         }
         S[e] = (T)r;
+        visit(e, (T)r);
       }
     }
-    __syncthreads();
   } else if constexpr (ALG == ALG_MASK_NSQECC) {
     const double sqF = sqrt(sumF2);
-    for (int i = t; i < ns0 * ns1; i += C::TPI) {
+    for (int i = t; i < nE; i += C::TPI) {
       const double c1 = (double)C1[i];
       const double c2 = (double)S[i];
       const double num = (-2.0 * c1 + sumF2) + c2;  // masked_nsqecc_ops.jl:130-132
       const double den = sqrt(fabs(c2)) * sqF;      // masked_nsqecc_ops.jl:135-136
       S[i] = (T)(den / (num + den));
+      visit(i, (T)(den / (num + den)));
     }
-    __syncthreads();
+  } else {
+    for (int i = t; i < nE; i += C::TPI) visit(i, S[i]);
   }
-
-  // ---- peak over [off_lo, 2SM - off_hi] per dim (common_ops.jl:98-124)
-  const int lo0 = g0.goff, hi0 = 2 * sm0 - g0.off_hi;
-  const int lo1 = g1.goff, hi1 = 2 * sm1 - g1.off_hi;
-  Best<T> bst{T(0), INT32_MAX};
-  const int w0 = hi0 - lo0 + 1, w1 = hi1 - lo1 + 1;
-  const int nwin = (w0 > 0 && w1 > 0) ? w0 * w1 : 0;
-  for (int e = t; e < nwin; e += C::TPI) {
-    const int p0 = lo0 + e % w0, p1 = lo1 + e / w0;
-    const int li = p0 + ns0 * p1;
-    const T v = S[li];
-    if (v > bst.v) bst = Best<T>{v, li};
-  }
+  // combined reduction: (peak value, index), SN maximum, SN sum; its barrier
+  // also makes the normalised window visible to the fit below
   {
 #pragma unroll
     for (int m = 1; m < Group<C::TPI>::kSpan; m <<= 1) {
       Best<T> o{__shfl_xor(bst.v, m), __shfl_xor(bst.i, m)};
       bst = best_of(bst, o);
+      mx = OpMaxNaN{}(mx, __shfl_xor(mx, m));
+      smv += __shfl_xor(smv, m);
     }
+    __syncthreads();
     if constexpr (C::TPI > 64) {
-      __syncthreads();
-      T* rv = reinterpret_cast<T*>(red);
-      int32_t* ri = reinterpret_cast<int32_t*>(red + 16);
       if ((t & 63) == 0) {
-        rv[t >> 6] = bst.v;
-        ri[t >> 6] = bst.i;
+        const int w = t >> 6;
+        red[w] = (double)bst.v;
+        red[16 + w] = (double)bst.i;
+        red[32 + w] = mx;
+        red[48 + w] = smv;
       }
       __syncthreads();
-      Best<T> r{rv[0], ri[0]};
-      for (int w = 1; w < C::NW; ++w) r = best_of(r, Best<T>{rv[w], ri[w]});
+      Best<T> r{(T)red[0], (int32_t)red[16]};
+      double rm = red[32], rs = red[48];
+#pragma unroll 4
+      for (int w = 1; w < C::NW; ++w) {
+        r = best_of(r, Best<T>{(T)red[w], (int32_t)red[16 + w]});
+        rm = OpMaxNaN{}(rm, red[32 + w]);
+        rs += red[48 + w];
+      }
       bst = r;
+      mx = rm;
+      smv = rs;
     }
   }
-
-  // ---- SN over [off_lo, 2SM-1-off_hi] (piv_ops.jl:336-367)
   double sn = 0.0;
-  if (J.compute_sn) {
-    const int shi0 = 2 * sm0 - 1 - g0.off_hi, shi1 = 2 * sm1 - 1 - g1.off_hi;
+  if (want_sn) {
     const int u0 = shi0 - lo0 + 1, u1 = shi1 - lo1 + 1;
     const int nsn = (u0 > 0 && u1 > 0) ? u0 * u1 : 0;
-    double mx = -INFINITY, sm = 0.0;
-    for (int e = t; e < nsn; e += C::TPI) {
-      const int p0 = lo0 + e % u0, p1 = lo1 + e / u0;
-      const double v = (double)S[p0 + ns0 * p1];
-      mx = (v > mx || v != v) ? v : mx;
-      sm += v;
-    }
-    Group<C::TPI>::reduce2(mx, sm, OpMaxNaN{}, OpAdd{}, red, t);
-    sn = nsn > 0 ? mx / ((sm - mx) / (double)(nsn - 1)) : (double)NAN;
+    sn = nsn > 0 ? mx / ((smv - mx) / (double)(nsn - 1)) : (double)NAN;
   }
 
   // ---- Gaussian subpixel + write (piv_ops.jl:249-326, 236-243).  The 3x3
