@@ -15,6 +15,8 @@ time.  The CPU baseline is the Float64 oracle's FFT mode (a restatement of the
 reference's PIV_CPU loop with per-IA transforms) over the same four runs,
 OpenMP over IAs on the host cores.
 
+A batched leg ("batched_masked") runs 16 uneven masked NSQECC pairs (512^2 to
+4096^2, interSize 16/32/64) as one captured HIP graph and as 16 separate calls.
 A 3-D leg ("volume_3d") times FFTCC and ZNCC on FP32 512 x 512 x 256 particle
 volumes (interSize 32^3, searchMargin 8, step 16) with its own CPU sample.
 A serving leg follows ("serving"): a Poisson stream (200 pairs/s) of mixed 2-D
@@ -23,7 +25,7 @@ server and by the plain per-request path on the same schedule, then the same
 requests all at once (saturation); p50/p99 latency and throughput of both, and
 a bit-for-bit comparison of their outputs.
 
-    python bench.py [--steps K] [--warmup W] [--mode auto|fp32|fp64] [--no-cpu] [--no-serve] [--no-3d]
+    python bench.py [--steps K] [--warmup W] [--mode auto|fp32|fp64] [--no-cpu] [--no-serve] [--no-3d] [--no-batched]
     python bench.py --serve [--serve-requests N] [--serve-rate R]    (serving leg only)
 """
 from __future__ import annotations
@@ -298,6 +300,64 @@ def volume_bench(mode: str, steps: int, cpu: bool, nthreads: int) -> dict:
     return out
 
 
+def batched_bench(mode: str, steps: int, seed: int = 4000) -> dict:
+    """Batched uneven masked NSQECC (one graph launch): 16 FP32 particle pairs
+    with sides drawn from the multiples of 256 in [512, 4096], uniform shifts
+    U[-6, 6]^2, a 20-ellipse mask (about 30% excluded) per pair, interSize
+    drawn from {16, 32, 64} with searchMargin = step = interSize / 2.  The
+    whole set runs as one captured HIP graph (integral-array-free masked
+    measure: one grouped kernel launch per precision / transform length);
+    the plain path runs the same pairs one call after another."""
+    import torch
+
+    from multi_quickpiv_amd import _lib, setPIVParameters, synth
+    from multi_quickpiv_amd.piv import BatchGraph, DeviceRequest, piv_batch
+
+    rng = np.random.default_rng(seed)
+    reqs = []
+    sides = []
+    for i in range(16):
+        S = int(rng.choice(np.arange(512, 4097, 256)))
+        isz = int(rng.choice([16, 32, 64]))
+        d = tuple(rng.uniform(-6, 6, size=2))
+        a, b = synth.particle_pair((S, S), synth.n_particles((S, S), 0.05), 1.2, synth.uniform(d), seed + i,
+                                   device="cuda", dtype=torch.float32)
+        m = synth.ellipse_mask((S, S), n=20, cover=0.3, seed=seed + 100 + i, device="cuda")
+        p = setPIVParameters(corr_alg="nsqecc", interSize=isz, searchMargin=isz // 2, step=isz // 2)
+        reqs.append(DeviceRequest(a, b, p, mask=m, mode=mode))
+        sides.append((S, isz))
+        del a, b, m
+    n_ia = sum(int(np.prod(r.vf_size)) for r in reqs)
+    ctx = _lib.context(0)
+    st = torch.cuda.Stream()
+    g = BatchGraph(reqs, ctx)
+    for _ in range(2):
+        g.launch(st.cuda_stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.launch(st.cuda_stream)
+    torch.cuda.synchronize()
+    t_graph = (time.perf_counter() - t0) / steps
+    # plain per-pair path (one batched call per pair = the plain single-pair launch)
+    for r in reqs:
+        piv_batch([r], ctx, st.cuda_stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for r in reqs:
+            piv_batch([r], ctx, st.cuda_stream)
+    torch.cuda.synchronize()
+    t_plain = (time.perf_counter() - t0) / steps
+    g.close()
+    return {"workload": "16 masked NSQECC FP32 pairs, sides 512-4096 (multiples of 256), interSize {16,32,64}, "
+                        "searchMargin = step = interSize/2, 20-ellipse masks (C4)",
+            "pairs": [{"side": S, "interSize": i} for S, i in sides], "ia_total": n_ia,
+            "graph": {"ms_per_set": round(t_graph * 1e3, 3), "ia_per_s": round(n_ia / t_graph, 1)},
+            "plain": {"ms_per_set": round(t_plain * 1e3, 3), "ia_per_s": round(n_ia / t_plain, 1)},
+            "speedup": round(t_plain / t_graph, 3)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -316,6 +376,7 @@ def main() -> int:
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg of the default run")
     ap.add_argument("--no-3d", action="store_true", help="skip the 3-D leg of the default run")
     ap.add_argument("--only-3d", action="store_true", help="run only the 3-D leg (no CPU sample)")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched masked leg of the default run")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-batch", type=int, default=32)
     args = ap.parse_args()
@@ -443,6 +504,10 @@ def main() -> int:
     if not args.no_3d:
         volume = volume_bench(args.mode, 3, not args.no_cpu, nthreads)
 
+    batched = None
+    if not args.no_batched:
+        batched = batched_bench(args.mode, 3)
+
     serving = None
     if not args.no_serve:
         serving = serve_bench(args.serve_requests, [args.serve_rate, 0.0], 11, args.serve_slots,
@@ -499,6 +564,7 @@ def main() -> int:
         "breakdown": breakdown,
         "outputs_finite": finite,
         "volume_3d": volume,
+        "batched_masked": batched,
         "serving": serving,
         "cpu_baseline": cpu,
     }
