@@ -28,15 +28,15 @@ template <> struct Lpl2<float, 64> { static constexpr int v = MQPIV_LPL_F32_64; 
 #endif
 template <> struct Lpl2<float, 96> { static constexpr int v = MQPIV_LPL_F32_96; };
 #ifndef MQPIV_LPL_F32_128
-#define MQPIV_LPL_F32_128 8
+#define MQPIV_LPL_F32_128 4
 #endif
 template <> struct Lpl2<float, 128> { static constexpr int v = MQPIV_LPL_F32_128; };
 #ifndef MQPIV_LPL_F64_16
-#define MQPIV_LPL_F64_16 2
+#define MQPIV_LPL_F64_16 1
 #endif
 template <> struct Lpl2<double, 16> { static constexpr int v = MQPIV_LPL_F64_16; };
 #ifndef MQPIV_LPL_F64_32
-#define MQPIV_LPL_F64_32 4
+#define MQPIV_LPL_F64_32 2
 #endif
 template <> struct Lpl2<double, 32> { static constexpr int v = MQPIV_LPL_F64_32; };
 #ifndef MQPIV_LPL_F64_48
