@@ -27,12 +27,13 @@ from .params import (  # noqa: F401
     setPIVParameters,
     toDims3,
 )
-from .piv import PIV, BatchGraph, DeviceRequest, Server, piv_batch, piv_device  # noqa: F401
+from .piv import PIV, BatchGraph, DeviceFrame, DeviceRequest, Server, piv_batch, piv_device  # noqa: F401
 
 __all__ = [
     "PIV",
     "piv_device",
     "piv_batch",
+    "DeviceFrame",
     "DeviceRequest",
     "BatchGraph",
     "Server",

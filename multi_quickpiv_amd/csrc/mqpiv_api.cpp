@@ -26,7 +26,7 @@ namespace mqpiv {
 int fft_len_2d(int fp64, int ss);
 size_t lds_2d(int fp64, int L, int alg, int ns0, int ns1, int ss0);
 hipError_t launch_piv2d(int fp64, int L, int alg, const Job& j0, const Job* d_jobs, const int64_t* d_prefix,
-                        int n_jobs, int64_t n_ia_total, int ns0, int ns1, int ss0, hipStream_t s);
+                        int n_jobs, int64_t n_ia_total, int ns0, int ns1, int ss0, bool f32_in, hipStream_t s);
 size_t sat_bytes(const int64_t* dims, int nd);
 size_t sat_tmp_bytes(const int64_t* dims, int nd);
 hipError_t build_sat(const void* img, int dt, int power, const int64_t* dims, int nd, double* S, double* tmp,
@@ -279,7 +279,7 @@ static int launch_job(mqpiv_ctx* c, Job& j, hipStream_t s, char* ws = nullptr) {
     hipError_t e;
     {
       KernelTimer kt(c, s);
-      e = launch_piv2d(fp64, L, j.alg, j, nullptr, nullptr, 1, j.n_ia, ns0, ns1, ss0, s);
+      e = launch_piv2d(fp64, L, j.alg, j, nullptr, nullptr, 1, j.n_ia, ns0, ns1, ss0, j.in_dtype == DT_F32, s);
     }
     if (e != hipSuccess) return fail(c, MQPIV_ERR_DEVICE, "2-D launch failed: %s", hipGetErrorString(e));
     return MQPIV_OK;
@@ -437,7 +437,9 @@ struct BatchGroup {
 struct BatchPlan {
   std::vector<Job> jobs2d;
   std::vector<size_t> sat_off;  // per 2-D job, offset of its integral arrays (or SIZE_MAX)
-  std::vector<int> nsat;
+  std::vector<int> nsat;        // integral arrays a 2-D job builds (0 when it reads another job's)
+  std::vector<int> need;        // integral arrays a 2-D job reads
+  std::vector<int> sat_src;     // the job whose arrays it reads (itself when it builds them)
   std::vector<int> req2d, req3d;  // request index of each 2-D / 3-D job
   std::vector<Job> jobs3d;
   std::vector<BatchGroup> groups;
@@ -451,6 +453,10 @@ size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
 static int plan_batch(mqpiv_ctx* c, const mqpiv_request* r, int n, BatchPlan& P) {
   if (n < 0 || (n > 0 && !r)) return fail(c, MQPIV_ERR_NULL, "requests is NULL");
   std::map<std::tuple<int, int, int>, int> key2g;
+  // jobs that build integral arrays, by second image (pointer, type, size):
+  // a later job of the same image reads the arrays of one that builds at
+  // least as many (power 2, then power 1) instead of building its own
+  std::map<std::tuple<const void*, int, int64_t, int64_t>, std::vector<int>> img2src;
   for (int i = 0; i < n; ++i) {
     Job j;
     int rc = build_job(c, &r[i].params, r[i].dims, r[i].in_dtype, r[i].img1, r[i].img2, r[i].mask, r[i].skip_mask,
@@ -470,8 +476,17 @@ static int plan_batch(mqpiv_ctx* c, const mqpiv_request* r, int n, BatchPlan& P)
     const int idx = (int)P.jobs2d.size();
     P.jobs2d.push_back(j);
     P.req2d.push_back(i);
-    P.nsat.push_back(ns);
+    int src = idx;
     if (ns) {
+      auto& srcs = img2src[std::make_tuple(j.img2, j.in_dtype, j.dims[0], j.dims[1])];
+      for (int o : srcs)
+        if (P.nsat[o] >= ns) src = o;
+      if (src == idx) srcs.push_back(idx);
+    }
+    P.need.push_back(ns);
+    P.sat_src.push_back(src);
+    P.nsat.push_back(src == idx ? ns : 0);
+    if (ns && src == idx) {
       P.sat_off.push_back(P.sat_total);
       P.sat_total += ns * al256(sat_bytes(j.dims, 2));
       P.sat_tmp = std::max(P.sat_tmp, al256(sat_tmp_bytes(j.dims, 2)));
@@ -509,16 +524,17 @@ static int plan_batch(mqpiv_ctx* c, const mqpiv_request* r, int n, BatchPlan& P)
 }
 
 // Fills the job tables (host image of the device tables at d_table) with the
-// integral-array pointers: inside sat_base, or in each request's own workspace
-// ws[request] when ws is given.
+// integral-array pointers: inside sat_base, or in the building request's own
+// workspace ws[request] when ws is given.
 static void fill_tables(BatchPlan& P, char* sat_base, char* host_table, void* const* ws = nullptr) {
   for (size_t i = 0; i < P.jobs2d.size(); ++i) {
     Job& j = P.jobs2d[i];
-    if (P.nsat[i]) {
+    if (P.need[i]) {
+      const int o = P.sat_src[i];
       const size_t one = al256(sat_bytes(j.dims, 2));
-      char* b = ws ? (char*)ws[P.req2d[i]] : sat_base + P.sat_off[i];
+      char* b = ws ? (char*)ws[P.req2d[o]] : sat_base + P.sat_off[o];
       j.sat2 = (const double*)b;
-      j.sat1 = P.nsat[i] >= 2 ? (const double*)(b + one) : nullptr;
+      j.sat1 = P.need[i] >= 2 ? (const double*)(b + one) : nullptr;
     }
   }
   for (auto& g : P.groups) {
@@ -554,8 +570,10 @@ static int enqueue_2d(mqpiv_ctx* c, BatchPlan& P, char* sat_base, char* d_table,
     hipError_t e;
     {
       KernelTimer kt(c, s);
+      bool f32_in = true;
+      for (int m : g.members) f32_in = f32_in && P.jobs2d[m].in_dtype == DT_F32;
       e = launch_piv2d(g.fp64, g.L, g.alg, P.jobs2d[g.members[0]], dj, dp, (int)g.members.size(), g.n_ia,
-                       g.ns0cap, g.ns1cap, g.ss0cap, s);
+                       g.ns0cap, g.ns1cap, g.ss0cap, f32_in, s);
     }
     if (e != hipSuccess) return fail(c, MQPIV_ERR_DEVICE, "batch launch failed: %s", hipGetErrorString(e));
   }

@@ -32,6 +32,7 @@
 
 #include "mqpiv_fft.hpp"
 #include "mqpiv_internal.h"
+#include "mqpiv_lpl2d.h"
 
 namespace mqpiv {
 
@@ -187,6 +188,52 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t nb) {
   return x * q + (x < r ? x : r) + k;
 }
 
+// Which job and job-local IA index a launch-global IA index belongs to.
+__device__ __forceinline__ void locate_ia(int64_t ia_c, const Job& j0, const Job* jobs, const int64_t* prefix,
+                                          int n_jobs, Job& J, int64_t& ia) {
+  J = j0;
+  ia = ia_c;
+  if (jobs) {  // batched launch: locate the owning job
+    const int jb = find_job(prefix, n_jobs, ia_c);
+    J = jobs[jb];
+    ia = ia_c - prefix[jb];
+  }
+}
+
+// Where one lane's part of an IA's windows sits in the images.  Lane (line =
+// k0, h) covers k1 = LPL*k + h.  Loads use 32-bit offsets from per-lane base
+// pointers (the host checks dims[0] * L < 2^31).  Every lane issues all its
+// loads before using any (one wait for the batch): positions outside the IA /
+// clipped search region read an in-image pixel instead (the IA origin, or the
+// search region's first row / column) and their values are discarded.
+// g_lane addresses column k1 of the padded buffer, whose image columns start
+// at k1 = goff1.
+struct LaneWin {
+  int64_t f_lane, g_lane;
+  uint32_t ud0;
+  int32_t is1, goff1, gend1;
+  bool f_row, g_row;
+  __device__ __forceinline__ bool f_at(int k1) const { return f_row && k1 < is1; }
+  __device__ __forceinline__ bool g_at(int k1) const { return g_row && k1 >= goff1 && k1 < gend1; }
+  __device__ __forceinline__ uint32_t f_off(int k1) const { return f_at(k1) ? ud0 * (uint32_t)k1 : 0u; }
+  __device__ __forceinline__ uint32_t g_off(int k1) const { return ud0 * (uint32_t)(g_at(k1) ? k1 : goff1); }
+};
+
+__device__ __forceinline__ LaneWin lane_window(const Geo& g0, const Geo& g1, int is0, int is1, int64_t d0, int k0) {
+  LaneWin w;
+  w.f_row = k0 < is0;
+  w.g_row = k0 >= g0.goff && k0 < g0.goff + g0.gsz;
+  const int64_t f_base = (g0.tlf + k0) + d0 * g1.tlf;
+  const int64_t g_base = (g0.sr + (k0 - g0.goff)) + d0 * (g1.sr - g1.goff);
+  w.f_lane = w.f_row ? f_base : (g0.tlf + d0 * g1.tlf);
+  w.g_lane = w.g_row ? g_base : (g0.sr + d0 * (g1.sr - g1.goff));
+  w.ud0 = (uint32_t)d0;
+  w.is1 = is1;
+  w.goff1 = g1.goff;
+  w.gend1 = g1.goff + g1.gsz;
+  return w;
+}
+
 // LDS layout of one IA: the L x L complex transform buffer (which after the
 // transform holds the similarity window), then the first correlation of the
 // masked measure, then the reduction slots.
@@ -339,22 +386,63 @@ constexpr int min_waves_2d() {
   return waves < 1 ? 1 : (waves > MQPIV_OCC_CAP ? MQPIV_OCC_CAP : waves);
 }
 
-template <int L, int ALG, typename T, int LPL>
+// One IA per block (PF = false), or a persistent grid (PF = true: FP32 inputs,
+// one IA per block) in which block i runs the IAs of virtual blocks i, i + G,
+// i + 2G, ... (G = gridDim.x, a multiple of 8 so that each block keeps its
+// XCD) and loads the next IA's window into registers before computing the
+// current one, so that the load latency overlaps the transforms.
+template <int L, int ALG, typename T, int LPL, bool PF>
 __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL>())) void piv2d_kernel(Job j0, const Job* __restrict__ jobs,
                                                                       const int64_t* __restrict__ prefix,
                                                                       int n_jobs, int64_t n_ia_total, int ns0_cap,
                                                                       int ns1_cap) {
   using C = Cfg2<L, T, LPL>;
   constexpr int H = C::H;
+  static_assert(!PF || (C::IPB == 1 && ALG != ALG_MASK_NSQECC), "persistent form: one IA per block, unmasked");
   extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int HP = PF ? H : 1;
+  float fpre[HP], gpre[HP];  // PF: the next IA's window (this lane's part)
+  int64_t vb = blockIdx.x;   // PF: the host launches at most n_ia_total blocks
+  auto fetch = [&](int64_t v) {
+    Job Jn;
+    int64_t ian;
+    locate_ia(xcd_remap(v, n_ia_total), j0, jobs, prefix, n_jobs, Jn, ian);
+    const Geo e0 = geo_of(ian % Jn.vf_n[0], Jn.is[0], Jn.sm[0], Jn.st[0], Jn.dims[0]);
+    const Geo e1 = geo_of(ian / Jn.vf_n[0], Jn.is[1], Jn.sm[1], Jn.st[1], Jn.dims[1]);
+    const int tl = (int)threadIdx.x;
+    const LaneWin w = lane_window(e0, e1, Jn.is[0], Jn.is[1], Jn.dims[0], tl / LPL);
+    const gptr<float> pf = as_global<float>(Jn.img1) + w.f_lane;
+    const gptr<float> pg = as_global<float>(Jn.img2) + w.g_lane;
+#pragma unroll
+    for (int k = 0; k < HP; ++k) {
+      fpre[k] = pf[w.f_off(LPL * k + tl % LPL)];
+      gpre[k] = pg[w.g_off(LPL * k + tl % LPL)];
+    }
+  };
+  if constexpr (PF) fetch(vb);
+
+  for (;;) {  // once unless PF
+  float fcur[HP], gcur[HP];
+  if constexpr (PF) {
+#pragma unroll
+    for (int k = 0; k < HP; ++k) {
+      fcur[k] = fpre[k];
+      gcur[k] = gpre[k];
+    }
+    if (vb + (int64_t)gridDim.x < n_ia_total) fetch(vb + (int64_t)gridDim.x);
+  }
+  // PF: the lane index is made opaque each iteration so that the values
+  // derived from it are recomputed per IA rather than held across the loop
+  int tid = (int)threadIdx.x;
+  if constexpr (PF) asm volatile("" : "+v"(tid));
   // one IA per block: every IA quantity below is uniform over the block, which
   // lets the compiler keep it in scalar registers
-  const int t = C::IPB == 1 ? (int)threadIdx.x : (int)(threadIdx.x % C::TPI);
-  const int sub = C::IPB == 1 ? 0 : (int)(threadIdx.x / C::TPI);
+  const int t = C::IPB == 1 ? tid : tid % C::TPI;
+  const int sub = C::IPB == 1 ? 0 : tid / C::TPI;
   const int line = t / LPL;
   const int h = t % LPL;  // lane within the line's group
 
-  const int64_t b = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t b = PF ? xcd_remap(vb, n_ia_total) : xcd_remap(blockIdx.x, gridDim.x);
   const int64_t ia_g = b * C::IPB + sub;
   const bool valid = ia_g < n_ia_total;
   const int64_t ia_c = valid ? ia_g : n_ia_total - 1;
@@ -423,13 +511,21 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
   const bool want_max = J.has_threshold || J.has_mask_threshold;
   auto load = [&](auto tin) {
     using TIN = decltype(tin);
-    const gptr<TIN> pf = as_global<TIN>(J.img1) + f_lane;
-    const gptr<TIN> pg = as_global<TIN>(J.img2) + g_lane;
     TIN fv[H], gv[H], mv[H];
+    if constexpr (PF) {
 #pragma unroll
-    for (int k = 0; k < H; ++k) {
-      fv[k] = pf[f_off(k1_of(k))];
-      gv[k] = pg[g_off(k1_of(k))];
+      for (int k = 0; k < H; ++k) {
+        fv[k] = fcur[k];
+        gv[k] = gcur[k];
+      }
+    } else {
+      const gptr<TIN> pf = as_global<TIN>(J.img1) + f_lane;
+      const gptr<TIN> pg = as_global<TIN>(J.img2) + g_lane;
+#pragma unroll
+      for (int k = 0; k < H; ++k) {
+        fv[k] = pf[f_off(k1_of(k))];
+        gv[k] = pg[g_off(k1_of(k))];
+      }
     }
     if constexpr (ALG == ALG_MASK_NSQECC) {
       const gptr<TIN> pm = as_global<TIN>(J.mask) + f_lane;
@@ -482,8 +578,12 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
       x[k] = cpx<T>{(T)a, (T)g};
     }
   };
-  if (dt == DT_F32) load(float{});
-  else load(double{});
+  if constexpr (PF) {
+    load(float{});  // the host launches PF for FP32 inputs only
+  } else {
+    if (dt == DT_F32) load(float{});
+    else load(double{});
+  }
   if constexpr (ALG != ALG_FFTCC) sumF2 = Group<C::TPI>::reduce(sumF2, OpAdd{}, red, t);
 
   // ---- skip test (piv_ops.jl:134-157)
@@ -734,6 +834,14 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
       if (J.sn) ((double*)J.sn)[ia] = sn;
     }
   }
+  if constexpr (!PF) {
+    break;
+  } else {
+    vb += gridDim.x;
+    if (vb >= n_ia_total) break;
+    __syncthreads();  // the next IA reuses the LDS
+  }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -747,37 +855,59 @@ size_t lds_bytes_2d(int alg, int ns0, int ns1, int ss0) {
   return (z + c1) * C::IPB + (size_t)C::IPB * kRedBytes;
 }
 
+template <int L, int ALG, typename T, int LPL, bool PF>
+hipError_t launch_2d_k(const Job& j0, const Job* d_jobs, const int64_t* d_prefix, int n_jobs, int64_t n_ia_total,
+                       int ns0, int ns1, size_t lds, hipStream_t s) {
+  using C = Cfg2<L, T, LPL>;
+  const void* fn = (const void*)piv2d_kernel<L, ALG, T, LPL, PF>;
+  static const bool attr_set =
+      (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024), true);
+  (void)attr_set;
+  int64_t nb = (n_ia_total + C::IPB - 1) / C::IPB;
+  if constexpr (PF) {
+    // as many blocks as fit on the device at once, a multiple of the 8 XCDs
+    thread_local int resident = 0;
+    thread_local size_t resident_lds = 0;
+    if (resident == 0 || resident_lds != lds) {
+      int dev = 0, per_cu = 0, n_cu = 0;
+      if (hipGetDevice(&dev) != hipSuccess ||
+          hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+          hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, C::NT, lds) != hipSuccess || per_cu < 1)
+        return hipErrorInvalidValue;
+      resident = per_cu * n_cu;
+      resident_lds = lds;
+    }
+    if (nb > resident) nb = resident >= 8 ? resident - resident % 8 : resident;
+  }
+  hipLaunchKernelGGL((piv2d_kernel<L, ALG, T, LPL, PF>), dim3((unsigned)nb), dim3(C::NT), lds, s, j0, d_jobs,
+                     d_prefix, n_jobs, n_ia_total, ns0, ns1);
+  return hipGetLastError();
+}
+
 template <int L, int ALG, typename T, int LPL>
 hipError_t launch_2d_t(const Job& j0, const Job* d_jobs, const int64_t* d_prefix, int n_jobs,
-                       int64_t n_ia_total, int ns0, int ns1, int ss0, hipStream_t s) {
-  using C = Cfg2<L, T, LPL>;
+                       int64_t n_ia_total, int ns0, int ns1, int ss0, bool f32_in, hipStream_t s) {
   size_t lds = lds_bytes_2d<L, T, LPL>(ALG, ns0, ns1, ss0);
   if (lds > 160 * 1024) return hipErrorInvalidValue;
-  static bool attr_set = false;
-  if (!attr_set) {
-    (void)hipFuncSetAttribute((const void*)piv2d_kernel<L, ALG, T, LPL>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    attr_set = true;
+  if constexpr (Pf2<T, L>::v && Cfg2<L, T, LPL>::IPB == 1 && ALG != ALG_MASK_NSQECC) {
+    if (f32_in) return launch_2d_k<L, ALG, T, LPL, true>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, lds, s);
   }
-  int64_t nb = (n_ia_total + C::IPB - 1) / C::IPB;
-  hipLaunchKernelGGL((piv2d_kernel<L, ALG, T, LPL>), dim3((unsigned)nb), dim3(C::NT), lds, s, j0,
-                     d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1);
-  return hipGetLastError();
+  return launch_2d_k<L, ALG, T, LPL, false>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, lds, s);
 }
 
 template <int L, typename T, int LPL>
 hipError_t launch_2d_l(int alg, const Job& j0, const Job* d_jobs, const int64_t* d_prefix, int n_jobs,
-                       int64_t n_ia_total, int ns0, int ns1, int ss0, hipStream_t s) {
+                       int64_t n_ia_total, int ns0, int ns1, int ss0, bool f32_in, hipStream_t s) {
   switch (alg) {
     case ALG_FFTCC:
-      return launch_2d_t<L, ALG_FFTCC, T, LPL>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, s);
+      return launch_2d_t<L, ALG_FFTCC, T, LPL>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, f32_in, s);
     case ALG_ZNCC:
-      return launch_2d_t<L, ALG_ZNCC, T, LPL>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, s);
+      return launch_2d_t<L, ALG_ZNCC, T, LPL>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, f32_in, s);
     case ALG_NSQECC:
-      return launch_2d_t<L, ALG_NSQECC, T, LPL>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, s);
+      return launch_2d_t<L, ALG_NSQECC, T, LPL>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, f32_in, s);
     case ALG_MASK_NSQECC:
       return launch_2d_t<L, ALG_MASK_NSQECC, T, LPL>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1,
-                                                    ss0, s);
+                                                    ss0, f32_in, s);
   }
   return hipErrorInvalidValue;
 }

@@ -173,37 +173,62 @@ def piv_device(ctx: "_lib.Context", pivparams: PIVParameters, shape: Tuple[int, 
     ctx.check(rc)
 
 
+class DeviceFrame:
+    """A frame pair resident on the device, stored column-major (dim 1
+    fastest) once.  Requests made from the same frame with ``DeviceRequest.
+    from_frame`` share its buffers, so a batch builds the second image's
+    integral arrays once for all of them."""
+
+    def __init__(self, img1, img2):
+        import torch
+
+        if img1.shape != img2.shape:
+            raise AssertionError("PIV inputs need to have the same size.")
+        self.shape = tuple(img1.shape)
+        dt = torch.float32 if (img1.dtype == torch.float32 and img2.dtype == torch.float32) else torch.float64
+        perm = tuple(reversed(range(img1.dim())))
+        self.dtype = dt
+        self.a = img1.to(dt).permute(*perm).contiguous()
+        self.b = img2.to(dt).permute(*perm).contiguous()
+        self.in_code = _lib.MQPIV_F32 if dt == torch.float32 else _lib.MQPIV_F64
+
+
 class DeviceRequest:
     """One frame pair resident on the device for batched / graph execution.
 
     ``img1``, ``img2`` (and ``mask``) are torch tensors of the frame's shape in
     the reference's axis order; they are stored column-major (dim 1 fastest)
-    once, here.  Outputs ``VF`` (N x nVF..., FP64) and ``SN`` are device tensors
-    in the reference layout, filled by ``piv_batch`` / graph launches.
+    once, here (or taken from a ``DeviceFrame``, see ``from_frame``).  Outputs
+    ``VF`` (N x nVF..., FP64) and ``SN`` are device tensors in the reference
+    layout, filled by ``piv_batch`` / graph launches.
     """
 
     def __init__(self, img1, img2, pivparams: PIVParameters, mask=None, mode: str = "auto",
-                 want_peaks: bool = False):
+                 want_peaks: bool = False, frame: Optional[DeviceFrame] = None):
         import torch
 
-        if img1.shape != img2.shape or (mask is not None and mask.shape != img1.shape):
+        if frame is None:
+            frame = DeviceFrame(img1, img2)
+        if mask is not None and tuple(mask.shape) != frame.shape:
             raise AssertionError("PIV inputs need to have the same size.")
-        N = img1.dim()
+        N = len(frame.shape)
         pivparams.ndims = N
-        self.shape = tuple(img1.shape)
-        dt = torch.float32 if (img1.dtype == torch.float32 and img2.dtype == torch.float32) else torch.float64
+        self.shape = frame.shape
         perm = tuple(reversed(range(N)))
-        self.a = img1.to(dt).permute(*perm).contiguous()
-        self.b = img2.to(dt).permute(*perm).contiguous()
-        self.m = mask.to(dt).permute(*perm).contiguous() if mask is not None else None
-        self.in_code = _lib.MQPIV_F32 if dt == torch.float32 else _lib.MQPIV_F64
+        self.a, self.b, self.in_code = frame.a, frame.b, frame.in_code
+        self.m = mask.to(frame.dtype).permute(*perm).contiguous() if mask is not None else None
         self.vf_size = get_vectorfield_size(self.shape, pivparams)
         n = int(np.prod(self.vf_size))
-        dev = img1.device
+        dev = frame.a.device
         self.VF = torch.zeros(n * N, dtype=torch.float64, device=dev)
         self.SN = torch.zeros(n, dtype=torch.float64, device=dev) if pivparams.computeSN else None
         self.PK = torch.zeros(n * N, dtype=torch.int32, device=dev) if want_peaks else None
         self.params = to_c_params(pivparams, mask_NSQECC.code if mask is not None else None, mode)
+
+    @classmethod
+    def from_frame(cls, frame: DeviceFrame, pivparams: PIVParameters, mask=None, mode: str = "auto",
+                   want_peaks: bool = False) -> "DeviceRequest":
+        return cls(None, None, pivparams, mask=mask, mode=mode, want_peaks=want_peaks, frame=frame)
 
     def c_request(self) -> "_lib.MqpivRequest":
         r = _lib.MqpivRequest()

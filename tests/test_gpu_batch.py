@@ -84,3 +84,58 @@ def test_graph_replays_match_eager():
     VFg, SNg = dreqs[0].results()
     assert np.array_equal(VFg, VF1, equal_nan=True)
     g.close()
+
+
+def test_requests_on_one_frame_share_integral_arrays():
+    """Requests made from one DeviceFrame read one set of integral arrays (built
+    by the first request that needs the most tables); each result is still
+    bit-identical to its own call, in a batch and in graph replays, including a
+    frame of another size and a double-precision frame in the same batch."""
+    from multi_quickpiv_amd.piv import DeviceFrame
+
+    a, b = _pair((288, 256), (2.2, -1.4), 321)
+    frame = DeviceFrame(a, b)
+    other = DeviceFrame(*_pair((200, 232), (-0.7, 2.5), 322))
+    a64, b64 = _pair((160, 192), (1.1, 0.4), 323, dtype=torch.float64)
+    frame64 = DeviceFrame(a64, b64)
+    specs = [  # NSQECC first: the ZNCC request then builds both tables itself
+        (frame, (a, b), "nsqecc", 32, 16, 16),
+        (frame, (a, b), "zncc", 32, 16, 16),
+        (frame, (a, b), "zncc", 64, 16, 16),
+        (other, None, "zncc", 24, 12, 12),
+        (frame, (a, b), "nsqecc", 16, 8, 8),
+        (frame64, (a64, b64), "nsqecc", 32, 16, 16),
+        (frame64, (a64, b64), "zncc", 16, 8, 8),
+        (frame, (a, b), "fft", 32, 16, 16),
+    ]
+    dreqs = []
+    for fr, _, alg, isz, sm, st in specs:
+        p = setPIVParameters(corr_alg=alg, interSize=isz, searchMargin=sm, step=st, computeSN=True)
+        dreqs.append(DeviceRequest.from_frame(fr, p, want_peaks=True))
+    assert dreqs[0].b.data_ptr() == dreqs[1].b.data_ptr()
+    piv_batch(dreqs)
+    torch.cuda.synchronize()
+    singles = []
+    for (fr, ab, alg, isz, sm, st), r in zip(specs, dreqs):
+        if ab is None:
+            ab = (fr.a.t().contiguous(), fr.b.t().contiguous())
+        p = setPIVParameters(corr_alg=alg, interSize=isz, searchMargin=sm, step=st, computeSN=True)
+        one = PIV(ab[0].cpu().numpy(), ab[1].cpu().numpy(), p, return_peaks=True)
+        got = r.results()
+        print(alg, isz, r.shape, flush=True)
+        for x, y in zip(got, one):
+            assert np.array_equal(x, y, equal_nan=True)
+        singles.append(one)
+    g = BatchGraph(dreqs)
+    for rep in range(2):
+        for r in dreqs:
+            r.VF.fill_(3.0)
+            r.SN.fill_(3.0)
+        torch.cuda.synchronize()
+        g.launch()
+        torch.cuda.synchronize()
+        for r, one in zip(dreqs, singles):
+            VFg, SNg, _ = r.results()
+            assert np.array_equal(VFg, one[0], equal_nan=True), rep
+            assert np.array_equal(SNg, one[1], equal_nan=True), rep
+    g.close()
