@@ -63,65 +63,154 @@ __device__ __forceinline__ double load_px(const void* img, int dt, int64_t i) {
   return dt == DT_F32 ? (double)((const float*)img)[i] : ((const double*)img)[i];
 }
 
-// Reductions over the lanes of one IA.  For groups of <= 64 lanes a butterfly
-// over the whole wave (idle lanes carry the identity); for multi-wave groups
-// the wave results are combined through LDS.
+// Cross-lane moves for reductions.  A mover returns, for every lane, the value
+// of the lane it is paired with in one reduction step; where a step has no
+// source lane the lane receives `id` (the reduction's identity).  32-bit DPP
+// moves (VALU) do the work; doubles move as two halves.
+template <int CTRL, int ROWM, int BANKM>
+__device__ __forceinline__ int dpp_i32(int id, int v) {
+  return __builtin_amdgcn_update_dpp(id, v, CTRL, ROWM, BANKM, false);
+}
+template <int CTRL, int ROWM, int BANKM>
+struct DppMove {
+  __device__ __forceinline__ int operator()(int id, int v) const { return dpp_i32<CTRL, ROWM, BANKM>(id, v); }
+  __device__ __forceinline__ float operator()(float id, float v) const {
+    return __builtin_bit_cast(float, dpp_i32<CTRL, ROWM, BANKM>(__builtin_bit_cast(int, id), __builtin_bit_cast(int, v)));
+  }
+  __device__ __forceinline__ double operator()(double id, double v) const {
+    const long long u = __builtin_bit_cast(long long, v), e = __builtin_bit_cast(long long, id);
+    const int lo = dpp_i32<CTRL, ROWM, BANKM>((int)(e & 0xffffffffll), (int)(u & 0xffffffffll));
+    const int hi = dpp_i32<CTRL, ROWM, BANKM>((int)(e >> 32), (int)(u >> 32));
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+  }
+};
+// lane ^ 16 within each half-wave (ds_swizzle, bit mode)
+struct Swz16Move {
+  __device__ __forceinline__ int operator()(int, int v) const { return __builtin_amdgcn_ds_swizzle(v, (16 << 10) | 0x1F); }
+  __device__ __forceinline__ float operator()(float id, float v) const {
+    return __builtin_bit_cast(float, (*this)(0, __builtin_bit_cast(int, v)));
+  }
+  __device__ __forceinline__ double operator()(double, double v) const {
+    const long long u = __builtin_bit_cast(long long, v);
+    const int lo = (*this)(0, (int)(u & 0xffffffffll));
+    const int hi = (*this)(0, (int)(u >> 32));
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+  }
+};
+__device__ __forceinline__ int bcast63(int v) { return __builtin_amdgcn_readlane(v, 63); }
+__device__ __forceinline__ float bcast63(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 63));
+}
+__device__ __forceinline__ double bcast63(double v) {
+  const long long u = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffll), 63);
+  const int hi = __builtin_amdgcn_readlane((int)(u >> 32), 63);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Reductions over the lanes of one IA.  Groups of <= 32 lanes: butterflies
+// (quad permutations, half-row and row mirrors, a half-wave swizzle) that leave
+// the total in every lane.  Groups of >= 64 lanes: inclusive prefix steps
+// within each 16-lane row (row_shr 1, 2, 4, 8), then the row broadcasts of
+// lanes 15 and 31, which leave the wave's total in lane 63; one wave is then
+// broadcast from lane 63, several are combined through LDS in wave order.
+// Every lane ends with the same value, from the same order of operations.
 template <int TPI>
 struct Group {
-  static constexpr int kSpan = TPI <= 32 ? TPI : 64;
+  static_assert(TPI == 1 || TPI == 2 || TPI == 4 || TPI == 8 || TPI == 16 || TPI == 32 || TPI % 64 == 0,
+                "a power of two up to 32 lanes, or whole waves");
+  static constexpr int NW = (TPI + 63) / 64;
+
+  // One reduction over the group: f(mv) is called once per step, with mv(id, v)
+  // the step's partner value.
+  template <typename F>
+  __device__ __forceinline__ static void steps(F&& f) {
+    if constexpr (TPI <= 32) {
+      if constexpr (TPI > 1) f(DppMove<0xB1, 0xF, 0xF>{});   // quad_perm [1,0,3,2]
+      if constexpr (TPI > 2) f(DppMove<0x4E, 0xF, 0xF>{});   // quad_perm [2,3,0,1]
+      if constexpr (TPI > 4) f(DppMove<0x141, 0xF, 0xF>{});  // row_half_mirror
+      if constexpr (TPI > 8) f(DppMove<0x140, 0xF, 0xF>{});  // row_mirror
+      if constexpr (TPI > 16) f(Swz16Move{});
+    } else {
+      f(DppMove<0x111, 0xF, 0xF>{});  // row_shr:1
+      f(DppMove<0x112, 0xF, 0xF>{});  // row_shr:2
+      f(DppMove<0x114, 0xF, 0xF>{});  // row_shr:4
+      f(DppMove<0x118, 0xF, 0xF>{});  // row_shr:8
+      f(DppMove<0x142, 0xA, 0xF>{});  // row_bcast:15 into rows 1 and 3
+      f(DppMove<0x143, 0xC, 0xF>{});  // row_bcast:31 into rows 2 and 3
+    }
+  }
 
   template <typename V, typename Op>
   __device__ __forceinline__ static V reduce(V v, Op op, double* red, int t) {
-#pragma unroll
-    for (int m = 1; m < kSpan; m <<= 1) v = op(v, __shfl_xor(v, m));
-    if constexpr (TPI > 64) {
+    steps([&](auto mv) { v = op(v, mv((V)Op::id(), v)); });
+    if constexpr (TPI == 64) {
+      v = bcast63(v);
+    } else if constexpr (TPI > 64) {
       V* rv = reinterpret_cast<V*>(red);
       __syncthreads();
-      if ((t & 63) == 0) rv[t >> 6] = v;
+      if ((t & 63) == 63) rv[t >> 6] = v;
       __syncthreads();
       V r = rv[0];
 #pragma unroll 4
-      for (int w = 1; w < (TPI + 63) / 64; ++w) r = op(r, rv[w]);
+      for (int w = 1; w < NW; ++w) r = op(r, rv[w]);
       v = r;
     }
     return v;
   }
 
-  // Two reductions sharing one exchange through LDS (slots 0-15 and 16-31).
-  template <typename OpA, typename OpB>
-  __device__ __forceinline__ static void reduce2(double& a, double& b, OpA opa, OpB opb, double* red, int t) {
-#pragma unroll
-    for (int m = 1; m < kSpan; m <<= 1) {
-      a = opa(a, __shfl_xor(a, m));
-      b = opb(b, __shfl_xor(b, m));
-    }
-    if constexpr (TPI > 64) {
+  // Up to three reductions sharing one exchange through LDS (slots 0-15,
+  // 16-31, 32-47); N = 2 leaves c untouched.
+  template <int N, typename OpA, typename OpB, typename OpC>
+  __device__ __forceinline__ static void reduce_n(double& a, double& b, double& c, OpA opa, OpB opb, OpC opc,
+                                                  double* red, int t) {
+    steps([&](auto mv) {
+      a = opa(a, mv(OpA::id(), a));
+      b = opb(b, mv(OpB::id(), b));
+      if constexpr (N > 2) c = opc(c, mv(OpC::id(), c));
+    });
+    if constexpr (TPI == 64) {
+      a = bcast63(a);
+      b = bcast63(b);
+      if constexpr (N > 2) c = bcast63(c);
+    } else if constexpr (TPI > 64) {
       __syncthreads();
-      if ((t & 63) == 0) {
+      if ((t & 63) == 63) {
         red[t >> 6] = a;
         red[16 + (t >> 6)] = b;
+        if constexpr (N > 2) red[32 + (t >> 6)] = c;
       }
       __syncthreads();
-      double ra = red[0], rb = red[16];
+      double ra = red[0], rb = red[16], rc = N > 2 ? red[32] : 0.0;
 #pragma unroll 4
-      for (int w = 1; w < (TPI + 63) / 64; ++w) {
+      for (int w = 1; w < NW; ++w) {
         ra = opa(ra, red[w]);
         rb = opb(rb, red[16 + w]);
+        if constexpr (N > 2) rc = opc(rc, red[32 + w]);
       }
       a = ra;
       b = rb;
+      if constexpr (N > 2) c = rc;
     }
+  }
+  template <typename OpA, typename OpB>
+  __device__ __forceinline__ static void reduce2(double& a, double& b, OpA opa, OpB opb, double* red, int t) {
+    double unused = 0.0;
+    reduce_n<2>(a, b, unused, opa, opb, opb, red, t);
   }
 };
 
 struct OpAdd {
+  __device__ static constexpr double id() { return 0.0; }
   __device__ double operator()(double a, double b) const { return a + b; }
 };
 struct OpMax {
+  __device__ static constexpr double id() { return -__builtin_inf(); }
   __device__ double operator()(double a, double b) const { return a > b ? a : b; }
 };
 // max that propagates NaN (Julia's `maximum`)
 struct OpMaxNaN {
+  __device__ static constexpr double id() { return -__builtin_inf(); }
   __device__ double operator()(double a, double b) const {
     return (a != a || b != b) ? (a + b) : (a > b ? a : b);
   }
@@ -533,20 +622,29 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
       for (int k = 0; k < H; ++k) mv[k] = pm[f_off(k1_of(k))];
     }
     if constexpr (ALG == ALG_ZNCC) {
-      double sf = 0.0, sg = 0.0;
+      // sums of F, G and F^2 in one reduction; sum((F - mean)^2) follows from
+      // them (exact algebra; it differs from summing the squares of the
+      // rounded differences by a rounding-level factor common to the whole IA,
+      // which leaves the peak and SN unchanged)
+      double sf = 0.0, sg = 0.0, sff = 0.0;
 #pragma unroll
       for (int k = 0; k < H; ++k) {
         const int k1 = k1_of(k);
-        if (f_at(k1)) sf += (double)fv[k];
+        const double f = f_at(k1) ? (double)fv[k] : 0.0;
+        sf += f;
+        sff += f * f;
         if (g_at(k1)) sg += (double)gv[k];
       }
-      Group<C::TPI>::reduce2(sf, sg, OpAdd{}, OpAdd{}, red, t);
+      Group<C::TPI>::template reduce_n<3>(sf, sg, sff, OpAdd{}, OpAdd{}, OpAdd{}, red, t);
       const int64_t nF = (int64_t)is0 * is1;
       const int64_t nG = (int64_t)g0.gsz * g1.gsz;
       meanF32 = (float)sf / (float)nF;
       meanG32 = (float)sg / (float)nG;
       meanF64 = sf / (double)nF;
       meanG64 = sg / (double)nG;
+      const double mu = std::is_same<TIN, float>::value ? (double)meanF32 : meanF64;
+      const double v = (sff - 2.0 * mu * sf) + (double)nF * mu * mu;
+      sumF2 = v > 0.0 ? v : 0.0;
     }
 #pragma unroll
     for (int k = 0; k < H; ++k) {
@@ -557,7 +655,6 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
         if (want_max) mx1 = fmax(mx1, (double)f);
         if constexpr (ALG == ALG_ZNCC) {
           a = std::is_same<TIN, float>::value ? (double)((float)f - meanF32) : (double)f - meanF64;
-          sumF2 += a * a;
         } else if constexpr (ALG == ALG_MASK_NSQECC) {
           const double m = (double)mv[k];
           if (want_max) mx2 = fmax(mx2, m);
@@ -584,7 +681,7 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
     if (dt == DT_F32) load(float{});
     else load(double{});
   }
-  if constexpr (ALG != ALG_FFTCC) sumF2 = Group<C::TPI>::reduce(sumF2, OpAdd{}, red, t);
+  if constexpr (ALG == ALG_NSQECC || ALG == ALG_MASK_NSQECC) sumF2 = Group<C::TPI>::reduce(sumF2, OpAdd{}, red, t);
 
   // ---- skip test (piv_ops.jl:134-157)
   int skip = 0;
@@ -736,16 +833,22 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
   // combined reduction: (peak value, index), SN maximum, SN sum; its barrier
   // also makes the normalised window visible to the fit below
   {
-#pragma unroll
-    for (int m = 1; m < Group<C::TPI>::kSpan; m <<= 1) {
-      Best<T> o{__shfl_xor(bst.v, m), __shfl_xor(bst.i, m)};
+    // (idle partners carry the identities: value 0 with no index, -inf, 0)
+    Group<C::TPI>::steps([&](auto mv) {
+      const Best<T> o{mv(T(0), bst.v), mv((int32_t)INT32_MAX, bst.i)};
       bst = best_of(bst, o);
-      mx = OpMaxNaN{}(mx, __shfl_xor(mx, m));
-      smv += __shfl_xor(smv, m);
+      mx = OpMaxNaN{}(mx, mv(OpMaxNaN::id(), mx));
+      smv += mv(0.0, smv);
+    });
+    if constexpr (C::TPI == 64) {
+      bst.v = bcast63(bst.v);
+      bst.i = bcast63(bst.i);
+      mx = bcast63(mx);
+      smv = bcast63(smv);
     }
     __syncthreads();
     if constexpr (C::TPI > 64) {
-      if ((t & 63) == 0) {
+      if ((t & 63) == 63) {
         const int w = t >> 6;
         red[w] = (double)bst.v;
         red[16 + w] = (double)bst.i;
@@ -860,8 +963,10 @@ hipError_t launch_2d_k(const Job& j0, const Job* d_jobs, const int64_t* d_prefix
                        int ns0, int ns1, size_t lds, hipStream_t s) {
   using C = Cfg2<L, T, LPL>;
   const void* fn = (const void*)piv2d_kernel<L, ALG, T, LPL, PF>;
-  static const bool attr_set =
-      (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024), true);
+  static const bool attr_set = [fn] {
+    (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    return true;
+  }();
   (void)attr_set;
   int64_t nb = (n_ia_total + C::IPB - 1) / C::IPB;
   if constexpr (PF) {

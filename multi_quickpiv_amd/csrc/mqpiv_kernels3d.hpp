@@ -449,16 +449,12 @@ __global__ __launch_bounds__(256) void finalize3d_kernel(Job J, Scratch3 sc, int
     const T v = S[li];
     if (v > bst.v) bst = Best<T>{v, li};
   }
-#pragma unroll
-  for (int m = 1; m < 64; m <<= 1) {
-    Best<T> o{__shfl_xor(bst.v, m), __shfl_xor(bst.i, m)};
-    bst = best_of(bst, o);
-  }
+  Group<NT>::steps([&](auto mv) { bst = best_of(bst, Best<T>{mv(T(0), bst.v), mv((int32_t)INT32_MAX, bst.i)}); });
   {
     T* rv = reinterpret_cast<T*>(red);
     int32_t* ri = reinterpret_cast<int32_t*>(red + 16);
     __syncthreads();
-    if ((t & 63) == 0) {
+    if ((t & 63) == 63) {
       rv[t >> 6] = bst.v;
       ri[t >> 6] = bst.i;
     }
