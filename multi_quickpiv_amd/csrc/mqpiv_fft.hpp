@@ -214,7 +214,7 @@ __host__ __device__ __forceinline__ void fft_line(cpx<T> (&x)[L]) {
 //
 // The partner exchange is a callable so the same code runs on the CPU in unit
 // tests.  On the GPU, q ^ 1 and q ^ 2 are DPP quad permutations (one VALU move
-// per dword) and q ^ 4 is a ds_swizzle xor.
+// per dword) and q ^ 4, q ^ 8 are ds_swizzle xors.
 
 template <int P>
 __host__ __device__ constexpr int chunk_of(int q) {
@@ -283,8 +283,9 @@ __host__ __device__ __forceinline__ cpx<T> twiddle_mul_neg(cpx<T> u) {
 // The lane-dependent factor of a cross-lane stage.  A stage's twiddle is
 // W_LEN^(k + c*M) with k < M compile-time and c in [0, NC) the lane's chunk;
 // since LEN = 2*NC*M it factors as W_LEN^k * exp(SIGN*i*pi*c/NC): NC = 1 needs
-// nothing, NC = 2 a rotation by SIGN*i (selects), NC = 4 one runtime complex
-// product with a factor picked once per stage.  No runtime-indexed tables.
+// nothing, NC = 2 a rotation by SIGN*i (selects), NC = 4 or 8 one runtime
+// complex product with a factor picked once per stage (NC = 8: the NC = 4
+// factor of c/2 times exp(SIGN*i*pi/8) for odd c).  No runtime-indexed tables.
 template <int NC, int SIGN, typename T>
 __host__ __device__ __forceinline__ cpx<T> lane_factor(int c) {
   if constexpr (NC == 4) {
@@ -293,6 +294,12 @@ __host__ __device__ __forceinline__ cpx<T> lane_factor(int c) {
     const T re = b1 ? (b0 ? -h : T(0)) : (b0 ? h : T(1));
     const T im = b1 ? (b0 ? h : T(1)) : (b0 ? h : T(0));
     return cpx<T>{re, SIGN > 0 ? im : -im};
+  } else if constexpr (NC == 8) {
+    const cpx<T> f = lane_factor<4, SIGN, T>(c >> 1);
+    constexpr T co = T(0.92387953251128675612818318939678829);  // cos(pi/8)
+    constexpr T si = T(SIGN > 0 ? 0.38268343236508977172845998403039887 : -0.38268343236508977172845998403039887);
+    if ((c & 1) == 0) return f;
+    return cpx<T>{f.x * co - f.y * si, f.x * si + f.y * co};
   } else {
     return cpx<T>{T(1), T(0)};
   }
@@ -365,8 +372,9 @@ __host__ __device__ __forceinline__ void group_stage(cpx<T> (&v)[M], int q, cons
 template <int L, int P, int SIGN, typename T, typename X>
 __host__ __device__ __forceinline__ void fft_group_dit(cpx<T> (&v)[L / P], int q, const X& xchg) {
   constexpr int M = L / P;
-  static_assert(P == 2 || P == 4 || P == 8, "groups of 2, 4 or 8 lanes");
+  static_assert(P == 2 || P == 4 || P == 8 || P == 16, "groups of 2, 4, 8 or 16 lanes");
   fft_line<M, SIGN, T>(v);
+  if constexpr (P >= 16) group_stage<L, P, 3, SIGN, true, M>(v, q, xchg);
   if constexpr (P >= 8) group_stage<L, P, 2, SIGN, true, M>(v, q, xchg);
   if constexpr (P >= 4) group_stage<L, P, 1, SIGN, true, M>(v, q, xchg);
   group_stage<L, P, 0, SIGN, true, M>(v, q, xchg);
@@ -376,10 +384,11 @@ __host__ __device__ __forceinline__ void fft_group_dit(cpx<T> (&v)[L / P], int q
 template <int L, int P, int SIGN, typename T, typename X>
 __host__ __device__ __forceinline__ void fft_group_dif(cpx<T> (&v)[L / P], int q, const X& xchg) {
   constexpr int M = L / P;
-  static_assert(P == 2 || P == 4 || P == 8, "groups of 2, 4 or 8 lanes");
+  static_assert(P == 2 || P == 4 || P == 8 || P == 16, "groups of 2, 4, 8 or 16 lanes");
   group_stage<L, P, 0, SIGN, false, M>(v, q, xchg);
   if constexpr (P >= 4) group_stage<L, P, 1, SIGN, false, M>(v, q, xchg);
   if constexpr (P >= 8) group_stage<L, P, 2, SIGN, false, M>(v, q, xchg);
+  if constexpr (P >= 16) group_stage<L, P, 3, SIGN, false, M>(v, q, xchg);
   fft_line<M, SIGN, T>(v);
 }
 

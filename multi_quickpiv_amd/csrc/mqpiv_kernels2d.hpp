@@ -357,6 +357,21 @@ __device__ __forceinline__ void line_inv(cpx<T> (&x)[L / LPL], int q) {
   else fft_group_dif<L, LPL, 1, T>(x, q, DppXchg{});
 }
 
+// Lanes per line for the last inverse pass (rows along dim 2, two per
+// transform): twice the forward passes' when the launch has the lanes for the
+// most row pairs a length-L transform can need (ns0 <= L/2 + 1) and the
+// per-lane length has a plan.  That pass has few lines, so shorter per-lane
+// chains shorten it.
+__host__ __device__ constexpr bool has_plan(int m) {
+  return m == 4 || m == 6 || m == 8 || m == 12 || m == 16 || m == 24 || m == 32 || m == 48 || m == 64 ||
+         m == 96 || m == 128;
+}
+template <int L, int LPL, int TPI>
+__host__ __device__ constexpr int c2r_lanes() {
+  constexpr int P = 2 * LPL;
+  return (P <= 16 && P * ((L / 2 + 2) / 2) <= TPI && has_plan(L / P)) ? P : LPL;
+}
+
 template <int L, int LPL, typename T>
 __device__ __forceinline__ void correlate_2d(cpx<T> (&x)[L / LPL], cpx<T>* __restrict__ Z, int t, int ns0,
                                              int ns1) {
@@ -415,36 +430,41 @@ __device__ __forceinline__ void correlate_2d(cpx<T> (&x)[L / LPL], cpx<T>* __res
   }
   __syncthreads();
   // --- inverse along dim 2, rows s0 = 2r and 2r + 1 packed as Ya + i Yb:
-  // line = r, lane q holds k1 = cM + k of the packed spectrum
+  // line = r, lane q2 holds k1 = cM2 + k of the packed spectrum (P2 lanes a line)
+  constexpr int P2 = c2r_lanes<L, LPL, Cfg2<L, T, LPL>::TPI>();
+  constexpr int M2 = L / P2;
+  const int line2 = t / P2, q2 = t % P2;
+  const int cM2 = (P2 == 1 ? 0 : chunk_of<P2>(q2)) * M2;
   const int npair = (ns0 + 1) / 2;
-  const bool act = line < npair;
+  const bool act = line2 < npair;
+  cpx<T> y[M2];
   if (act) {
-    const int sa = 2 * line, sb = 2 * line + 1;
+    const int sa = 2 * line2, sb = 2 * line2 + 1;
     const bool has_b = sb < ns0;
 #pragma unroll
-    for (int k = 0; k < M; ++k) {
-      const int k1 = cM + k;
+    for (int k = 0; k < M2; ++k) {
+      const int k1 = cM2 + k;
       const bool mirror = k1 >= KH;
       const int kc = mirror ? L - k1 : k1;
       const cpx<T> ya = Z[kc * LD + sa];
       const cpx<T> yb = has_b ? Z[kc * LD + sb] : cpx<T>{T(0), T(0)};
       // Y(s, k1) = conj(Y(s, L - k1)) above L/2
       const T ayi = mirror ? -ya.y : ya.y, byi = mirror ? -yb.y : yb.y;
-      x[k] = cpx<T>{ya.x - byi, ayi + yb.x};
+      y[k] = cpx<T>{ya.x - byi, ayi + yb.x};
     }
-    line_inv<L, LPL, T>(x, q);
+    line_inv<L, P2, T>(y, q2);
   }
   __syncthreads();
   if (act) {
     T* S = reinterpret_cast<T*>(Z);
     const T kScale = T(1) / T(L * L);
-    const int sa = 2 * line, sb = 2 * line + 1;
+    const int sa = 2 * line2, sb = 2 * line2 + 1;
 #pragma unroll
-    for (int m = 0; m < M; ++m) {
-      const int s1 = LPL * m + q;
+    for (int m = 0; m < M2; ++m) {
+      const int s1 = P2 * m + q2;
       if (s1 < ns1) {
-        S[s1 * ns0 + sa] = x[m].x * kScale;
-        if (sb < ns0) S[s1 * ns0 + sb] = x[m].y * kScale;
+        S[s1 * ns0 + sa] = y[m].x * kScale;
+        if (sb < ns0) S[s1 * ns0 + sb] = y[m].y * kScale;
       }
     }
   }
