@@ -3,6 +3,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+#include <set>
+#include <utility>
+
 namespace mqpiv {
 
 enum Alg : int32_t { ALG_FFTCC = 0, ALG_ZNCC = 1, ALG_NSQECC = 2, ALG_MASK_NSQECC = 3 };
@@ -45,5 +49,17 @@ struct Batch {
   int32_t pad_;
   int64_t n_ia_total;
 };
+
+// Raises a kernel's dynamic-LDS limit to the CU's 160 KB on the current device,
+// once per (kernel, device); safe to call from any thread before a launch.
+inline void allow_full_lds(const void* fn) {
+  static std::mutex mu;
+  static std::set<std::pair<const void*, int>> done;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return;
+  std::lock_guard<std::mutex> lock(mu);
+  if (done.insert({fn, dev}).second)
+    (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+}
 
 }  // namespace mqpiv

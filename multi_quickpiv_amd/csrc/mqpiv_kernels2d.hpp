@@ -983,11 +983,7 @@ hipError_t launch_2d_k(const Job& j0, const Job* d_jobs, const int64_t* d_prefix
                        int ns0, int ns1, size_t lds, hipStream_t s) {
   using C = Cfg2<L, T, LPL>;
   const void* fn = (const void*)piv2d_kernel<L, ALG, T, LPL, PF>;
-  static const bool attr_set = [fn] {
-    (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    return true;
-  }();
-  (void)attr_set;
+  allow_full_lds(fn);
   int64_t nb = (n_ia_total + C::IPB - 1) / C::IPB;
   if constexpr (PF) {
     // as many blocks as fit on the device at once, a multiple of the 8 XCDs

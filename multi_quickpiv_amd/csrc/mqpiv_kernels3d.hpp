@@ -561,20 +561,11 @@ hipError_t launch3d_chunk(const Job& J, void* scratch, int64_t chunk_cap, int64_
   const size_t lds1 = (size_t)L * C::LD * sizeof(cpx<T>) + 16 * sizeof(double);
   const size_t lds2 = (size_t)2 * L * C::LD * sizeof(cpx<T>);
   const size_t lds3 = (size_t)L * C::LD * sizeof(cpx<T>);
-  static bool attr = false;
-  if (!attr) {
-    (void)hipFuncSetAttribute((const void*)slab_fwd_kernel<L, ALG_FFTCC, T, LPL>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    (void)hipFuncSetAttribute((const void*)slab_fwd_kernel<L, ALG_ZNCC, T, LPL>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    (void)hipFuncSetAttribute((const void*)slab_fwd_kernel<L, ALG_NSQECC, T, LPL>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    (void)hipFuncSetAttribute((const void*)zline_kernel<L, T, LPL>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              160 * 1024);
-    (void)hipFuncSetAttribute((const void*)slab_inv_kernel<L, T, LPL>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              160 * 1024);
-    attr = true;
-  }
+  allow_full_lds((const void*)slab_fwd_kernel<L, ALG_FFTCC, T, LPL>);
+  allow_full_lds((const void*)slab_fwd_kernel<L, ALG_ZNCC, T, LPL>);
+  allow_full_lds((const void*)slab_fwd_kernel<L, ALG_NSQECC, T, LPL>);
+  allow_full_lds((const void*)zline_kernel<L, T, LPL>);
+  allow_full_lds((const void*)slab_inv_kernel<L, T, LPL>);
   switch (J.alg) {
     case ALG_FFTCC:
       hipLaunchKernelGGL((slab_fwd_kernel<L, ALG_FFTCC, T, LPL>), dim3((unsigned)(n * L)), dim3(C::TPL), lds1, s, J,
