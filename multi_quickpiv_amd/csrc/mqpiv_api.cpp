@@ -177,6 +177,13 @@ static int build_job(mqpiv_ctx* c, const mqpiv_params* p, const int64_t* dims, i
     }
     j->n_ia *= j->vf_n[d];
   }
+  // the kernels index IAs and image coordinates with 32-bit integers
+  for (int d = 0; d < p->ndims; ++d)
+    if (dims[d] >= ((int64_t)1 << 31))
+      return fail(c, MQPIV_ERR_UNSUPPORTED, "dims[%d] = %lld is 2^31 or more", d, (long long)dims[d]);
+  if (j->n_ia >= ((int64_t)1 << 31))
+    return fail(c, MQPIV_ERR_UNSUPPORTED, "%lld IAs in one call (2^31 or more); split the frame",
+                (long long)j->n_ia);
   j->alg = p->corr_alg;
   j->in_dtype = in_dtype;
   j->out_dtype = out_dtype;
@@ -510,6 +517,8 @@ static int plan_batch(mqpiv_ctx* c, const mqpiv_request* r, int n, BatchPlan& P)
     BatchGroup& bg = P.groups[g];
     bg.members.push_back(idx);
     bg.n_ia += j.n_ia;
+    if (bg.n_ia >= ((int64_t)1 << 31))
+      return fail(c, MQPIV_ERR_UNSUPPORTED, "request %d: a batch group holds 2^31 IAs or more", i);
     bg.ns0cap = std::max(bg.ns0cap, 2 * j.sm[0] + 1);
     bg.ns1cap = std::max(bg.ns1cap, 2 * j.sm[1] + 1);
     bg.ss0cap = std::max(bg.ss0cap, j.is[0] + 2 * j.sm[0]);
@@ -572,8 +581,12 @@ static int enqueue_2d(mqpiv_ctx* c, BatchPlan& P, char* sat_base, char* d_table,
       KernelTimer kt(c, s);
       bool f32_in = true;
       for (int m : g.members) f32_in = f32_in && P.jobs2d[m].in_dtype == DT_F32;
-      e = launch_piv2d(g.fp64, g.L, g.alg, P.jobs2d[g.members[0]], dj, dp, (int)g.members.size(), g.n_ia,
-                       g.ns0cap, g.ns1cap, g.ss0cap, f32_in, s);
+      // a group of one pair passes its job in the kernel arguments (read
+      // through the constant cache) instead of the device table, whose record
+      // every workgroup would load before it can address its windows
+      const bool one = g.members.size() == 1;
+      e = launch_piv2d(g.fp64, g.L, g.alg, P.jobs2d[g.members[0]], one ? nullptr : dj, one ? nullptr : dp,
+                       (int)g.members.size(), g.n_ia, g.ns0cap, g.ns1cap, g.ss0cap, f32_in, s);
     }
     if (e != hipSuccess) return fail(c, MQPIV_ERR_DEVICE, "batch launch failed: %s", hipGetErrorString(e));
   }

@@ -239,21 +239,26 @@ struct Geo {
   int32_t off_hi;  // SR_BRB_off
 };
 
-__device__ __forceinline__ Geo geo_of(int64_t v, int32_t is, int32_t sm, int32_t st, int64_t dim) {
+// In 32-bit arithmetic: coordinates stay below dim < 2^31 (the host checks
+// the sizes), and the scalar unit that evaluates this for every wave of a
+// workgroup is shared by the CU's SIMDs.
+__device__ __forceinline__ Geo geo_of(int32_t v, int32_t is, int32_t sm, int32_t st, int32_t dim) {
   Geo g;
-  g.tlf = v * st;                // TLF - 1
-  int64_t brb = g.tlf + is - 1;  // BRB - 1
-  int64_t lo = g.tlf - sm;
-  int64_t hi = brb + sm;
-  g.sr = lo > 0 ? lo : 0;
-  int64_t srb = hi < dim - 1 ? hi : dim - 1;
-  g.gsz = (int32_t)(srb - g.sr + 1);
+  const int32_t tlf = v * st;     // TLF - 1
+  const int32_t brb = tlf + is - 1;  // BRB - 1
+  const int32_t lo = tlf - sm;
+  const int32_t hi = brb + sm;
+  const int32_t sr = lo > 0 ? lo : 0;
+  const int32_t srb = hi < dim - 1 ? hi : dim - 1;
+  g.tlf = tlf;
+  g.sr = sr;
+  g.gsz = srb - sr + 1;
   // SR_TLF_off = |min(0, TLF - (SM - 1))| with 1-based TLF (piv_ops.jl:72)
-  int64_t a = (g.tlf + 1) - (sm - 1);
-  g.goff = (int32_t)(a < 0 ? -a : 0);
+  const int32_t a = (tlf + 1) - (sm - 1);
+  g.goff = a < 0 ? -a : 0;
   // SR_BRB_off = |min(0, dim - (BRB + SM))| (piv_ops.jl:73)
-  int64_t b = dim - ((brb + 1) + sm);
-  g.off_hi = (int32_t)(b < 0 ? -b : 0);
+  const int32_t b = dim - ((brb + 1) + sm);
+  g.off_hi = b < 0 ? -b : 0;
   return g;
 }
 
@@ -270,6 +275,10 @@ __device__ __forceinline__ int find_job(const int64_t* __restrict__ prefix, int 
 
 // XCD-aware remap: consecutive hardware block ids land on different XCDs; give
 // each XCD a contiguous range of IAs so neighbouring windows share its L2.
+__device__ __forceinline__ int32_t xcd_remap32(int32_t b, int32_t nb) {
+  const int32_t q = nb >> 3, r = nb & 7, x = b & 7, k = b >> 3;
+  return x * q + (x < r ? x : r) + k;
+}
 __device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t nb) {
   constexpr int64_t kX = 8;
   int64_t q = nb / kX, r = nb % kX;
@@ -516,8 +525,8 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
     Job Jn;
     int64_t ian;
     locate_ia(xcd_remap(v, n_ia_total), j0, jobs, prefix, n_jobs, Jn, ian);
-    const Geo e0 = geo_of(ian % Jn.vf_n[0], Jn.is[0], Jn.sm[0], Jn.st[0], Jn.dims[0]);
-    const Geo e1 = geo_of(ian / Jn.vf_n[0], Jn.is[1], Jn.sm[1], Jn.st[1], Jn.dims[1]);
+    const Geo e0 = geo_of((int32_t)(ian % Jn.vf_n[0]), Jn.is[0], Jn.sm[0], Jn.st[0], (int32_t)Jn.dims[0]);
+    const Geo e1 = geo_of((int32_t)(ian / Jn.vf_n[0]), Jn.is[1], Jn.sm[1], Jn.st[1], (int32_t)Jn.dims[1]);
     const int tl = (int)threadIdx.x;
     const LaneWin w = lane_window(e0, e1, Jn.is[0], Jn.is[1], Jn.dims[0], tl / LPL);
     const gptr<float> pf = as_global<float>(Jn.img1) + w.f_lane;
@@ -551,17 +560,21 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
   const int line = t / LPL;
   const int h = t % LPL;  // lane within the line's group
 
-  const int64_t b = PF ? xcd_remap(vb, n_ia_total) : xcd_remap(blockIdx.x, gridDim.x);
-  const int64_t ia_g = b * C::IPB + sub;
-  const bool valid = ia_g < n_ia_total;
-  const int64_t ia_c = valid ? ia_g : n_ia_total - 1;
+  // IA indices in 32-bit arithmetic (the host keeps a launch below 2^31 IAs)
+  const int32_t nt = (int32_t)n_ia_total;
+  const int32_t b =
+      PF ? (int32_t)xcd_remap(vb, n_ia_total) : xcd_remap32((int32_t)blockIdx.x, (int32_t)gridDim.x);
+  const int32_t ia_g = b * C::IPB + sub;
+  const bool valid = ia_g < nt;
+  const int32_t ia_c = valid ? ia_g : nt - 1;
   Job J = j0;
-  int64_t ia = ia_c;
+  int32_t ia32 = ia_c;
   if (jobs) {  // batched launch: locate the owning job
     const int jb = find_job(prefix, n_jobs, ia_c);
     J = jobs[jb];
-    ia = ia_c - prefix[jb];
+    ia32 = ia_c - (int32_t)prefix[jb];
   }
+  const int64_t ia = ia32;
 
   const int sm0 = J.sm[0], sm1 = J.sm[1];
   const int is0 = J.is[0], is1 = J.is[1];
@@ -579,10 +592,11 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
   T* C1 = reinterpret_cast<T*>(base + zbytes);
   double* red = reinterpret_cast<double*>(smem + (size_t)C::IPB * per_ia + (size_t)sub * kRedBytes);
 
-  const int64_t v0 = ia % J.vf_n[0];
-  const int64_t v1 = ia / J.vf_n[0];
-  const Geo g0 = geo_of(v0, is0, sm0, J.st[0], J.dims[0]);
-  const Geo g1 = geo_of(v1, is1, sm1, J.st[1], J.dims[1]);
+  const uint32_t vn0 = (uint32_t)J.vf_n[0];
+  const int32_t v1 = (int32_t)((uint32_t)ia32 / vn0);
+  const int32_t v0 = ia32 - v1 * (int32_t)vn0;
+  const Geo g0 = geo_of(v0, is0, sm0, J.st[0], (int32_t)J.dims[0]);
+  const Geo g1 = geo_of(v1, is1, sm1, J.st[1], (int32_t)J.dims[1]);
   const int dt = J.in_dtype;
   const int64_t d0 = J.dims[0];
 

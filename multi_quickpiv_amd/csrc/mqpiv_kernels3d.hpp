@@ -61,13 +61,15 @@ struct Geo3 {
 
 __device__ __forceinline__ Geo3 geo3_of(const Job& J, int64_t ia) {
   Geo3 r;
-  int64_t v0 = ia % J.vf_n[0];
-  int64_t rest = ia / J.vf_n[0];
-  int64_t v1 = rest % J.vf_n[1];
-  int64_t v2 = rest / J.vf_n[1];
-  r.g[0] = geo_of(v0, J.is[0], J.sm[0], J.st[0], J.dims[0]);
-  r.g[1] = geo_of(v1, J.is[1], J.sm[1], J.st[1], J.dims[1]);
-  r.g[2] = geo_of(v2, J.is[2], J.sm[2], J.st[2], J.dims[2]);
+  // 32-bit (the host keeps IA counts below 2^31)
+  const uint32_t i = (uint32_t)ia, n0 = (uint32_t)J.vf_n[0], n1 = (uint32_t)J.vf_n[1];
+  const uint32_t rest = i / n0;
+  const uint32_t v0 = i - rest * n0;
+  const uint32_t v2 = rest / n1;
+  const uint32_t v1 = rest - v2 * n1;
+  r.g[0] = geo_of((int32_t)v0, J.is[0], J.sm[0], J.st[0], (int32_t)J.dims[0]);
+  r.g[1] = geo_of((int32_t)v1, J.is[1], J.sm[1], J.st[1], (int32_t)J.dims[1]);
+  r.g[2] = geo_of((int32_t)v2, J.is[2], J.sm[2], J.st[2], (int32_t)J.dims[2]);
   return r;
 }
 
