@@ -126,6 +126,15 @@ static int check_params(mqpiv_ctx* c, const mqpiv_params* p, const int64_t* dims
     if (p->inter_size[d] > (1 << 20) || p->search_margin[d] > (1 << 20))
       return fail(c, MQPIV_ERR_BAD_PARAMS, "window too large");
   }
+  // the kernels index IAs and image coordinates with 32-bit integers
+  int64_t n_ia = 1;
+  for (int d = 0; d < p->ndims; ++d) {
+    if (dims[d] >= ((int64_t)1 << 31))
+      return fail(c, MQPIV_ERR_UNSUPPORTED, "dims[%d] = %lld is 2^31 or more", d, (long long)dims[d]);
+    n_ia *= vf_len(dims[d], p->inter_size[d], p->step[d]);
+    if (n_ia >= ((int64_t)1 << 31))
+      return fail(c, MQPIV_ERR_UNSUPPORTED, "2^31 or more IAs in one call; split the frame");
+  }
   return MQPIV_OK;
 }
 
@@ -177,13 +186,6 @@ static int build_job(mqpiv_ctx* c, const mqpiv_params* p, const int64_t* dims, i
     }
     j->n_ia *= j->vf_n[d];
   }
-  // the kernels index IAs and image coordinates with 32-bit integers
-  for (int d = 0; d < p->ndims; ++d)
-    if (dims[d] >= ((int64_t)1 << 31))
-      return fail(c, MQPIV_ERR_UNSUPPORTED, "dims[%d] = %lld is 2^31 or more", d, (long long)dims[d]);
-  if (j->n_ia >= ((int64_t)1 << 31))
-    return fail(c, MQPIV_ERR_UNSUPPORTED, "%lld IAs in one call (2^31 or more); split the frame",
-                (long long)j->n_ia);
   j->alg = p->corr_alg;
   j->in_dtype = in_dtype;
   j->out_dtype = out_dtype;

@@ -172,3 +172,21 @@ def test_server_argument_errors_without_gpu():
     assert lib.mqpiv_server_wait(None, 1, 0.0, C.byref(st)) == _lib.MQPIV_ERR_NULL
     assert lib.mqpiv_server_stats_get(None, None) == _lib.MQPIV_ERR_NULL
     assert lib.mqpiv_server_destroy(None) == _lib.MQPIV_ERR_NULL
+
+
+def test_sizes_beyond_32bit_indices_are_rejected_without_gpu():
+    """The kernels index IAs and image coordinates in 32 bits; the parameter
+    check (shared by every entry point) rejects larger calls up front."""
+    lib = _lib.load()
+    p = setPIVParameters(interSize=2, searchMargin=0, step=1)
+    p.ndims = 2
+    cp = to_c_params(p)
+    vf = (C.c_int64 * 3)()
+    n = C.c_int64()
+    ok = (C.c_int64 * 3)(46340, 46341, 1)  # (46339 * 46340) IAs < 2^31
+    assert lib.mqpiv_vf_size(C.byref(cp), ok, vf, C.byref(n)) == _lib.MQPIV_OK
+    assert n.value == 46339 * 46340 < 2 ** 31
+    many = (C.c_int64 * 3)(46342, 46342, 1)  # 46341^2 IAs > 2^31
+    assert lib.mqpiv_vf_size(C.byref(cp), many, vf, C.byref(n)) == _lib.MQPIV_ERR_UNSUPPORTED
+    wide = (C.c_int64 * 3)(2 ** 31, 4, 1)
+    assert lib.mqpiv_vf_size(C.byref(cp), wide, vf, C.byref(n)) == _lib.MQPIV_ERR_UNSUPPORTED
