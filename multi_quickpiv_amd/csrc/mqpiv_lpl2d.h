@@ -50,7 +50,7 @@ template <> struct Lpl2<double, 48> { static constexpr int v = MQPIV_LPL_F64_48;
 #endif
 template <> struct Lpl2<double, 64> { static constexpr int v = MQPIV_LPL_F64_64; };
 #ifndef MQPIV_LPL_F64_96
-#define MQPIV_LPL_F64_96 8
+#define MQPIV_LPL_F64_96 4
 #endif
 template <> struct Lpl2<double, 96> { static constexpr int v = MQPIV_LPL_F64_96; };
 

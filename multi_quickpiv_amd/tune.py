@@ -36,12 +36,12 @@ REPS = 25
 
 def _valid(tag: str, L: int, lpl: int) -> bool:
     """Shapes the kernel supports and that fit in registers: a line of L
-    samples over LPL lanes with at most 32 complex values per lane (16 for
+    samples over LPL lanes with at most 32 complex values per lane (24 for
     FP64), and at most 1024 lanes per IA."""
     if L % lpl or L * lpl > 1024:
         return False
     per_lane = L // lpl
-    return per_lane <= (32 if tag == "f32" else 16) and per_lane >= 4
+    return per_lane <= (32 if tag == "f32" else 24) and per_lane >= 4
 
 
 def build(jobs: int = 8) -> list[Path]:

@@ -115,6 +115,7 @@ int main() {
   both<48, 8, double>("f64");
   both<64, 4, double>("f64");
   both<64, 8, double>("f64");
+  both<96, 4, double>("f64");
   both<96, 8, double>("f64");
   both<96, 16, double>("f64");
   return 0;

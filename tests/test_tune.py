@@ -20,7 +20,7 @@ def _times(base: float, **over):
 def test_valid_shapes():
     assert tune._valid("f32", 64, 4) and tune._valid("f32", 32, 1)
     assert not tune._valid("f32", 48, 1)  # 48 values per lane
-    assert not tune._valid("f64", 32, 1)  # FP64 holds at most 16 per lane
+    assert not tune._valid("f64", 32, 1)  # FP64 holds at most 24 per lane
     assert not tune._valid("f32", 16, 8)  # 2 per lane
     assert not tune._valid("f32", 48, 32)
 
@@ -35,7 +35,7 @@ def test_choose_fastest_agreeing_candidate():
     best = tune.choose({"times_ms": times, "agree": agree})
     assert best[("f32", 64)] == 4
     assert best[("f32", 32)] == 4  # 1 is slower in sum, 2 disagrees
-    assert best[("f64", 96)] == 8  # the only valid LPL for FP64 L = 96
+    assert best[("f64", 96)] == 4  # 24 values a lane is the most FP64 allows
     assert best[("f32", 16)] == 1
 
 
