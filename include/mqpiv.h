@@ -114,6 +114,31 @@ int mqpiv_piv_device(mqpiv_ctx* ctx, const mqpiv_params* p, const int64_t* dims,
                      const uint8_t* d_skip_mask, int32_t out_dtype, void* d_VF, void* d_SN,
                      void* stream);
 
+/* Multipass PIV: the coarse-to-fine loop `for scale in multipass:-1:1` of
+ * PIV_CPU (src/multi_quickPIV.jl:57-88), with the search-region offset hook of
+ * get_search_coordinates(..., displacement) (src/utils/piv_ops.jl:59-76) wired
+ * as piv_ops.jl:160-173 describes; the reference leaves both unfinished (its
+ * passes ignore the offset, and scale > 1 only rescales the peak window).
+ * Pass k = multipass..1 runs with interSize, searchMargin and step times k
+ * (piv_parameters.jl:85-88).  Each pass after the first shifts every search
+ * region by d = the previous pass's vector field interpolated (multilinear, in
+ * vector-field index space) at the IA's centre and rounded to the nearest
+ * integer (ties to even; a NaN input gives 0), clamped so that the IA moved by d
+ * stays inside the image.  The shifted search region is the one the reference
+ * builds for an IA at TLF + d (quirk Q1 included); the IA stays at TLF, and the
+ * pass reports d + its measured displacement.  VF and SN (computeSN: last pass
+ * only) have the multipass = 1 shapes.  Every pass must fit a transform
+ * (2-D search size <= 128, 3-D <= 64; MQPIV_ERR_UNSUPPORTED otherwise).  The
+ * threshold test runs on each pass's own IAs; there is no skip mask.
+ * multipass = 1 equals mqpiv_piv. */
+int mqpiv_piv_multipass(mqpiv_ctx* ctx, const mqpiv_params* p, int32_t multipass, const int64_t* dims,
+                        int32_t in_dtype, const void* img1, const void* img2, const void* mask,
+                        int32_t out_dtype, void* VF, void* SN);
+/* The same over device buffers, enqueued on `stream` (NULL = the context's stream). */
+int mqpiv_piv_multipass_device(mqpiv_ctx* ctx, const mqpiv_params* p, int32_t multipass, const int64_t* dims,
+                               int32_t in_dtype, const void* d_img1, const void* d_img2, const void* d_mask,
+                               int32_t out_dtype, void* d_VF, void* d_SN, void* stream);
+
 /* One frame pair of a batch (device buffers). */
 typedef struct mqpiv_request {
   mqpiv_params params;

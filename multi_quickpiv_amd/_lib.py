@@ -45,6 +45,8 @@ EXPORTS = (
     "mqpiv_piv",
     "mqpiv_piv_ex",
     "mqpiv_piv_device",
+    "mqpiv_piv_multipass",
+    "mqpiv_piv_multipass_device",
     "mqpiv_version",
     "mqpiv_profile_begin",
     "mqpiv_profile_end",
@@ -160,6 +162,12 @@ def load() -> C.CDLL:
         lib.mqpiv_piv_device.restype = C.c_int
         lib.mqpiv_piv_device.argtypes = [vp, C.POINTER(MqpivParams), C.POINTER(C.c_int64), C.c_int32,
                                          vp, vp, vp, vp, C.c_int32, vp, vp, vp]
+        lib.mqpiv_piv_multipass.restype = C.c_int
+        lib.mqpiv_piv_multipass.argtypes = [vp, C.POINTER(MqpivParams), C.c_int32, C.POINTER(C.c_int64),
+                                            C.c_int32, vp, vp, vp, C.c_int32, vp, vp]
+        lib.mqpiv_piv_multipass_device.restype = C.c_int
+        lib.mqpiv_piv_multipass_device.argtypes = [vp, C.POINTER(MqpivParams), C.c_int32, C.POINTER(C.c_int64),
+                                                   C.c_int32, vp, vp, vp, C.c_int32, vp, vp, vp]
         lib.mqpiv_piv_batch_device.restype = C.c_int
         lib.mqpiv_piv_batch_device.argtypes = [vp, C.POINTER(MqpivRequest), C.c_int32, vp]
         lib.mqpiv_graph_create.restype = C.c_int

@@ -57,6 +57,7 @@ SOURCES = [
     "mqpiv_kernels2d.hip",
     "mqpiv_kernels3d.hip",
     "mqpiv_sat.hip",
+    "mqpiv_multipass.hip",
     "mqpiv_api.cpp",
     "mqpiv_server.cpp",
     "mqpiv_post.hip",

@@ -16,6 +16,7 @@
 #include <string>
 #include <utility>
 #include <vector>
+#include <vector>
 
 #include "../../include/mqpiv.h"
 #include "mqpiv_batch.h"
@@ -26,7 +27,7 @@ namespace mqpiv {
 int fft_len_2d(int fp64, int ss);
 size_t lds_2d(int fp64, int L, int alg, int ns0, int ns1, int ss0);
 hipError_t launch_piv2d(int fp64, int L, int alg, const Job& j0, const Job* d_jobs, const int64_t* d_prefix,
-                        int n_jobs, int64_t n_ia_total, int ns0, int ns1, int ss0, bool f32_in, hipStream_t s);
+                        int n_jobs, int64_t n_ia_total, int ns0, int ns1, int ss0, hipStream_t s);
 size_t sat_bytes(const int64_t* dims, int nd);
 size_t sat_tmp_bytes(const int64_t* dims, int nd);
 hipError_t build_sat(const void* img, int dt, int power, const int64_t* dims, int nd, double* S, double* tmp,
@@ -36,6 +37,9 @@ hipError_t build_sat_pair(const void* img, int dt, const int64_t* dims, int nd, 
 int fft_len_3d(int fp64, int ss);
 size_t scratch_3d_bytes(int fp64, int L, const Job& j);
 hipError_t launch_piv3d(int fp64, int L, const Job& j, void* scratch, const double* sat1_img1, hipStream_t s);
+hipError_t launch_coarse_to_shift(const double* vfc, const int32_t* nc, const int32_t* isc, const int32_t* stc,
+                                  const int32_t* nf, const int32_t* isf, const int32_t* stf, int nd,
+                                  int64_t n_fine, int32_t* out, hipStream_t s);
 }  // namespace mqpiv
 
 using namespace mqpiv;
@@ -97,6 +101,7 @@ extern "C" int mqpiv_ctx_destroy(mqpiv_ctx* c) {
   if (c->d_table) hipFree(c->d_table);
   if (c->h_table) hipHostFree(c->h_table);
   if (c->d_pp) hipFree(c->d_pp);
+  if (c->d_mp) hipFree(c->d_mp);
   if (c->table_done) hipEventDestroy(c->table_done);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
@@ -288,7 +293,7 @@ static int launch_job(mqpiv_ctx* c, Job& j, hipStream_t s, char* ws = nullptr) {
     hipError_t e;
     {
       KernelTimer kt(c, s);
-      e = launch_piv2d(fp64, L, j.alg, j, nullptr, nullptr, 1, j.n_ia, ns0, ns1, ss0, j.in_dtype == DT_F32, s);
+      e = launch_piv2d(fp64, L, j.alg, j, nullptr, nullptr, 1, j.n_ia, ns0, ns1, ss0, s);
     }
     if (e != hipSuccess) return fail(c, MQPIV_ERR_DEVICE, "2-D launch failed: %s", hipGetErrorString(e));
     return MQPIV_OK;
@@ -334,9 +339,13 @@ extern "C" int mqpiv_piv(mqpiv_ctx* c, const mqpiv_params* p, const int64_t* dim
   return mqpiv_piv_ex(c, p, dims, in_dtype, img1, img2, mask, skip, out_dtype, VF, SN, nullptr);
 }
 
-extern "C" int mqpiv_piv_ex(mqpiv_ctx* c, const mqpiv_params* p, const int64_t* dims, int32_t in_dtype,
-                            const void* img1, const void* img2, const void* mask, const uint8_t* skip,
-                            int32_t out_dtype, void* VF, void* SN, int32_t* peaks) {
+// Host buffers: validates the request, copies the inputs into the context's
+// staging buffer, runs `run` on the job with device pointers, copies the outputs
+// back and synchronises.
+template <typename Run>
+static int run_staged(mqpiv_ctx* c, const mqpiv_params* p, const int64_t* dims, int32_t in_dtype,
+                      const void* img1, const void* img2, const void* mask, const uint8_t* skip,
+                      int32_t out_dtype, void* VF, void* SN, int32_t* peaks, Run&& run) {
   if (!c) return MQPIV_ERR_NULL;
   std::lock_guard<std::mutex> lk(c->mu);
   if (hipSetDevice(c->device) != hipSuccess) return fail(c, MQPIV_ERR_DEVICE, "hipSetDevice failed");
@@ -387,7 +396,7 @@ extern "C" int mqpiv_piv_ex(mqpiv_ctx* c, const mqpiv_params* p, const int64_t* 
   j.vf = dv;
   j.sn = b_sn ? ds : nullptr;
   j.peaks = b_pk ? (int32_t*)dp : nullptr;
-  rc = launch_job(c, j, s);
+  rc = run(j, s);
   if (rc) return rc;
   if (j.n_ia > 0) {
     HIP_OR_FAIL(c, hipMemcpyAsync(VF, dv, j.n_ia * j.ndims * osz, hipMemcpyDeviceToHost, s));
@@ -397,6 +406,95 @@ extern "C" int mqpiv_piv_ex(mqpiv_ctx* c, const mqpiv_params* p, const int64_t* 
   }
   HIP_OR_FAIL(c, hipStreamSynchronize(s));
   return MQPIV_OK;
+}
+
+extern "C" int mqpiv_piv_ex(mqpiv_ctx* c, const mqpiv_params* p, const int64_t* dims, int32_t in_dtype,
+                            const void* img1, const void* img2, const void* mask, const uint8_t* skip,
+                            int32_t out_dtype, void* VF, void* SN, int32_t* peaks) {
+  return run_staged(c, p, dims, in_dtype, img1, img2, mask, skip, out_dtype, VF, SN, peaks,
+                    [&](Job& j, hipStream_t s) { return launch_job(c, j, s); });
+}
+
+// ---------------------------------------------------------------------------
+// Multipass: pass k = m..1 runs with interSize, searchMargin and step times k
+// (src/piv_parameters.jl:85-88); every pass after the first shifts each search
+// region by the previous pass's field at the IA (mqpiv_multipass.hip) and adds
+// the shift to the displacement it measures; SN comes from the last pass
+// (multi_quickPIV.jl:86).  `base` carries the final pass's buffers.
+static int run_multipass(mqpiv_ctx* c, const mqpiv_params* p, int32_t m, const Job& base, hipStream_t s) {
+  const int nd = p->ndims;
+  std::vector<Job> js(m + 1);
+  size_t vf_max = 0, shift_max = 0;
+  for (int k = m; k >= 2; --k) {
+    mqpiv_params q = *p;
+    for (int d = 0; d < nd; ++d) {
+      q.inter_size[d] *= k;
+      q.search_margin[d] *= k;
+      q.step[d] *= k;
+    }
+    q.compute_sn = 0;
+    // buffers are placed below; a non-null VF placeholder passes validation
+    int rc = build_job(c, &q, base.dims, base.in_dtype, base.img1, base.img2, base.mask, nullptr, MQPIV_F64,
+                       (void*)&js, nullptr, &js[k]);
+    if (rc) return rc;
+    vf_max = std::max(vf_max, (size_t)js[k].n_ia * nd * sizeof(double));
+  }
+  js[1] = base;
+  for (int k = m - 1; k >= 1; --k) shift_max = std::max(shift_max, (size_t)js[k].n_ia * 3 * sizeof(int32_t));
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const size_t need = 2 * al(vf_max) + al(shift_max);
+  if (need) {
+    int rc = ensure(c, &c->d_mp, &c->mp_cap, need, s);
+    if (rc) return rc;
+  }
+  char* slot[2] = {(char*)c->d_mp, (char*)c->d_mp + al(vf_max)};
+  int32_t* shift = (int32_t*)((char*)c->d_mp + 2 * al(vf_max));
+  const Job* prev = nullptr;
+  for (int k = m; k >= 1; --k) {
+    Job& j = js[k];
+    if (k >= 2) j.vf = slot[k & 1];
+    j.shift = nullptr;
+    if (prev) {
+      hipError_t e = launch_coarse_to_shift((const double*)prev->vf, prev->vf_n, prev->is, prev->st, j.vf_n, j.is,
+                                            j.st, nd, j.n_ia, shift, s);
+      if (e != hipSuccess) return fail(c, MQPIV_ERR_DEVICE, "multipass shift launch failed: %s", hipGetErrorString(e));
+      j.shift = shift;
+    }
+    int rc = launch_job(c, j, s);
+    if (rc) return rc;
+    prev = j.n_ia > 0 ? &j : nullptr;
+  }
+  return MQPIV_OK;
+}
+
+static int check_multipass(mqpiv_ctx* c, int32_t m) {
+  if (m < 1 || m > 64) return fail(c, MQPIV_ERR_BAD_PARAMS, "multipass = %d (1..64)", m);
+  return MQPIV_OK;
+}
+
+extern "C" int mqpiv_piv_multipass(mqpiv_ctx* c, const mqpiv_params* p, int32_t multipass, const int64_t* dims,
+                                   int32_t in_dtype, const void* img1, const void* img2, const void* mask,
+                                   int32_t out_dtype, void* VF, void* SN) {
+  if (!c) return MQPIV_ERR_NULL;
+  int rc = check_multipass(c, multipass);
+  if (rc) return rc;
+  return run_staged(c, p, dims, in_dtype, img1, img2, mask, nullptr, out_dtype, VF, SN, nullptr,
+                    [&](Job& j, hipStream_t s) { return run_multipass(c, p, multipass, j, s); });
+}
+
+extern "C" int mqpiv_piv_multipass_device(mqpiv_ctx* c, const mqpiv_params* p, int32_t multipass,
+                                          const int64_t* dims, int32_t in_dtype, const void* d_img1,
+                                          const void* d_img2, const void* d_mask, int32_t out_dtype, void* d_VF,
+                                          void* d_SN, void* stream) {
+  if (!c) return MQPIV_ERR_NULL;
+  std::lock_guard<std::mutex> lk(c->mu);
+  int rc = check_multipass(c, multipass);
+  if (rc) return rc;
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, MQPIV_ERR_DEVICE, "hipSetDevice failed");
+  Job j;
+  rc = build_job(c, p, dims, in_dtype, d_img1, d_img2, d_mask, nullptr, out_dtype, d_VF, d_SN, &j);
+  if (rc) return rc;
+  return run_multipass(c, p, multipass, j, stream ? (hipStream_t)stream : c->stream);
 }
 
 extern "C" int mqpiv_profile_begin(mqpiv_ctx* c) {
@@ -581,14 +679,12 @@ static int enqueue_2d(mqpiv_ctx* c, BatchPlan& P, char* sat_base, char* d_table,
     hipError_t e;
     {
       KernelTimer kt(c, s);
-      bool f32_in = true;
-      for (int m : g.members) f32_in = f32_in && P.jobs2d[m].in_dtype == DT_F32;
       // a group of one pair passes its job in the kernel arguments (read
       // through the constant cache) instead of the device table, whose record
       // every workgroup would load before it can address its windows
       const bool one = g.members.size() == 1;
       e = launch_piv2d(g.fp64, g.L, g.alg, P.jobs2d[g.members[0]], one ? nullptr : dj, one ? nullptr : dp,
-                       (int)g.members.size(), g.n_ia, g.ns0cap, g.ns1cap, g.ss0cap, f32_in, s);
+                       (int)g.members.size(), g.n_ia, g.ns0cap, g.ns1cap, g.ss0cap, s);
     }
     if (e != hipSuccess) return fail(c, MQPIV_ERR_DEVICE, "batch launch failed: %s", hipGetErrorString(e));
   }

@@ -22,6 +22,7 @@ struct Job {
   void* vf;                // N x nVF, component fastest
   void* sn;                // nVF, or nullptr
   int32_t* peaks;          // optional: N int32 per IA, 1-based integer peak (0 = none)
+  const int32_t* shift;    // optional: 3 int32 per IA, search-region shift (multipass)
   const double* sat1;      // FP64 integral array of img2 (sum of pixels), or nullptr
   const double* sat2;      // FP64 integral array of img2^2, or nullptr
   const double* sat1f;     // FP64 integral array of img1 (3-D ZNCC means), or nullptr

@@ -13,40 +13,40 @@ template <int L, typename T, int LPL>
 size_t lds_bytes_2d(int alg, int ns0, int ns1, int ss0);
 template <int L, typename T, int LPL>
 hipError_t launch_2d_l(int alg, const Job& j0, const Job* d_jobs, const int64_t* d_prefix, int n_jobs,
-                       int64_t n_ia_total, int ns0, int ns1, int ss0, bool f32_in, hipStream_t s);
+                       int64_t n_ia_total, int ns0, int ns1, int ss0, hipStream_t s);
 extern template size_t lds_bytes_2d<16, float, Lpl2<float, 16>::v>(int, int, int, int);
 extern template hipError_t launch_2d_l<16, float, Lpl2<float, 16>::v>(int, const Job&, const Job*, const int64_t*, int,
-                                                    int64_t, int, int, int, bool, hipStream_t);
+                                                    int64_t, int, int, int, hipStream_t);
 extern template size_t lds_bytes_2d<32, float, Lpl2<float, 32>::v>(int, int, int, int);
 extern template hipError_t launch_2d_l<32, float, Lpl2<float, 32>::v>(int, const Job&, const Job*, const int64_t*, int,
-                                                    int64_t, int, int, int, bool, hipStream_t);
+                                                    int64_t, int, int, int, hipStream_t);
 extern template size_t lds_bytes_2d<48, float, Lpl2<float, 48>::v>(int, int, int, int);
 extern template hipError_t launch_2d_l<48, float, Lpl2<float, 48>::v>(int, const Job&, const Job*, const int64_t*, int,
-                                                    int64_t, int, int, int, bool, hipStream_t);
+                                                    int64_t, int, int, int, hipStream_t);
 extern template size_t lds_bytes_2d<64, float, Lpl2<float, 64>::v>(int, int, int, int);
 extern template hipError_t launch_2d_l<64, float, Lpl2<float, 64>::v>(int, const Job&, const Job*, const int64_t*, int,
-                                                    int64_t, int, int, int, bool, hipStream_t);
+                                                    int64_t, int, int, int, hipStream_t);
 extern template size_t lds_bytes_2d<96, float, Lpl2<float, 96>::v>(int, int, int, int);
 extern template hipError_t launch_2d_l<96, float, Lpl2<float, 96>::v>(int, const Job&, const Job*, const int64_t*, int,
-                                                    int64_t, int, int, int, bool, hipStream_t);
+                                                    int64_t, int, int, int, hipStream_t);
 extern template size_t lds_bytes_2d<128, float, Lpl2<float, 128>::v>(int, int, int, int);
 extern template hipError_t launch_2d_l<128, float, Lpl2<float, 128>::v>(int, const Job&, const Job*, const int64_t*, int,
-                                                    int64_t, int, int, int, bool, hipStream_t);
+                                                    int64_t, int, int, int, hipStream_t);
 extern template size_t lds_bytes_2d<16, double, Lpl2<double, 16>::v>(int, int, int, int);
 extern template hipError_t launch_2d_l<16, double, Lpl2<double, 16>::v>(int, const Job&, const Job*, const int64_t*, int,
-                                                    int64_t, int, int, int, bool, hipStream_t);
+                                                    int64_t, int, int, int, hipStream_t);
 extern template size_t lds_bytes_2d<32, double, Lpl2<double, 32>::v>(int, int, int, int);
 extern template hipError_t launch_2d_l<32, double, Lpl2<double, 32>::v>(int, const Job&, const Job*, const int64_t*, int,
-                                                    int64_t, int, int, int, bool, hipStream_t);
+                                                    int64_t, int, int, int, hipStream_t);
 extern template size_t lds_bytes_2d<48, double, Lpl2<double, 48>::v>(int, int, int, int);
 extern template hipError_t launch_2d_l<48, double, Lpl2<double, 48>::v>(int, const Job&, const Job*, const int64_t*, int,
-                                                    int64_t, int, int, int, bool, hipStream_t);
+                                                    int64_t, int, int, int, hipStream_t);
 extern template size_t lds_bytes_2d<64, double, Lpl2<double, 64>::v>(int, int, int, int);
 extern template hipError_t launch_2d_l<64, double, Lpl2<double, 64>::v>(int, const Job&, const Job*, const int64_t*, int,
-                                                    int64_t, int, int, int, bool, hipStream_t);
+                                                    int64_t, int, int, int, hipStream_t);
 extern template size_t lds_bytes_2d<96, double, Lpl2<double, 96>::v>(int, int, int, int);
 extern template hipError_t launch_2d_l<96, double, Lpl2<double, 96>::v>(int, const Job&, const Job*, const int64_t*, int,
-                                                    int64_t, int, int, int, bool, hipStream_t);
+                                                    int64_t, int, int, int, hipStream_t);
 
 // Transform length for a search size: the smallest supported length >= ss
 // (FP32: up to 128, FP64: up to 96), or -1.
@@ -86,24 +86,24 @@ size_t lds_2d(int fp64, int L, int alg, int ns0, int ns1, int ss0) {
 }
 
 hipError_t launch_piv2d(int fp64, int L, int alg, const Job& j0, const Job* d_jobs, const int64_t* d_prefix,
-                        int n_jobs, int64_t n_ia_total, int ns0, int ns1, int ss0, bool f32_in, hipStream_t s) {
+                        int n_jobs, int64_t n_ia_total, int ns0, int ns1, int ss0, hipStream_t s) {
   if (n_ia_total <= 0) return hipSuccess;
   if (fp64) {
     switch (L) {
-      case 16: return launch_2d_l<16, double, Lpl2<double, 16>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, f32_in, s);
-      case 32: return launch_2d_l<32, double, Lpl2<double, 32>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, f32_in, s);
-      case 48: return launch_2d_l<48, double, Lpl2<double, 48>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, f32_in, s);
-      case 64: return launch_2d_l<64, double, Lpl2<double, 64>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, f32_in, s);
-      case 96: return launch_2d_l<96, double, Lpl2<double, 96>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, f32_in, s);
+      case 16: return launch_2d_l<16, double, Lpl2<double, 16>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, s);
+      case 32: return launch_2d_l<32, double, Lpl2<double, 32>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, s);
+      case 48: return launch_2d_l<48, double, Lpl2<double, 48>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, s);
+      case 64: return launch_2d_l<64, double, Lpl2<double, 64>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, s);
+      case 96: return launch_2d_l<96, double, Lpl2<double, 96>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, s);
     }
   } else {
     switch (L) {
-      case 16: return launch_2d_l<16, float, Lpl2<float, 16>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, f32_in, s);
-      case 32: return launch_2d_l<32, float, Lpl2<float, 32>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, f32_in, s);
-      case 48: return launch_2d_l<48, float, Lpl2<float, 48>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, f32_in, s);
-      case 64: return launch_2d_l<64, float, Lpl2<float, 64>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, f32_in, s);
-      case 96: return launch_2d_l<96, float, Lpl2<float, 96>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, f32_in, s);
-      case 128: return launch_2d_l<128, float, Lpl2<float, 128>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, f32_in, s);
+      case 16: return launch_2d_l<16, float, Lpl2<float, 16>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, s);
+      case 32: return launch_2d_l<32, float, Lpl2<float, 32>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, s);
+      case 48: return launch_2d_l<48, float, Lpl2<float, 48>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, s);
+      case 64: return launch_2d_l<64, float, Lpl2<float, 64>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, s);
+      case 96: return launch_2d_l<96, float, Lpl2<float, 96>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, s);
+      case 128: return launch_2d_l<128, float, Lpl2<float, 128>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, s);
     }
   }
   return hipErrorInvalidValue;

@@ -231,6 +231,17 @@ __device__ __forceinline__ Best<T> best_of(Best<T> a, Best<T> b) {
 }
 
 // Offsets of one IA in one dim (0-based).
+// Search-region shift of IA ia (index within its job) along dim k, whose
+// vector-field coordinate is v: 0 without a shift array, else clamped so that the
+// shifted IA [TLF + d, BRB + d] stays inside the image.
+__device__ __forceinline__ int32_t shift_of(const Job& J, int32_t ia, int k, int32_t v) {
+  if (!J.shift) return 0;
+  const int32_t d = J.shift[3 * ia + k];
+  const int32_t tlf = v * J.st[k];
+  const int32_t lo = -tlf, hi = (int32_t)J.dims[k] - J.is[k] - tlf;
+  return d < lo ? lo : (d > hi ? hi : d);
+}
+
 struct Geo {
   int64_t tlf;     // IA start
   int64_t sr;      // clipped search-region start
@@ -242,11 +253,16 @@ struct Geo {
 // In 32-bit arithmetic: coordinates stay below dim < 2^31 (the host checks
 // the sizes), and the scalar unit that evaluates this for every wave of a
 // workgroup is shared by the CU's SIMDs.
-__device__ __forceinline__ Geo geo_of(int32_t v, int32_t is, int32_t sm, int32_t st, int32_t dim) {
+// d shifts the search region (multipass): it is the region the reference builds
+// for an IA at TLF + d, while the IA itself stays at TLF.  The caller keeps the
+// shifted IA inside the image (shift_of).
+__device__ __forceinline__ Geo geo_of(int32_t v, int32_t is, int32_t sm, int32_t st, int32_t dim,
+                                      int32_t d = 0) {
   Geo g;
   const int32_t tlf = v * st;     // TLF - 1
-  const int32_t brb = tlf + is - 1;  // BRB - 1
-  const int32_t lo = tlf - sm;
+  const int32_t stl = tlf + d;    // TLF - 1 of the search region's IA
+  const int32_t brb = stl + is - 1;  // BRB - 1
+  const int32_t lo = stl - sm;
   const int32_t hi = brb + sm;
   const int32_t sr = lo > 0 ? lo : 0;
   const int32_t srb = hi < dim - 1 ? hi : dim - 1;
@@ -254,7 +270,7 @@ __device__ __forceinline__ Geo geo_of(int32_t v, int32_t is, int32_t sm, int32_t
   g.sr = sr;
   g.gsz = srb - sr + 1;
   // SR_TLF_off = |min(0, TLF - (SM - 1))| with 1-based TLF (piv_ops.jl:72)
-  const int32_t a = (tlf + 1) - (sm - 1);
+  const int32_t a = (stl + 1) - (sm - 1);
   g.goff = a < 0 ? -a : 0;
   // SR_BRB_off = |min(0, dim - (BRB + SM))| (piv_ops.jl:73)
   const int32_t b = dim - ((brb + 1) + sm);
@@ -278,58 +294,6 @@ __device__ __forceinline__ int find_job(const int64_t* __restrict__ prefix, int 
 __device__ __forceinline__ int32_t xcd_remap32(int32_t b, int32_t nb) {
   const int32_t q = nb >> 3, r = nb & 7, x = b & 7, k = b >> 3;
   return x * q + (x < r ? x : r) + k;
-}
-__device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t nb) {
-  constexpr int64_t kX = 8;
-  int64_t q = nb / kX, r = nb % kX;
-  int64_t x = b % kX, k = b / kX;
-  return x * q + (x < r ? x : r) + k;
-}
-
-// Which job and job-local IA index a launch-global IA index belongs to.
-__device__ __forceinline__ void locate_ia(int64_t ia_c, const Job& j0, const Job* jobs, const int64_t* prefix,
-                                          int n_jobs, Job& J, int64_t& ia) {
-  J = j0;
-  ia = ia_c;
-  if (jobs) {  // batched launch: locate the owning job
-    const int jb = find_job(prefix, n_jobs, ia_c);
-    J = jobs[jb];
-    ia = ia_c - prefix[jb];
-  }
-}
-
-// Where one lane's part of an IA's windows sits in the images.  Lane (line =
-// k0, h) covers k1 = LPL*k + h.  Loads use 32-bit offsets from per-lane base
-// pointers (the host checks dims[0] * L < 2^31).  Every lane issues all its
-// loads before using any (one wait for the batch): positions outside the IA /
-// clipped search region read an in-image pixel instead (the IA origin, or the
-// search region's first row / column) and their values are discarded.
-// g_lane addresses column k1 of the padded buffer, whose image columns start
-// at k1 = goff1.
-struct LaneWin {
-  int64_t f_lane, g_lane;
-  uint32_t ud0;
-  int32_t is1, goff1, gend1;
-  bool f_row, g_row;
-  __device__ __forceinline__ bool f_at(int k1) const { return f_row && k1 < is1; }
-  __device__ __forceinline__ bool g_at(int k1) const { return g_row && k1 >= goff1 && k1 < gend1; }
-  __device__ __forceinline__ uint32_t f_off(int k1) const { return f_at(k1) ? ud0 * (uint32_t)k1 : 0u; }
-  __device__ __forceinline__ uint32_t g_off(int k1) const { return ud0 * (uint32_t)(g_at(k1) ? k1 : goff1); }
-};
-
-__device__ __forceinline__ LaneWin lane_window(const Geo& g0, const Geo& g1, int is0, int is1, int64_t d0, int k0) {
-  LaneWin w;
-  w.f_row = k0 < is0;
-  w.g_row = k0 >= g0.goff && k0 < g0.goff + g0.gsz;
-  const int64_t f_base = (g0.tlf + k0) + d0 * g1.tlf;
-  const int64_t g_base = (g0.sr + (k0 - g0.goff)) + d0 * (g1.sr - g1.goff);
-  w.f_lane = w.f_row ? f_base : (g0.tlf + d0 * g1.tlf);
-  w.g_lane = w.g_row ? g_base : (g0.sr + d0 * (g1.sr - g1.goff));
-  w.ud0 = (uint32_t)d0;
-  w.is1 = is1;
-  w.goff1 = g1.goff;
-  w.gend1 = g1.goff + g1.gsz;
-  return w;
 }
 
 // LDS layout of one IA: the L x L complex transform buffer (which after the
@@ -504,55 +468,20 @@ constexpr int min_waves_2d() {
   return waves < 1 ? 1 : (waves > MQPIV_OCC_CAP ? MQPIV_OCC_CAP : waves);
 }
 
-// One IA per block (PF = false), or a persistent grid (PF = true: FP32 inputs,
-// one IA per block) in which block i runs the IAs of virtual blocks i, i + G,
-// i + 2G, ... (G = gridDim.x, a multiple of 8 so that each block keeps its
-// XCD) and loads the next IA's window into registers before computing the
-// current one, so that the load latency overlaps the transforms.
-template <int L, int ALG, typename T, int LPL, bool PF>
+// One IA per block-slot.  SH: search regions shifted per IA (Job::shift,
+// multipass); a separate instantiation, so that kernels without shifts carry
+// none of its registers.
+template <int L, int ALG, typename T, int LPL, bool SH>
 __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL>())) void piv2d_kernel(Job j0, const Job* __restrict__ jobs,
                                                                       const int64_t* __restrict__ prefix,
                                                                       int n_jobs, int64_t n_ia_total, int ns0_cap,
                                                                       int ns1_cap) {
   using C = Cfg2<L, T, LPL>;
   constexpr int H = C::H;
-  static_assert(!PF || (C::IPB == 1 && ALG != ALG_MASK_NSQECC), "persistent form: one IA per block, unmasked");
   extern __shared__ __align__(16) unsigned char smem[];
-  constexpr int HP = PF ? H : 1;
-  float fpre[HP], gpre[HP];  // PF: the next IA's window (this lane's part)
-  int64_t vb = blockIdx.x;   // PF: the host launches at most n_ia_total blocks
-  auto fetch = [&](int64_t v) {
-    Job Jn;
-    int64_t ian;
-    locate_ia(xcd_remap(v, n_ia_total), j0, jobs, prefix, n_jobs, Jn, ian);
-    const Geo e0 = geo_of((int32_t)(ian % Jn.vf_n[0]), Jn.is[0], Jn.sm[0], Jn.st[0], (int32_t)Jn.dims[0]);
-    const Geo e1 = geo_of((int32_t)(ian / Jn.vf_n[0]), Jn.is[1], Jn.sm[1], Jn.st[1], (int32_t)Jn.dims[1]);
-    const int tl = (int)threadIdx.x;
-    const LaneWin w = lane_window(e0, e1, Jn.is[0], Jn.is[1], Jn.dims[0], tl / LPL);
-    const gptr<float> pf = as_global<float>(Jn.img1) + w.f_lane;
-    const gptr<float> pg = as_global<float>(Jn.img2) + w.g_lane;
-#pragma unroll
-    for (int k = 0; k < HP; ++k) {
-      fpre[k] = pf[w.f_off(LPL * k + tl % LPL)];
-      gpre[k] = pg[w.g_off(LPL * k + tl % LPL)];
-    }
-  };
-  if constexpr (PF) fetch(vb);
 
-  for (;;) {  // once unless PF
-  float fcur[HP], gcur[HP];
-  if constexpr (PF) {
-#pragma unroll
-    for (int k = 0; k < HP; ++k) {
-      fcur[k] = fpre[k];
-      gcur[k] = gpre[k];
-    }
-    if (vb + (int64_t)gridDim.x < n_ia_total) fetch(vb + (int64_t)gridDim.x);
-  }
-  // PF: the lane index is made opaque each iteration so that the values
-  // derived from it are recomputed per IA rather than held across the loop
-  int tid = (int)threadIdx.x;
-  if constexpr (PF) asm volatile("" : "+v"(tid));
+  for (;;) {  // runs once (a loop keeps the register allocation of this body as measured)
+  const int tid = (int)threadIdx.x;
   // one IA per block: every IA quantity below is uniform over the block, which
   // lets the compiler keep it in scalar registers
   const int t = C::IPB == 1 ? tid : tid % C::TPI;
@@ -562,8 +491,7 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
 
   // IA indices in 32-bit arithmetic (the host keeps a launch below 2^31 IAs)
   const int32_t nt = (int32_t)n_ia_total;
-  const int32_t b =
-      PF ? (int32_t)xcd_remap(vb, n_ia_total) : xcd_remap32((int32_t)blockIdx.x, (int32_t)gridDim.x);
+  const int32_t b = xcd_remap32((int32_t)blockIdx.x, (int32_t)gridDim.x);
   const int32_t ia_g = b * C::IPB + sub;
   const bool valid = ia_g < nt;
   const int32_t ia_c = valid ? ia_g : nt - 1;
@@ -595,8 +523,9 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
   const uint32_t vn0 = (uint32_t)J.vf_n[0];
   const int32_t v1 = (int32_t)((uint32_t)ia32 / vn0);
   const int32_t v0 = ia32 - v1 * (int32_t)vn0;
-  const Geo g0 = geo_of(v0, is0, sm0, J.st[0], (int32_t)J.dims[0]);
-  const Geo g1 = geo_of(v1, is1, sm1, J.st[1], (int32_t)J.dims[1]);
+  const int32_t sh0 = SH ? shift_of(J, ia32, 0, v0) : 0, sh1 = SH ? shift_of(J, ia32, 1, v1) : 0;
+  const Geo g0 = geo_of(v0, is0, sm0, J.st[0], (int32_t)J.dims[0], sh0);
+  const Geo g1 = geo_of(v1, is1, sm1, J.st[1], (int32_t)J.dims[1], sh1);
   const int dt = J.in_dtype;
   const int64_t d0 = J.dims[0];
 
@@ -621,8 +550,6 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
   auto f_off = [&](int k1) -> uint32_t { return f_at(k1) ? ud0 * (uint32_t)k1 : 0u; };
   auto g_off = [&](int k1) -> uint32_t { return ud0 * (uint32_t)(g_at(k1) ? k1 : g1.goff); };
 
-  // ---- ZNCC means over the IA and the clipped search region (zncc_ops.jl:10,22;
-  // computed in the input element type, quirk Q12)
   // ZNCC means over the IA and the clipped search region (zncc_ops.jl:10,22),
   // computed in the input element type (quirk Q12) from the loaded window
   float meanF32 = 0.f, meanG32 = 0.f;
@@ -635,13 +562,7 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
   auto load = [&](auto tin) {
     using TIN = decltype(tin);
     TIN fv[H], gv[H], mv[H];
-    if constexpr (PF) {
-#pragma unroll
-      for (int k = 0; k < H; ++k) {
-        fv[k] = fcur[k];
-        gv[k] = gcur[k];
-      }
-    } else {
+    {
       const gptr<TIN> pf = as_global<TIN>(J.img1) + f_lane;
       const gptr<TIN> pg = as_global<TIN>(J.img2) + g_lane;
 #pragma unroll
@@ -709,12 +630,8 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
       x[k] = cpx<T>{(T)a, (T)g};
     }
   };
-  if constexpr (PF) {
-    load(float{});  // the host launches PF for FP32 inputs only
-  } else {
-    if (dt == DT_F32) load(float{});
-    else load(double{});
-  }
+  if (dt == DT_F32) load(float{});
+  else load(double{});
   if constexpr (ALG == ALG_NSQECC || ALG == ALG_MASK_NSQECC) sumF2 = Group<C::TPI>::reduce(sumF2, OpAdd{}, red, t);
 
   // ---- skip test (piv_ops.jl:134-157)
@@ -955,6 +872,10 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
         r1 = ((double)(pk1 + 1) + q1) - (double)(sm1 + 1);
       }
     }
+    if (SH && !skip) {  // the displacement is measured from the shifted search region
+      r0 += (double)sh0;
+      r1 += (double)sh1;
+    }
     if (J.peaks) {
       J.peaks[2 * ia + 0] = has_peak ? pk0 + 1 : 0;
       J.peaks[2 * ia + 1] = has_peak ? pk1 + 1 : 0;
@@ -971,13 +892,7 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
       if (J.sn) ((double*)J.sn)[ia] = sn;
     }
   }
-  if constexpr (!PF) {
-    break;
-  } else {
-    vb += gridDim.x;
-    if (vb >= n_ia_total) break;
-    __syncthreads();  // the next IA reuses the LDS
-  }
+  break;
   }
 }
 
@@ -992,57 +907,42 @@ size_t lds_bytes_2d(int alg, int ns0, int ns1, int ss0) {
   return (z + c1) * C::IPB + (size_t)C::IPB * kRedBytes;
 }
 
-template <int L, int ALG, typename T, int LPL, bool PF>
+template <int L, int ALG, typename T, int LPL, bool SH>
 hipError_t launch_2d_k(const Job& j0, const Job* d_jobs, const int64_t* d_prefix, int n_jobs, int64_t n_ia_total,
                        int ns0, int ns1, size_t lds, hipStream_t s) {
   using C = Cfg2<L, T, LPL>;
-  const void* fn = (const void*)piv2d_kernel<L, ALG, T, LPL, PF>;
-  allow_full_lds(fn);
-  int64_t nb = (n_ia_total + C::IPB - 1) / C::IPB;
-  if constexpr (PF) {
-    // as many blocks as fit on the device at once, a multiple of the 8 XCDs
-    thread_local int resident = 0;
-    thread_local size_t resident_lds = 0;
-    if (resident == 0 || resident_lds != lds) {
-      int dev = 0, per_cu = 0, n_cu = 0;
-      if (hipGetDevice(&dev) != hipSuccess ||
-          hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-          hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, C::NT, lds) != hipSuccess || per_cu < 1)
-        return hipErrorInvalidValue;
-      resident = per_cu * n_cu;
-      resident_lds = lds;
-    }
-    if (nb > resident) nb = resident >= 8 ? resident - resident % 8 : resident;
-  }
-  hipLaunchKernelGGL((piv2d_kernel<L, ALG, T, LPL, PF>), dim3((unsigned)nb), dim3(C::NT), lds, s, j0, d_jobs,
+  allow_full_lds((const void*)piv2d_kernel<L, ALG, T, LPL, SH>);
+  const int64_t nb = (n_ia_total + C::IPB - 1) / C::IPB;
+  hipLaunchKernelGGL((piv2d_kernel<L, ALG, T, LPL, SH>), dim3((unsigned)nb), dim3(C::NT), lds, s, j0, d_jobs,
                      d_prefix, n_jobs, n_ia_total, ns0, ns1);
   return hipGetLastError();
 }
 
+// A single-job launch whose job carries search-region shifts runs the SH
+// instantiation (batches never carry shifts).
 template <int L, int ALG, typename T, int LPL>
 hipError_t launch_2d_t(const Job& j0, const Job* d_jobs, const int64_t* d_prefix, int n_jobs,
-                       int64_t n_ia_total, int ns0, int ns1, int ss0, bool f32_in, hipStream_t s) {
+                       int64_t n_ia_total, int ns0, int ns1, int ss0, hipStream_t s) {
   size_t lds = lds_bytes_2d<L, T, LPL>(ALG, ns0, ns1, ss0);
   if (lds > 160 * 1024) return hipErrorInvalidValue;
-  if constexpr (Pf2<T, L>::v && Cfg2<L, T, LPL>::IPB == 1 && ALG != ALG_MASK_NSQECC) {
-    if (f32_in) return launch_2d_k<L, ALG, T, LPL, true>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, lds, s);
-  }
+  if (j0.shift && !d_jobs)
+    return launch_2d_k<L, ALG, T, LPL, true>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, lds, s);
   return launch_2d_k<L, ALG, T, LPL, false>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, lds, s);
 }
 
 template <int L, typename T, int LPL>
 hipError_t launch_2d_l(int alg, const Job& j0, const Job* d_jobs, const int64_t* d_prefix, int n_jobs,
-                       int64_t n_ia_total, int ns0, int ns1, int ss0, bool f32_in, hipStream_t s) {
+                       int64_t n_ia_total, int ns0, int ns1, int ss0, hipStream_t s) {
   switch (alg) {
     case ALG_FFTCC:
-      return launch_2d_t<L, ALG_FFTCC, T, LPL>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, f32_in, s);
+      return launch_2d_t<L, ALG_FFTCC, T, LPL>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, s);
     case ALG_ZNCC:
-      return launch_2d_t<L, ALG_ZNCC, T, LPL>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, f32_in, s);
+      return launch_2d_t<L, ALG_ZNCC, T, LPL>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, s);
     case ALG_NSQECC:
-      return launch_2d_t<L, ALG_NSQECC, T, LPL>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, f32_in, s);
+      return launch_2d_t<L, ALG_NSQECC, T, LPL>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, s);
     case ALG_MASK_NSQECC:
       return launch_2d_t<L, ALG_MASK_NSQECC, T, LPL>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1,
-                                                    ss0, f32_in, s);
+                                                    ss0, s);
   }
   return hipErrorInvalidValue;
 }

@@ -57,6 +57,7 @@ struct Scratch3 {
 
 struct Geo3 {
   Geo g[3];
+  int32_t d[3];  // search-region shifts (multipass; 0 without)
 };
 
 __device__ __forceinline__ Geo3 geo3_of(const Job& J, int64_t ia) {
@@ -67,9 +68,11 @@ __device__ __forceinline__ Geo3 geo3_of(const Job& J, int64_t ia) {
   const uint32_t v0 = i - rest * n0;
   const uint32_t v2 = rest / n1;
   const uint32_t v1 = rest - v2 * n1;
-  r.g[0] = geo_of((int32_t)v0, J.is[0], J.sm[0], J.st[0], (int32_t)J.dims[0]);
-  r.g[1] = geo_of((int32_t)v1, J.is[1], J.sm[1], J.st[1], (int32_t)J.dims[1]);
-  r.g[2] = geo_of((int32_t)v2, J.is[2], J.sm[2], J.st[2], (int32_t)J.dims[2]);
+  const int32_t v[3] = {(int32_t)v0, (int32_t)v1, (int32_t)v2};
+  for (int k = 0; k < 3; ++k) {
+    r.d[k] = shift_of(J, (int32_t)i, k, v[k]);
+    r.g[k] = geo_of(v[k], J.is[k], J.sm[k], J.st[k], (int32_t)J.dims[k], r.d[k]);
+  }
   return r;
 }
 
@@ -518,6 +521,8 @@ __global__ __launch_bounds__(256) void finalize3d_kernel(Job J, Scratch3 sc, int
         }
       }
     }
+    if (!skip)  // the displacement is measured from the shifted search region
+      for (int d = 0; d < 3; ++d) r[d] += (double)G.d[d];
     if (J.peaks) {
       const bool has = !skip && bst.i != INT32_MAX;
       J.peaks[3 * ia + 0] = has ? bst.i % ns0 + 1 : 0;

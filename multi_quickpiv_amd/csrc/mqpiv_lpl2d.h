@@ -1,8 +1,6 @@
 // Shape of the fused 2-D kernel for each (scalar type, transform length):
-// Lpl2 = lanes that share one line of the transform (1, 2, 4 or 8); Pf2 = 1
-// runs FP32-input launches on a persistent grid that preloads the next IA's
-// window (one IA per block only).  Tuning builds may override an entry with
-// -DMQPIV_LPL_<F32|F64>_<L>=n or -DMQPIV_PF_<F32|F64>_<L>=0|1.
+// Lpl2 = lanes that share one line of the transform (1, 2, 4 or 8).  Tuning
+// builds may override an entry with -DMQPIV_LPL_<F32|F64>_<L>=n.
 #pragma once
 
 namespace mqpiv {
@@ -53,52 +51,5 @@ template <> struct Lpl2<double, 64> { static constexpr int v = MQPIV_LPL_F64_64;
 #define MQPIV_LPL_F64_96 4
 #endif
 template <> struct Lpl2<double, 96> { static constexpr int v = MQPIV_LPL_F64_96; };
-
-template <typename T, int L>
-struct Pf2 { static constexpr int v = 0; };
-#ifndef MQPIV_PF_F32_16
-#define MQPIV_PF_F32_16 0
-#endif
-template <> struct Pf2<float, 16> { static constexpr int v = MQPIV_PF_F32_16; };
-#ifndef MQPIV_PF_F32_32
-#define MQPIV_PF_F32_32 0
-#endif
-template <> struct Pf2<float, 32> { static constexpr int v = MQPIV_PF_F32_32; };
-#ifndef MQPIV_PF_F32_48
-#define MQPIV_PF_F32_48 0
-#endif
-template <> struct Pf2<float, 48> { static constexpr int v = MQPIV_PF_F32_48; };
-#ifndef MQPIV_PF_F32_64
-#define MQPIV_PF_F32_64 0
-#endif
-template <> struct Pf2<float, 64> { static constexpr int v = MQPIV_PF_F32_64; };
-#ifndef MQPIV_PF_F32_96
-#define MQPIV_PF_F32_96 0
-#endif
-template <> struct Pf2<float, 96> { static constexpr int v = MQPIV_PF_F32_96; };
-#ifndef MQPIV_PF_F32_128
-#define MQPIV_PF_F32_128 0
-#endif
-template <> struct Pf2<float, 128> { static constexpr int v = MQPIV_PF_F32_128; };
-#ifndef MQPIV_PF_F64_16
-#define MQPIV_PF_F64_16 0
-#endif
-template <> struct Pf2<double, 16> { static constexpr int v = MQPIV_PF_F64_16; };
-#ifndef MQPIV_PF_F64_32
-#define MQPIV_PF_F64_32 0
-#endif
-template <> struct Pf2<double, 32> { static constexpr int v = MQPIV_PF_F64_32; };
-#ifndef MQPIV_PF_F64_48
-#define MQPIV_PF_F64_48 0
-#endif
-template <> struct Pf2<double, 48> { static constexpr int v = MQPIV_PF_F64_48; };
-#ifndef MQPIV_PF_F64_64
-#define MQPIV_PF_F64_64 0
-#endif
-template <> struct Pf2<double, 64> { static constexpr int v = MQPIV_PF_F64_64; };
-#ifndef MQPIV_PF_F64_96
-#define MQPIV_PF_F64_96 0
-#endif
-template <> struct Pf2<double, 96> { static constexpr int v = MQPIV_PF_F64_96; };
 
 }  // namespace mqpiv

@@ -30,16 +30,20 @@ from .params import (
 )
 
 
-def to_c_params(p: PIVParameters, corr_code: Optional[int] = None, mode: str = "auto") -> _lib.MqpivParams:
+def to_c_params(p: PIVParameters, corr_code: Optional[int] = None, mode: str = "auto",
+                multipass_ok: bool = False) -> _lib.MqpivParams:
     """Resolve a PIVParameters into the C struct (step = interSize - overlap when
     ``step`` is ``nothing``, piv_parameters.jl:88).  ``mode`` picks the transform
-    precision: "fp32", "fp64" or "auto" (FP64 where FP32 misses the parity bar)."""
+    precision: "fp32", "fp64" or "auto" (FP64 where FP32 misses the parity bar).
+    ``multipass`` travels separately (mqpiv_piv_multipass); callers that run a
+    single pass leave ``multipass_ok`` False and reject multipass > 1."""
     if not (p.unpadded and p.good_pad):
         # piv_ops.jl:261-268: the padded branch prints an error and mis-indexes.
         raise NotImplementedError("only unpadded = good_pad = true is supported (as in the reference)")
-    if p.multipass != 1:
-        # multi_quickPIV.jl:57 with piv_ops.jl:254: multipass > 1 is inconsistent.
-        raise NotImplementedError("multipass > 1 is not supported")
+    if p.multipass < 1:
+        raise ValueError("multipass must be at least 1")
+    if p.multipass != 1 and not multipass_ok:
+        raise NotImplementedError("multipass > 1 runs through PIV / piv_device only")
     cp = _lib.MqpivParams()
     cp.corr_alg = p.corr_alg.code if corr_code is None else corr_code
     cp.ndims = p.ndims
@@ -140,14 +144,25 @@ def PIV(input1: Any, input2: Any, *args: Any, precision: Any = np.float64,  # no
     PK = np.zeros((N, *vf), dtype=np.int32, order="F") if return_peaks else None
     skip = None
     custom = (pivparams.filtFun is not maxval) or (masked and pivparams.mask_filtFun is not maxval)
+    mp = int(pivparams.multipass)
+    if mp > 1 and (custom or len(masks) == 2 or return_peaks):
+        raise NotImplementedError("multipass > 1 takes the default filtFun, one mask and no peaks output")
     if custom or len(masks) == 2:
         skip = host_skip_mask(img1, m_vf, pivparams, masked)
     if int(np.prod(vf)) == 0:
         return (VF, SN, PK) if return_peaks else (VF, SN)
 
-    cp = to_c_params(pivparams, corr_code, mode)
+    cp = to_c_params(pivparams, corr_code, mode, multipass_ok=True)
     dims = (C.c_int64 * 3)(*(list(img1.shape) + [1] * (3 - N)))
     ctx = _lib.context(device)
+    if mp > 1:
+        # multi_quickPIV.jl:57-88, coarse to fine (mqpiv_piv_multipass)
+        rc = ctx.lib.mqpiv_piv_multipass(
+            ctx.handle, C.byref(cp), mp, dims, in_code, a1.ctypes.data, a2.ctypes.data,
+            m_ia.ctypes.data if m_ia is not None else None,
+            out_code, VF.ctypes.data, SN.ctypes.data if SN is not None else None)
+        ctx.check(rc)
+        return VF, SN
     rc = ctx.lib.mqpiv_piv_ex(
         ctx.handle, C.byref(cp), dims, in_code,
         a1.ctypes.data, a2.ctypes.data,
@@ -165,11 +180,16 @@ def piv_device(ctx: "_lib.Context", pivparams: PIVParameters, shape: Tuple[int, 
                d_sn: Optional[int], stream: Optional[int] = None, masked: bool = False,
                mode: str = "auto") -> None:
     """Asynchronous PIV over device-resident buffers (raw device pointers, e.g.
-    ``tensor.data_ptr()``; tensors laid out column-major as in ``PIV``)."""
-    cp = to_c_params(pivparams, mask_NSQECC.code if masked else None, mode)
+    ``tensor.data_ptr()``; tensors laid out column-major as in ``PIV``).
+    ``pivparams.multipass`` > 1 runs the coarse-to-fine passes."""
+    cp = to_c_params(pivparams, mask_NSQECC.code if masked else None, mode, multipass_ok=True)
     dims = (C.c_int64 * 3)(*(list(shape) + [1] * (3 - len(shape))))
-    rc = ctx.lib.mqpiv_piv_device(ctx.handle, C.byref(cp), dims, in_dtype, d_img1, d_img2, d_mask,
-                                  None, out_dtype, d_vf, d_sn, stream)
+    if pivparams.multipass > 1:
+        rc = ctx.lib.mqpiv_piv_multipass_device(ctx.handle, C.byref(cp), int(pivparams.multipass), dims, in_dtype,
+                                                d_img1, d_img2, d_mask, out_dtype, d_vf, d_sn, stream)
+    else:
+        rc = ctx.lib.mqpiv_piv_device(ctx.handle, C.byref(cp), dims, in_dtype, d_img1, d_img2, d_mask,
+                                      None, out_dtype, d_vf, d_sn, stream)
     ctx.check(rc)
 
 

@@ -29,6 +29,21 @@
  *   orc_piv                src/multi_quickPIV.jl:40-96  (PIV_CPU loop), with the
  *                          masked variant following the intended semantics of
  *                          multi_quickPIV.jl:99-154 and piv_ops.jl:148-157.
+ *   orc_shift_field,       multipass: the loop of multi_quickPIV.jl:57-88 with the
+ *   orc_piv_multipass      scaled sizes of piv_parameters.jl:85-88 and the
+ *                          search-region offset of get_search_coordinates
+ *                          (piv_ops.jl:59-76), wired as piv_ops.jl:160-173
+ *                          describes (the reference leaves it unfinished, so this
+ *                          is the definition the device follows, stated once):
+ *                          pass k = m..1 scales interSize, searchMargin, step by k;
+ *                          each later pass shifts every search region by d = the
+ *                          previous field interpolated multilinearly (in
+ *                          vector-field index space) at the IA centre, rounded to
+ *                          nearest (ties to even), NaN -> 0, clamped so that the
+ *                          IA moved by d stays in the image; the search region is
+ *                          the one the reference builds for an IA at TLF + d (Q1
+ *                          included), the IA stays at TLF, and the pass reports d
+ *                          plus its displacement; SN from the last pass.
  *
  * Two correlation modes:
  *   ORC_DIRECT : C(s) = sum_x F[x] G[x+s] by direct summation in double (the exact
@@ -142,6 +157,26 @@ void orc_coords_of(const orc_params* p, const int64_t* dims, const int64_t* vf,
     /* abs(min(0, dim - (BRB + SM))) -- piv_ops.jl:73 */
     int64_t b = dims[d] - (brb + sm);
     c->off_hi[d] = b < 0 ? -b : 0;
+  }
+}
+
+/* The coordinates of IA vf_idx whose search region is shifted by d (1-based
+ * TLF + d, clamped into [1, dim - is + 1]): SR corners and offsets as for an IA
+ * at TLF + d; the IA corners stay.  Writes the clamped shift back into d.     */
+void orc_coords_shifted(const orc_params* p, const int64_t* dims, const int64_t* vf,
+                               int64_t vf_idx, int64_t* d, orc_coords* c) {
+  orc_coords_of(p, dims, vf, vf_idx, c);
+  for (int k = 0; k < p->ndims; ++k) {
+    int64_t lo = 1 - c->tlf[k], hi = dims[k] - c->brb[k];
+    if (d[k] < lo) d[k] = lo;
+    if (d[k] > hi) d[k] = hi;
+    int64_t tlf = c->tlf[k] + d[k], brb = c->brb[k] + d[k], sm = p->sm[k];
+    c->sr_tlf[k] = (tlf - sm > 1) ? tlf - sm : 1;
+    c->sr_brb[k] = (brb + sm < dims[k]) ? brb + sm : dims[k];
+    int64_t a = tlf - (sm - 1);
+    c->off_lo[k] = a < 0 ? -a : 0;
+    int64_t b = dims[k] - (brb + sm);
+    c->off_hi[k] = b < 0 ? -b : 0;
   }
 }
 
@@ -755,9 +790,15 @@ static int skip_ia(const orc_params* p, const int64_t* dims, const void* img1,
 static int one_ia(const orc_params* p, const int64_t* dims, const int64_t* vf,
                   const void* img1, const void* img2, const void* mask,
                   const uint8_t* skip_mask, int64_t vf_idx, int mode, orc_work* w,
-                  double* disp, double* sn, double* aux) {
+                  const int32_t* shift, double* disp, double* sn, double* aux) {
   orc_coords c;
-  orc_coords_of(p, dims, vf, vf_idx, &c);
+  int64_t dsh[3] = {0, 0, 0};
+  if (shift) {
+    for (int k = 0; k < 3; ++k) dsh[k] = shift[3 * (vf_idx - 1) + k];
+    orc_coords_shifted(p, dims, vf, vf_idx, dsh, &c);
+  } else {
+    orc_coords_of(p, dims, vf, vf_idx, &c);
+  }
   int skipped = skip_mask ? (skip_mask[vf_idx - 1] != 0) : skip_ia(p, dims, img1, mask, &c);
   if (skipped) return 1;
   load_windows(p, dims, img1, img2, mask, &c, w);
@@ -766,6 +807,7 @@ static int one_ia(const orc_params* p, const int64_t* dims, const int64_t* vf,
   double val;
   orc_peak(w->S, w->ns, p->ndims, p->sm, c.off_lo, c.off_hi, peak, &val);
   orc_subpixel(w->S, w->ns, p->ndims, p->sm, c.off_lo, c.off_hi, peak, val, disp);
+  for (int k = 0; k < p->ndims; ++k) disp[k] += (double)dsh[k];
   if (sn) *sn = orc_sn(w->S, w->ns, p->ndims, p->sm, c.off_lo, c.off_hi);
   if (aux) {
     aux[0] = (double)peak[0]; aux[1] = (double)peak[1]; aux[2] = (double)peak[2];
@@ -781,9 +823,23 @@ static int one_ia(const orc_params* p, const int64_t* dims, const int64_t* vf,
  * are left untouched (the caller zero-initialises, as allocate_outputs does).
  * aux (optional): 6 doubles per IA (peak r,c,z, peak value, runner-up value, 0).
  * Returns 0, or -1 on bad parameters.                                          */
+static int piv_shifted(const orc_params* p, const int64_t* dims, const void* img1, const void* img2,
+                       const void* mask, const uint8_t* skip_mask, const int32_t* shift, int mode,
+                       int nthreads, int64_t ia_begin, int64_t ia_end, double* VF, double* SN,
+                       double* aux);
+
 int orc_piv(const orc_params* p, const int64_t* dims, const void* img1, const void* img2,
             const void* mask, const uint8_t* skip_mask, int mode, int nthreads,
             int64_t ia_begin, int64_t ia_end, double* VF, double* SN, double* aux) {
+  return piv_shifted(p, dims, img1, img2, mask, skip_mask, NULL, mode, nthreads, ia_begin, ia_end,
+                     VF, SN, aux);
+}
+
+/* orc_piv with per-IA search-region shifts (3 int32 per IA; NULL = none). */
+static int piv_shifted(const orc_params* p, const int64_t* dims, const void* img1, const void* img2,
+                       const void* mask, const uint8_t* skip_mask, const int32_t* shift, int mode,
+                       int nthreads, int64_t ia_begin, int64_t ia_end, double* VF, double* SN,
+                       double* aux) {
   if (p->ndims < 2 || p->ndims > 3) return -1;
   if (p->alg == ORC_MASK_NSQECC && !mask) return -1;
   int64_t vf[3];
@@ -804,7 +860,7 @@ int orc_piv(const orc_params* p, const int64_t* dims, const void* img1, const vo
 #endif
     for (int64_t i = ia_begin; i < ia_end; ++i) {
       double disp[3] = {0, 0, 0}, sn = 0.0;
-      int skipped = one_ia(p, dims, vf, img1, img2, mask, skip_mask, i + 1, mode, &w, disp,
+      int skipped = one_ia(p, dims, vf, img1, img2, mask, skip_mask, i + 1, mode, &w, shift, disp,
                            SN ? &sn : NULL, aux ? aux + 6 * i : NULL);
       if (skipped) continue;
       for (int d = 0; d < nd; ++d) VF[d + nd * i] = disp[d];
@@ -847,4 +903,116 @@ int orc_ia_correlation(const orc_params* p, const int64_t* dims, const void* img
   orc_params q = *p;
   q.alg = ORC_FFTCC;
   return orc_ia_similarity(&q, dims, img1, img2, NULL, vf_idx, mode, out, NULL);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Multipass (definition in the header of this file).                        */
+
+/* Search-region shifts (3 int32 per fine IA, 0 beyond nd) from the coarse
+ * field vfc (nd x nVF_c doubles, component fastest).  Each dim places the fine
+ * IA's centre v*stf + (isf-1)/2 (0-based pixels) in coarse index units:
+ * t = (2*v*stf + (isf-1) - (isc-1)) / (2*stc), clamped to [0, nc-1]; corners
+ * a = min(floor(t), max(nc-2, 0)), b = min(a+1, nc-1), weight of b = t - a.
+ * The corners are summed with b0 fastest, weights multiplied dim 0 first.   */
+void orc_shift_field(int nd, const double* vfc, const int64_t* nc, const int64_t* isc,
+                     const int64_t* stc, const int64_t* nf, const int64_t* isf, const int64_t* stf,
+                     int32_t* out) {
+  int64_t n = 1;
+  for (int k = 0; k < nd; ++k) n *= nf[k];
+  for (int64_t i = 0; i < n; ++i) {
+    int64_t ia[3] = {0, 0, 0}, ib[3] = {0, 0, 0};
+    double w[3] = {0.0, 0.0, 0.0};
+    int64_t r = i;
+    for (int k = 0; k < nd; ++k) {
+      int64_t v = r % nf[k];
+      r /= nf[k];
+      int64_t num = 2 * v * stf[k] + (isf[k] - 1) - (isc[k] - 1);
+      double t = (double)num / (double)(2 * stc[k]);
+      double tmax = (double)(nc[k] - 1);
+      t = t < 0.0 ? 0.0 : (t > tmax ? tmax : t);
+      int64_t a = (int64_t)floor(t);
+      int64_t top = nc[k] >= 2 ? nc[k] - 2 : 0;
+      if (a > top) a = top;
+      ia[k] = a;
+      ib[k] = a + 1 < nc[k] ? a + 1 : nc[k] - 1;
+      w[k] = t - (double)a;
+    }
+    for (int c = 0; c < 3; ++c) {
+      int32_t d = 0;
+      if (c < nd) {
+        double acc = 0.0;
+        for (int b2 = 0; b2 < (nd == 3 ? 2 : 1); ++b2)
+          for (int b1 = 0; b1 < 2; ++b1)
+            for (int b0 = 0; b0 < 2; ++b0) {
+              double wt = (b0 ? w[0] : 1.0 - w[0]) * (b1 ? w[1] : 1.0 - w[1]);
+              if (nd == 3) wt = wt * (b2 ? w[2] : 1.0 - w[2]);
+              int64_t u = (b0 ? ib[0] : ia[0]) + nc[0] * ((b1 ? ib[1] : ia[1]) + nc[1] * (b2 ? ib[2] : ia[2]));
+              acc = acc + wt * vfc[nd * u + c];
+            }
+        if (acc == acc) {
+          const double lim = 1073741824.0;
+          acc = acc > lim ? lim : (acc < -lim ? -lim : acc);
+          d = (int32_t)nearbyint(acc);
+        }
+      }
+      out[3 * i + c] = d;
+    }
+  }
+}
+
+/* Multipass PIV over all IAs: VF/SN of the last pass (the caller zero-fills
+ * them).  Returns 0, or -1 on bad parameters.                               */
+int orc_piv_multipass(const orc_params* p, int m, const int64_t* dims, const void* img1,
+                      const void* img2, const void* mask, int mode, int nthreads, double* VF,
+                      double* SN) {
+  if (m < 1 || p->ndims < 2 || p->ndims > 3) return -1;
+  const int nd = p->ndims;
+  double* prev = NULL;
+  int64_t pn[3] = {1, 1, 1}, pis[3] = {1, 1, 1}, pst[3] = {1, 1, 1};
+  for (int k = m; k >= 1; --k) {
+    orc_params q = *p;
+    for (int d = 0; d < nd; ++d) {
+      q.is[d] *= k;
+      q.sm[d] *= k;
+      q.st[d] *= k;
+    }
+    if (k > 1) q.compute_sn = 0;
+    int64_t vf[3];
+    int64_t n = orc_vf_size(&q, dims, vf);
+    int32_t* shift = NULL;
+    if (prev && n > 0) {
+      shift = (int32_t*)calloc(3 * n, sizeof(int32_t));
+      orc_shift_field(nd, prev, pn, pis, pst, vf, q.is, q.st, shift);
+    }
+    double* out = VF;
+    double* sn = SN;
+    if (k > 1) {
+      out = (double*)calloc(n * nd > 0 ? n * nd : 1, sizeof(double));
+      sn = NULL;
+    }
+    int rc = n > 0 ? piv_shifted(&q, dims, img1, img2, mask, NULL, shift, mode, nthreads, 0, n, out,
+                                 q.compute_sn ? sn : NULL, NULL)
+                   : 0;
+    free(shift);
+    free(prev);
+    prev = NULL;
+    if (rc) {
+      if (k > 1) free(out);
+      return rc;
+    }
+    if (k > 1) {
+      if (n > 0) {
+        prev = out;
+        for (int d = 0; d < 3; ++d) {
+          pn[d] = vf[d];
+          pis[d] = q.is[d];
+          pst[d] = q.st[d];
+        }
+      } else {
+        free(out);
+      }
+    }
+  }
+  free(prev);
+  return 0;
 }

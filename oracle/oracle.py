@@ -52,6 +52,14 @@ def load() -> C.CDLL:
         lib.orc_piv.restype = C.c_int
         lib.orc_piv.argtypes = [C.POINTER(OrcParams), C.POINTER(C.c_int64), vp, vp, vp, vp, C.c_int,
                                 C.c_int, C.c_int64, C.c_int64, vp, vp, vp]
+        lib.orc_piv_multipass.restype = C.c_int
+        lib.orc_piv_multipass.argtypes = [C.POINTER(OrcParams), C.c_int, C.POINTER(C.c_int64), vp, vp, vp,
+                                          C.c_int, C.c_int, vp, vp]
+        lib.orc_coords_shifted.restype = None
+        lib.orc_coords_shifted.argtypes = [C.POINTER(OrcParams), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                           C.c_int64, C.POINTER(C.c_int64), vp]
+        lib.orc_shift_field.restype = None
+        lib.orc_shift_field.argtypes = [C.c_int, vp] + [C.POINTER(C.c_int64)] * 6 + [vp]
         lib.orc_ia_similarity.restype = C.c_int
         lib.orc_ia_similarity.argtypes = [C.POINTER(OrcParams), C.POINTER(C.c_int64), vp, vp, vp,
                                           C.c_int64, C.c_int, vp, vp]
@@ -138,6 +146,47 @@ def piv(img1, img2, p: OrcParams, mask=None, skip_mask=None, mode: int = DIRECT,
         if rc != 0:
             raise ValueError(f"orc_piv failed ({rc})")
     return VF, SN, aux
+
+
+def piv_multipass(img1, img2, p: OrcParams, multipass: int, mask=None, mode: int = DIRECT,
+                  nthreads: int = 1):
+    """Multipass PIV (orc_piv_multipass): (VF, SN) of the last pass."""
+    lib = load()
+    dt = np.float32 if p.in_dtype == 0 else np.float64
+    a1, a2, am = _prep(img1, dt), _prep(img2, dt), _prep(mask, dt)
+    shape = a1.shape
+    vf = vf_size(p, shape)
+    VF = np.zeros((p.ndims, *vf), dtype=np.float64, order="F")
+    SN = np.zeros(vf, dtype=np.float64, order="F") if p.compute_sn else None
+    rc = lib.orc_piv_multipass(C.byref(p), multipass, _i3(shape, 1), a1.ctypes.data, a2.ctypes.data,
+                               am.ctypes.data if am is not None else None, mode, nthreads, VF.ctypes.data,
+                               SN.ctypes.data if SN is not None else None)
+    if rc != 0:
+        raise ValueError(f"orc_piv_multipass failed ({rc})")
+    return VF, SN
+
+
+def coords_shifted(p: OrcParams, shape: Sequence[int], vf_idx: int, d: Sequence[int]):
+    """(coords, clamped d): the 18 coordinates (TLF, BRB, SR_TLF, SR_BRB,
+    SR_TLF_off, SR_BRB_off; 3 each) of IA vf_idx (1-based) whose search region
+    is shifted by d, and the shift after clamping."""
+    lib = load()
+    vf = (C.c_int64 * 3)(*(list(vf_size(p, shape)) + [1] * (3 - p.ndims)))
+    dd = _i3(d, 0)
+    buf = np.zeros(18, dtype=np.int64)
+    lib.orc_coords_shifted(C.byref(p), _i3(shape, 1), vf, vf_idx, dd, buf.ctypes.data)
+    return buf.reshape(6, 3), tuple(dd[k] for k in range(p.ndims))
+
+
+def shift_field(vf_coarse: np.ndarray, is_c, st_c, vf_fine_shape, is_f, st_f) -> np.ndarray:
+    """Search-region shifts (shape (3, nVF_fine...), int32) that a pass with IA
+    size is_f and step st_f takes from the coarse field vf_coarse (N, nVF_c...)."""
+    nd = vf_coarse.shape[0]
+    vc = np.asfortranarray(vf_coarse, dtype=np.float64)
+    out = np.zeros((3, *vf_fine_shape), dtype=np.int32, order="F")
+    load().orc_shift_field(nd, vc.ctypes.data, _i3(vf_coarse.shape[1:], 1), _i3(is_c, 1), _i3(st_c, 1),
+                           _i3(vf_fine_shape, 1), _i3(is_f, 1), _i3(st_f, 1), out.ctypes.data)
+    return out
 
 
 def ia_similarity(img1, img2, p: OrcParams, vf_idx: int, mask=None, mode: int = DIRECT):

@@ -190,3 +190,22 @@ def test_sizes_beyond_32bit_indices_are_rejected_without_gpu():
     assert lib.mqpiv_vf_size(C.byref(cp), many, vf, C.byref(n)) == _lib.MQPIV_ERR_UNSUPPORTED
     wide = (C.c_int64 * 3)(2 ** 31, 4, 1)
     assert lib.mqpiv_vf_size(C.byref(cp), wide, vf, C.byref(n)) == _lib.MQPIV_ERR_UNSUPPORTED
+
+
+def test_multipass_argument_checks_without_gpu():
+    """multipass > 1 takes the default filtFun, at most one mask and no peaks
+    output; multipass < 1 is rejected; all before any device call."""
+    img = np.random.default_rng(0).random((64, 64))
+    p = setPIVParameters(interSize=16, searchMargin=4, multipass=2, filtFun=lambda w: float(np.mean(w)),
+                         threshold=0.1)
+    with pytest.raises(NotImplementedError):
+        mq.PIV(img, img, p)
+    p = setPIVParameters(interSize=16, searchMargin=4, multipass=2)
+    with pytest.raises(NotImplementedError):
+        mq.PIV(img, img, img, img, p)
+    with pytest.raises(NotImplementedError):
+        mq.PIV(img, img, p, return_peaks=True)
+    p = setPIVParameters(multipass=0)
+    p.ndims = 2
+    with pytest.raises(ValueError):
+        to_c_params(p, multipass_ok=True)
