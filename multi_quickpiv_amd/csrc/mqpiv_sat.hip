@@ -112,6 +112,38 @@ __global__ __launch_bounds__(256) void sat_chunk_scan_kernel(double* __restrict_
   tot[p * nch + c] = acc;
 }
 
+// Strided pass along dim d in one sweep: every thread scans one whole column in
+// place, its loads issued kScanBatch at a time.  Used when the columns alone give
+// enough threads to fill the device (3-D volumes), halving the table traffic of
+// the chunked scan + fix.
+__global__ __launch_bounds__(256) void sat_column_scan_kernel(double* __restrict__ S0, int64_t table_stride,
+                                                              int64_t n_pos, int64_t len, int64_t stride,
+                                                              int64_t pos_mod, int64_t pos_mul) {
+  double* S = S0 + blockIdx.y * table_stride;
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_pos) return;
+  const int64_t base = (p % pos_mod) + (p / pos_mod) * pos_mul;
+  double acc = 0.0;
+  for (int64_t jb = 0; jb < len; jb += kScanBatch) {
+    double v[kScanBatch];
+#pragma unroll
+    for (int u = 0; u < kScanBatch; ++u) {
+      const int64_t j = jb + u < len ? jb + u : len - 1;
+      v[u] = S[base + j * stride];
+    }
+#pragma unroll
+    for (int u = 0; u < kScanBatch; ++u) {
+      if (jb + u < len) {
+        acc += v[u];
+        S[base + (jb + u) * stride] = acc;
+      }
+    }
+  }
+}
+
+// Columns at least this many (over the tables built together): one sweep.
+constexpr int64_t kColumnSweepMin = 65536;
+
 // Adds the running total of the preceding chunks to every entry of a chunk.
 __global__ __launch_bounds__(256) void sat_chunk_fix_kernel(double* __restrict__ S0, int64_t table_stride,
                                                             int64_t n_pos, int64_t len, int64_t stride,
@@ -179,6 +211,11 @@ static hipError_t build_tables(const void* img, int dt, int power, const int64_t
     const int64_t pos_mod = D0;
     const int64_t pos_mul = d == 1 ? D0 * D1 : D0;
     const int64_t n_pos = d == 1 ? D0 * D2 : D0 * D1;
+    if (n_pos * ntab >= kColumnSweepMin) {
+      hipLaunchKernelGGL(sat_column_scan_kernel, dim3((unsigned)((n_pos + 255) / 256), ntab), dim3(256), 0, s, Sa,
+                         table_stride, n_pos, len, stride, pos_mod, pos_mul);
+      continue;
+    }
     const int nch = (int)((len + chunk - 1) / chunk);
     const int64_t n1 = n_pos * nch;
     hipLaunchKernelGGL(sat_chunk_scan_kernel, dim3((unsigned)((n1 + 255) / 256), ntab), dim3(256), 0, s, Sa,

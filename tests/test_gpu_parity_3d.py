@@ -49,6 +49,26 @@ def test_3d_parity(oracle, alg, shape, isz, sm, st, mode):
         assert rep.ok(), rep
 
 
+@pytest.mark.parametrize("alg,shape", [
+    ("zncc", (256, 16, 128)),    # table pair, dim 1: 257 x 129 columns (one sweep), dim 2 chunked
+    ("zncc", (256, 128, 12)),    # table pair, dim 2: 257 x 129 columns (one sweep), dim 1 chunked
+    ("nsqecc", (512, 8, 128)),   # single table, dim 1: 513 x 129 columns (one sweep)
+])
+def test_3d_parity_with_column_sweep_integral_arrays(oracle, alg, shape):
+    """Volumes whose integral arrays take the one-sweep column scan (at least
+    65536 columns over the tables built together) on one dim and the chunked
+    scan on the other."""
+    a, b = _vol(shape, density=0.02)
+    p = setPIVParameters(corr_alg=alg, interSize=8, searchMargin=2, step=8, computeSN=True)
+    print(f"case 3d sweep alg={alg} shape={shape}", flush=True)
+    VF, SN, PK = PIV(a, b, p, return_peaks=True, mode="fp64")
+    op = oracle.make_params(alg, 3, 8, 2, 8, True, np.float32)
+    VFo, SNo, aux = oracle.piv(a, b, op, mode=oracle.DIRECT, nthreads=8, want_aux=True)
+    rep = compare(VF, SN, PK, VFo, SNo, aux)
+    print(rep, flush=True)
+    assert rep.ok(), rep
+
+
 def test_3d_shear_recovers_displacement(oracle):
     """Linear shear u = 0.01 z + drift on a 64^3 volume: interior vectors follow it."""
     shape = (64, 64, 64)
