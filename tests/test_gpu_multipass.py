@@ -56,6 +56,33 @@ def test_2d_multipass_matches_oracle(oracle, alg, m):
     _check(VF, SN, VFo, SNo)
 
 
+def test_coarse_pass_without_ias_leaves_the_next_pass_unshifted(oracle):
+    """40x44 image, interSize 16, three passes: pass 3 (interSize 48) has no IA,
+    pass 2 (interSize 32, one IA) runs unshifted, pass 1 is shifted by pass 2."""
+    a, b = _pair2((40, 44), seed=12, peak=3.0)
+    p = setPIVParameters(corr_alg="nsqecc", interSize=16, searchMargin=4, step=8, computeSN=True, multipass=3)
+    VF, SN = PIV(a, b, p, mode="fp64")
+    op = oracle.make_params("nsqecc", 2, 16, 4, 8, True, a.dtype)
+    VFo, SNo = oracle.piv_multipass(a, b, op, 3, nthreads=8)
+    _check(VF, SN, VFo, SNo)
+
+
+@pytest.mark.parametrize("alg", ["nsqecc", "zncc"])
+def test_threshold_skips_inside_every_pass(oracle, alg):
+    """A threshold that skips some IAs of each pass: skipped coarse IAs
+    contribute zero vectors to the next pass's shifts, skipped final IAs stay 0."""
+    a, b = _pair2((160, 152), seed=17)
+    a = a.copy()
+    a[:70, :60] *= 0.2  # a dim corner: its IAs fall below the threshold
+    p = setPIVParameters(corr_alg=alg, interSize=16, searchMargin=4, step=8, computeSN=True, multipass=2,
+                         threshold=0.5)
+    VF, SN = PIV(a, b, p, mode="fp64")
+    op = oracle.make_params(alg, 2, 16, 4, 8, True, a.dtype, threshold=0.5)
+    VFo, SNo = oracle.piv_multipass(a, b, op, 2, nthreads=8)
+    assert (VFo[:, 0, 0] == 0).all()  # the corner IA is skipped
+    _check(VF, SN, VFo, SNo)
+
+
 @pytest.mark.parametrize("alg", ["fft", "zncc"])
 def test_3d_multipass_matches_oracle(oracle, alg):
     shape = (40, 36, 32)
