@@ -529,26 +529,60 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
   const int dt = J.in_dtype;
   const int64_t d0 = J.dims[0];
 
-  // Lane (line = k0, q) covers k1 = LPL*k + q, k < H.  Loads use 32-bit offsets
-  // from per-lane base pointers (the host checks dims[0] * L < 2^31).
+  // Lane (line = k0, q) covers k1 = LPL*k + q, k < H.  Every lane issues all its
+  // loads before using any (one wait for the batch), and every load reads an
+  // in-image pixel; f_at / g_at discard the values outside the IA / the clipped
+  // search region.  Two address forms (the host keeps (dims[0] + 1) * 1024 <= 2^32,
+  // so both fit 32-bit offsets):
+  // * FP64 transforms: a block-uniform base (the IA's first pixel; the clipped
+  //   search region's first pixel) plus a per-lane byte offset; rows and columns
+  //   outside the window are clamped to its nearest row / column, and column
+  //   k1 = LPL*k + h lies k * LPL * d0 elements after column h (an add and a
+  //   clamp per load instead of a multiply, selects and a 64-bit add);
+  // * FP32 transforms: a per-lane base plus d0 * k1, with positions outside the
+  //   window sent to one pixel (the IA origin, or the search region's first row /
+  //   column), which measured faster there than the clamped form.
+  constexpr bool kClampedGather = std::is_same<T, double>::value;
   const int k0 = line;
   const bool f_row = k0 < is0;
   const bool g_row = k0 >= g0.goff && k0 < g0.goff + g0.gsz;
-  const int64_t f_base = (g0.tlf + k0) + d0 * g1.tlf;
-  const int64_t g_base = (g0.sr + (k0 - g0.goff)) + d0 * (g1.sr - g1.goff);
-  const uint32_t ud0 = (uint32_t)d0;
   auto k1_of = [&](int k) { return LPL * k + h; };
   auto f_at = [&](int k1) -> bool { return f_row && k1 < is1; };
   auto g_at = [&](int k1) -> bool { return g_row && k1 >= g1.goff && k1 < g1.goff + g1.gsz; };
-  // Every lane issues all its loads before using any (one wait for the batch):
-  // positions outside the IA / clipped search region read an in-image pixel
-  // instead (the IA origin, or the search region's first row / column) and
-  // their values are discarded.  g_base addresses column k1 of the padded
-  // buffer, whose image columns start at k1 = goff1.
-  const int64_t f_lane = f_row ? f_base : (g0.tlf + d0 * g1.tlf);
-  const int64_t g_lane = g_row ? g_base : (g0.sr + d0 * (g1.sr - g1.goff));
+  const int64_t f_org = g0.tlf + d0 * g1.tlf;
+  const int64_t g_org = g0.sr + d0 * g1.sr;
+  const int32_t id0 = (int32_t)d0;
+  const int32_t f_r = k0 < is0 - 1 ? k0 : is0 - 1;
+  const int32_t g_r = min(max(k0 - g0.goff, 0), g0.gsz - 1);
+  const int32_t f_c0 = id0 * h, f_cmax = id0 * (is1 - 1);
+  const int32_t g_c0 = id0 * (h - g1.goff), g_cmax = id0 * (g1.gsz - 1);
+  const int32_t c_step = id0 * LPL;
+  auto f_el = [&](int k) -> uint32_t {
+    const int32_t c = f_c0 + k * c_step;
+    return (uint32_t)(f_r + (c < f_cmax ? c : f_cmax));
+  };
+  auto g_el = [&](int k) -> uint32_t { return (uint32_t)(g_r + min(max(g_c0 + k * c_step, 0), g_cmax)); };
+  auto at_byte = [](const void* base, int64_t org, uint32_t el, auto tin) {
+    using TIN = decltype(tin);
+    const gptr<char> b = as_global<char>(base) + org * (int64_t)sizeof(TIN);
+    return *(gptr<TIN>)(b + el * (uint32_t)sizeof(TIN));
+  };
+  const uint32_t ud0 = (uint32_t)d0;
+  const int64_t f_lane = f_row ? (g0.tlf + k0) + d0 * g1.tlf : f_org;
+  const int64_t g_lane = g_row ? (g0.sr + (k0 - g0.goff)) + d0 * (g1.sr - g1.goff) : g0.sr + d0 * (g1.sr - g1.goff);
   auto f_off = [&](int k1) -> uint32_t { return f_at(k1) ? ud0 * (uint32_t)k1 : 0u; };
   auto g_off = [&](int k1) -> uint32_t { return ud0 * (uint32_t)(g_at(k1) ? k1 : g1.goff); };
+  // F (or mask) and G sample of column k1_of(k) for this lane
+  auto ld_f = [&](const void* base, int k, auto tin) {
+    using TIN = decltype(tin);
+    if constexpr (kClampedGather) return at_byte(base, f_org, f_el(k), tin);
+    else return (as_global<TIN>(base) + f_lane)[f_off(k1_of(k))];
+  };
+  auto ld_g = [&](const void* base, int k, auto tin) {
+    using TIN = decltype(tin);
+    if constexpr (kClampedGather) return at_byte(base, g_org, g_el(k), tin);
+    else return (as_global<TIN>(base) + g_lane)[g_off(k1_of(k))];
+  };
 
   // ZNCC means over the IA and the clipped search region (zncc_ops.jl:10,22),
   // computed in the input element type (quirk Q12) from the loaded window
@@ -562,19 +596,14 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
   auto load = [&](auto tin) {
     using TIN = decltype(tin);
     TIN fv[H], gv[H], mv[H];
-    {
-      const gptr<TIN> pf = as_global<TIN>(J.img1) + f_lane;
-      const gptr<TIN> pg = as_global<TIN>(J.img2) + g_lane;
 #pragma unroll
-      for (int k = 0; k < H; ++k) {
-        fv[k] = pf[f_off(k1_of(k))];
-        gv[k] = pg[g_off(k1_of(k))];
-      }
+    for (int k = 0; k < H; ++k) {
+      fv[k] = ld_f(J.img1, k, tin);
+      gv[k] = ld_g(J.img2, k, tin);
     }
     if constexpr (ALG == ALG_MASK_NSQECC) {
-      const gptr<TIN> pm = as_global<TIN>(J.mask) + f_lane;
 #pragma unroll
-      for (int k = 0; k < H; ++k) mv[k] = pm[f_off(k1_of(k))];
+      for (int k = 0; k < H; ++k) mv[k] = ld_f(J.mask, k, tin);
     }
     if constexpr (ALG == ALG_ZNCC) {
       // sums of F, G and F^2 in one reduction; sum((F - mean)^2) follows from
@@ -653,13 +682,11 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
     // second correlation: mask (x) G^2 (mask_nsqecc.jl:155)
     auto load2 = [&](auto tin) {
       using TIN = decltype(tin);
-      const gptr<TIN> pm = as_global<TIN>(J.mask) + f_lane;
-      const gptr<TIN> pg = as_global<TIN>(J.img2) + g_lane;
       TIN mv[H], gv[H];
 #pragma unroll
       for (int k = 0; k < H; ++k) {
-        mv[k] = pm[f_off(k1_of(k))];
-        gv[k] = pg[g_off(k1_of(k))];
+        mv[k] = ld_f(J.mask, k, tin);
+        gv[k] = ld_g(J.img2, k, tin);
       }
 #pragma unroll
       for (int k = 0; k < H; ++k) {
