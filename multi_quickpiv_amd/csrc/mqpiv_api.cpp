@@ -257,9 +257,10 @@ static int choose_2d(mqpiv_ctx* c, const Job& j, int* fp64_out, int* L_out) {
   const int ss0 = j.is[0] + 2 * j.sm[0], ss1 = j.is[1] + 2 * j.sm[1];
   const int ss = std::max(ss0, ss1);
   const int ns0 = 2 * j.sm[0] + 1, ns1 = 2 * j.sm[1] + 1;
-  // the 2-D gather addresses a window with 32-bit byte offsets:
-  // (L - 1) * (dims[0] + 1) elements of at most 8 bytes, L <= 128
-  if (((int64_t)j.dims[0] + 1) * 1024 > ((int64_t)1 << 32))
+  // the 2-D kernels address a window (image, or FP64 integral array with rows
+  // of dims[0] + 1) with 32-bit byte offsets: at most 128 rows of (dims[0] + 1)
+  // entries of at most 8 bytes, plus one row
+  if (((int64_t)j.dims[0] + 2) * 1024 >= ((int64_t)1 << 32))
     return fail(c, MQPIV_ERR_UNSUPPORTED, "dims[0] = %lld is too large for 32-bit in-window offsets",
                 (long long)j.dims[0]);
   auto fits = [&](int fp64) {

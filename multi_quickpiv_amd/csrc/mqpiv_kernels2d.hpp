@@ -532,7 +532,7 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
   // Lane (line = k0, q) covers k1 = LPL*k + q, k < H.  Every lane issues all its
   // loads before using any (one wait for the batch), and every load reads an
   // in-image pixel; f_at / g_at discard the values outside the IA / the clipped
-  // search region.  Two address forms (the host keeps (dims[0] + 1) * 1024 <= 2^32,
+  // search region.  Two address forms (the host keeps (dims[0] + 2) * 1024 < 2^32,
   // so both fit 32-bit offsets):
   // * FP64 transforms: a block-uniform base (the IA's first pixel; the clipped
   //   search region's first pixel) plus a per-lane byte offset; rows and columns
@@ -719,8 +719,19 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
   const int nE = ns0 * ns1;
   Best<T> bst{T(0), INT32_MAX};
   double mx = -INFINITY, smv = 0.0;
-  auto visit = [&](int e, T v) {
-    const int p0 = e % ns0, p1 = e / ns0;
+  // window position (p0, p1) of this lane's elements e = t + m * TPI, m = 0, 1,
+  // ...: one division here, then a step of (TPI % ns0, TPI / ns0) with a carry
+  const int st0 = C::TPI % ns0, st1 = C::TPI / ns0;
+  int w0 = t % ns0, w1 = t / ns0;
+  auto step = [&]() {
+    w0 += st0;
+    w1 += st1;
+    if (w0 >= ns0) {
+      w0 -= ns0;
+      ++w1;
+    }
+  };
+  auto visit = [&](int e, int p0, int p1, T v) {
     if (p0 >= lo0 && p0 <= hi0 && p1 >= lo1 && p1 <= hi1 && v > bst.v) bst = Best<T>{v, e};
     if (want_sn && p0 >= lo0 && p0 <= shi0 && p1 >= lo1 && p1 <= shi1) {
       const double dv = (double)v;
@@ -736,41 +747,51 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
     // before using any.  Corner indices are clamped so that an empty window
     // (no image data) reads four equal corners and sums to exactly 0, and
     // elements past the end read element 0's corners; no load is predicated.
+    // Corners are addressed as 32-bit byte offsets from the table entry at the
+    // clipped search region's first pixel (the host keeps (dims[0] + 2) * 1024
+    // < 2^32, and a window spans at most 128 table rows).
     const int64_t D0 = d0 + 1;
     const double sqF = sqrt(sumF2);
     const double mu = (dt == DT_F32) ? (double)meanG32 : meanG64;
-    const gptr<double> sat2 = as_global<double>(J.sat2);
-    const gptr<double> sat1 = as_global<double>(J.sat1);
+    const int64_t corner0 = (g0.sr + D0 * g1.sr) * (int64_t)sizeof(double);
+    const gptr<char> sat2 = as_global<char>(J.sat2) + corner0;
+    const gptr<char> sat1 = as_global<char>(J.sat1) + corner0;
+    const uint32_t rowb = (uint32_t)D0 * (uint32_t)sizeof(double);
+    auto at = [](gptr<char> b, uint32_t off) { return *(gptr<double>)(b + off); };
     constexpr int NB = C::TPI >= 384 ? 2 : 4;  // wide IAs: fewer elements per lane, fewer live registers
     constexpr int NQ = (ALG == ALG_ZNCC) ? 2 : 1;
     for (int e0 = t; e0 < nE; e0 += NB * C::TPI) {
       double q[NB][NQ][4];
       double nbox[NB];
+      int pw0[NB], pw1[NB];
 #pragma unroll
       for (int j = 0; j < NB; ++j) {
         const int e = e0 + j * C::TPI;
-        const int ec = e < nE ? e : 0;
-        const int s0 = ec % ns0, s1 = ec / ns0;
+        pw0[j] = w0;
+        pw1[j] = w1;
+        step();
+        const int s0 = e < nE ? pw0[j] : 0, s1 = e < nE ? pw1[j] : 0;
         const int lo0w = s0 > g0.goff ? s0 : g0.goff;
         int hi0w = (s0 + is0) < (g0.goff + g0.gsz) ? (s0 + is0) : (g0.goff + g0.gsz);
         hi0w = hi0w > lo0w ? hi0w : lo0w;
         const int lo1w = s1 > g1.goff ? s1 : g1.goff;
         int hi1w = (s1 + is1) < (g1.goff + g1.gsz) ? (s1 + is1) : (g1.goff + g1.gsz);
         hi1w = hi1w > lo1w ? hi1w : lo1w;
-        const int64_t a0 = g0.sr + (lo0w - g0.goff), b0 = g0.sr + (hi0w - g0.goff);
-        const int64_t a1 = g1.sr + (lo1w - g1.goff), b1 = g1.sr + (hi1w - g1.goff);
-        const int64_t c11 = b0 + D0 * b1, c01 = a0 + D0 * b1, c10 = b0 + D0 * a1, c00 = a0 + D0 * a1;
-        q[j][0][0] = sat2[c11];
-        q[j][0][1] = sat2[c01];
-        q[j][0][2] = sat2[c10];
-        q[j][0][3] = sat2[c00];
+        const uint32_t a0 = (uint32_t)(lo0w - g0.goff) * (uint32_t)sizeof(double);
+        const uint32_t b0 = (uint32_t)(hi0w - g0.goff) * (uint32_t)sizeof(double);
+        const uint32_t a1 = (uint32_t)(lo1w - g1.goff) * rowb, b1 = (uint32_t)(hi1w - g1.goff) * rowb;
+        const uint32_t c11 = b0 + b1, c01 = a0 + b1, c10 = b0 + a1, c00 = a0 + a1;
+        q[j][0][0] = at(sat2, c11);
+        q[j][0][1] = at(sat2, c01);
+        q[j][0][2] = at(sat2, c10);
+        q[j][0][3] = at(sat2, c00);
         if constexpr (ALG == ALG_ZNCC) {
-          q[j][NQ - 1][0] = sat1[c11];
-          q[j][NQ - 1][1] = sat1[c01];
-          q[j][NQ - 1][2] = sat1[c10];
-          q[j][NQ - 1][3] = sat1[c00];
+          q[j][NQ - 1][0] = at(sat1, c11);
+          q[j][NQ - 1][1] = at(sat1, c01);
+          q[j][NQ - 1][2] = at(sat1, c10);
+          q[j][NQ - 1][3] = at(sat1, c00);
         }
-        nbox[j] = (double)((b0 - a0) * (b1 - a1));
+        nbox[j] = (double)((hi0w - lo0w) * (hi1w - lo1w));
       }
 #pragma unroll
       for (int j = 0; j < NB; ++j) {
@@ -792,7 +813,7 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
           r = den / (num + den);                     // nsqecc_ops.jl:72-73
         }
         S[e] = (T)r;
-        visit(e, (T)r);
+        visit(e, pw0[j], pw1[j], (T)r);
       }
     }
   } else if constexpr (ALG == ALG_MASK_NSQECC) {
@@ -803,10 +824,14 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
       const double num = (-2.0 * c1 + sumF2) + c2;  // masked_nsqecc_ops.jl:130-132
       const double den = sqrt(fabs(c2)) * sqF;      // masked_nsqecc_ops.jl:135-136
       S[i] = (T)(den / (num + den));
-      visit(i, (T)(den / (num + den)));
+      visit(i, w0, w1, (T)(den / (num + den)));
+      step();
     }
   } else {
-    for (int i = t; i < nE; i += C::TPI) visit(i, S[i]);
+    for (int i = t; i < nE; i += C::TPI) {
+      visit(i, w0, w1, S[i]);
+      step();
+    }
   }
   // combined reduction: (peak value, index), SN maximum, SN sum; its barrier
   // also makes the normalised window visible to the fit below
