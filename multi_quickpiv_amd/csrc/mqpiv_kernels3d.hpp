@@ -76,6 +76,37 @@ __device__ __forceinline__ Geo3 geo3_of(const Job& J, int64_t ia) {
   return r;
 }
 
+// Column-major position (a, b, c) in an n0 x n1 x (any) box of the elements
+// e = first + m * stride, m = 0, 1, ...: the divisions once, then a step with
+// carries.
+struct BoxWalk {
+  int a, b, c, sa, sb, sc, n0, n1;
+  __device__ __forceinline__ BoxWalk(int first, int stride, int n0_, int n1_)
+      : n0(n0_ > 0 ? n0_ : 1), n1(n1_ > 0 ? n1_ : 1) {
+    a = first % n0;
+    const int r = first / n0;
+    b = r % n1;
+    c = r / n1;
+    sa = stride % n0;
+    const int q = stride / n0;
+    sb = q % n1;
+    sc = q / n1;
+  }
+  __device__ __forceinline__ void step() {
+    a += sa;
+    b += sb;
+    c += sc;
+    if (a >= n0) {
+      a -= n0;
+      ++b;
+    }
+    if (b >= n1) {
+      b -= n1;
+      ++c;
+    }
+  }
+};
+
 // Box sum of a 3-D FP64 integral array over [a, b) per dim.
 __device__ __forceinline__ double sat_box3(const double* __restrict__ S, int64_t D0, int64_t D01, int64_t a0,
                                            int64_t b0, int64_t a1, int64_t b1, int64_t a2, int64_t b2) {
@@ -377,14 +408,16 @@ __global__ __launch_bounds__(256) void finalize3d_kernel(Job J, Scratch3 sc, int
     constexpr int NB = 2;
     const gptr<double> sat2 = as_global<double>(J.sat2);
     const gptr<double> sat1 = as_global<double>(J.sat1);
+    BoxWalk wk(t, NT, ns0, ns1);
     for (int e0 = t; e0 < nS; e0 += NB * NT) {
       double q2[NB][8], q1[NB][8], nbox[NB];
       bool empty[NB];
 #pragma unroll
       for (int j = 0; j < NB; ++j) {
         const int e = e0 + j * NT;
-        const int ec = e < nS ? e : 0;
-        const int s[3] = {ec % ns0, (ec / ns0) % ns1, ec / (ns0 * ns1)};
+        const bool in = e < nS;
+        const int s[3] = {in ? wk.a : 0, in ? wk.b : 0, in ? wk.c : 0};
+        wk.step();
         const int isz[3] = {is0, is1, is2};
         int64_t a[3], bb[3];
         empty[j] = false;
@@ -448,8 +481,9 @@ __global__ __launch_bounds__(256) void finalize3d_kernel(Job J, Scratch3 sc, int
   const int w0 = hi0 - lo0 + 1, w1 = hi1 - lo1 + 1, w2 = hi2 - lo2 + 1;
   const int nwin = (w0 > 0 && w1 > 0 && w2 > 0) ? w0 * w1 * w2 : 0;
   Best<T> bst{T(0), INT32_MAX};
-  for (int e = t; e < nwin; e += NT) {
-    const int p0 = lo0 + e % w0, p1 = lo1 + (e / w0) % w1, p2 = lo2 + e / (w0 * w1);
+  BoxWalk pw(t, NT, w0, w1);
+  for (int e = t; e < nwin; e += NT, pw.step()) {
+    const int p0 = lo0 + pw.a, p1 = lo1 + pw.b, p2 = lo2 + pw.c;
     const int li = p0 + ns0 * (p1 + ns1 * p2);
     const T v = S[li];
     if (v > bst.v) bst = Best<T>{v, li};
@@ -477,8 +511,9 @@ __global__ __launch_bounds__(256) void finalize3d_kernel(Job J, Scratch3 sc, int
     const int u2 = (2 * sm2 - 1 - g2.off_hi) - lo2 + 1;
     const int nsn = (u0 > 0 && u1 > 0 && u2 > 0) ? u0 * u1 * u2 : 0;
     double mx = -INFINITY, sm = 0.0;
-    for (int e = t; e < nsn; e += NT) {
-      const int p0 = lo0 + e % u0, p1 = lo1 + (e / u0) % u1, p2 = lo2 + e / (u0 * u1);
+    BoxWalk sw(t, NT, u0, u1);
+    for (int e = t; e < nsn; e += NT, sw.step()) {
+      const int p0 = lo0 + sw.a, p1 = lo1 + sw.b, p2 = lo2 + sw.c;
       const double v = (double)S[p0 + ns0 * (p1 + ns1 * p2)];
       mx = (v > mx || v != v) ? v : mx;
       sm += v;
