@@ -44,12 +44,16 @@ struct mqpiv_ctx {
 };
 
 // Brackets one correlation-kernel launch with events when profiling is on.
+// The events only time the launch (mqpiv_profile_end waits on them before
+// reading the interval), so they skip the system-scope fence: no cache
+// writeback and invalidation between the timed kernels.
 struct KernelTimer {
   mqpiv_ctx* c;
   hipStream_t s;
   hipEvent_t a = nullptr, b = nullptr;
   KernelTimer(mqpiv_ctx* c_, hipStream_t s_) : c(c_), s(s_) {
-    if (c->profiling && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess)
+    if (c->profiling && hipEventCreateWithFlags(&a, hipEventDisableSystemFence) == hipSuccess &&
+        hipEventCreateWithFlags(&b, hipEventDisableSystemFence) == hipSuccess)
       hipEventRecord(a, s);
   }
   ~KernelTimer() {
