@@ -344,6 +344,13 @@ __host__ __device__ constexpr int c2r_lanes() {
   constexpr int P = 2 * LPL;
   return (P <= 16 && P * ((L / 2 + 2) / 2) <= TPI && has_plan(L / P)) ? P : LPL;
 }
+// A wider group for the same pass, used when an IA's line pairs fit the block
+// (checked at run time); c2r_lanes itself when there is none.
+template <int L, int LPL, int TPI>
+__host__ __device__ constexpr int c2r_wide_lanes() {
+  constexpr int P = 2 * c2r_lanes<L, LPL, TPI>();
+  return (P <= 16 && P <= TPI && has_plan(L / P)) ? P : c2r_lanes<L, LPL, TPI>();
+}
 
 template <int L, int LPL, typename T>
 __device__ __forceinline__ void correlate_2d(cpx<T> (&x)[L / LPL], cpx<T>* __restrict__ Z, int t, int ns0,
@@ -404,44 +411,58 @@ __device__ __forceinline__ void correlate_2d(cpx<T> (&x)[L / LPL], cpx<T>* __res
   __syncthreads();
   // --- inverse along dim 2, rows s0 = 2r and 2r + 1 packed as Ya + i Yb:
   // line = r, lane q2 holds k1 = cM2 + k of the packed spectrum (P2 lanes a line)
-  constexpr int P2 = c2r_lanes<L, LPL, Cfg2<L, T, LPL>::TPI>();
-  constexpr int M2 = L / P2;
-  const int line2 = t / P2, q2 = t % P2;
-  const int cM2 = (P2 == 1 ? 0 : chunk_of<P2>(q2)) * M2;
   const int npair = (ns0 + 1) / 2;
-  const bool act = line2 < npair;
-  cpx<T> y[M2];
-  if (act) {
-    const int sa = 2 * line2, sb = 2 * line2 + 1;
-    const bool has_b = sb < ns0;
+  auto c2r = [&](auto p2c) {
+    constexpr int P2 = decltype(p2c)::value;
+    constexpr int M2 = L / P2;
+    const int line2 = t / P2, q2 = t % P2;
+    const int cM2 = (P2 == 1 ? 0 : chunk_of<P2>(q2)) * M2;
+    const bool act = line2 < npair;
+    cpx<T> y[M2];
+    if (act) {
+      const int sa = 2 * line2, sb = 2 * line2 + 1;
+      const bool has_b = sb < ns0;
 #pragma unroll
-    for (int k = 0; k < M2; ++k) {
-      const int k1 = cM2 + k;
-      const bool mirror = k1 >= KH;
-      const int kc = mirror ? L - k1 : k1;
-      const cpx<T> ya = Z[kc * LD + sa];
-      const cpx<T> yb = has_b ? Z[kc * LD + sb] : cpx<T>{T(0), T(0)};
-      // Y(s, k1) = conj(Y(s, L - k1)) above L/2
-      const T ayi = mirror ? -ya.y : ya.y, byi = mirror ? -yb.y : yb.y;
-      y[k] = cpx<T>{ya.x - byi, ayi + yb.x};
+      for (int k = 0; k < M2; ++k) {
+        const int k1 = cM2 + k;
+        const bool mirror = k1 >= KH;
+        const int kc = mirror ? L - k1 : k1;
+        const cpx<T> ya = Z[kc * LD + sa];
+        const cpx<T> yb = has_b ? Z[kc * LD + sb] : cpx<T>{T(0), T(0)};
+        // Y(s, k1) = conj(Y(s, L - k1)) above L/2
+        const T ayi = mirror ? -ya.y : ya.y, byi = mirror ? -yb.y : yb.y;
+        y[k] = cpx<T>{ya.x - byi, ayi + yb.x};
+      }
+      line_inv<L, P2, T>(y, q2);
     }
-    line_inv<L, P2, T>(y, q2);
-  }
-  __syncthreads();
-  if (act) {
-    T* S = reinterpret_cast<T*>(Z);
-    const T kScale = T(1) / T(L * L);
-    const int sa = 2 * line2, sb = 2 * line2 + 1;
+    __syncthreads();
+    if (act) {
+      T* S = reinterpret_cast<T*>(Z);
+      const T kScale = T(1) / T(L * L);
+      const int sa = 2 * line2, sb = 2 * line2 + 1;
 #pragma unroll
-    for (int m = 0; m < M2; ++m) {
-      const int s1 = P2 * m + q2;
-      if (s1 < ns1) {
-        S[s1 * ns0 + sa] = y[m].x * kScale;
-        if (sb < ns0) S[s1 * ns0 + sb] = y[m].y * kScale;
+      for (int m = 0; m < M2; ++m) {
+        const int s1 = P2 * m + q2;
+        if (s1 < ns1) {
+          S[s1 * ns0 + sa] = y[m].x * kScale;
+          if (sb < ns0) S[s1 * ns0 + sb] = y[m].y * kScale;
+        }
       }
     }
+    __syncthreads();
+  };
+  // P2 lanes per line fit every margin this length allows; when the block holds
+  // one IA (so the choice is block-uniform) and this IA's line pairs fit, twice
+  // as many lanes halve each lane's serial chain
+  constexpr int TPI = Cfg2<L, T, LPL>::TPI;
+  constexpr int P2 = c2r_lanes<L, LPL, TPI>();
+  constexpr int P2w = c2r_wide_lanes<L, LPL, TPI>();
+  if constexpr (P2w != P2 && Cfg2<L, T, LPL>::IPB == 1) {
+    if (npair * P2w <= TPI) c2r(std::integral_constant<int, P2w>{});
+    else c2r(std::integral_constant<int, P2>{});
+  } else {
+    c2r(std::integral_constant<int, P2>{});
   }
-  __syncthreads();
 }
 
 // Box sum of an FP64 integral array over image rows [a0, b0) x [a1, b1).

@@ -96,25 +96,30 @@ void both(const char* tn) {
 
 int main() {
   // every (length, lanes per line) the 2-D kernels instantiate, and the doubled
-  // groups of their last inverse pass
+  // and quadrupled groups of their last inverse pass
   both<16, 2, float>("f32");
   both<32, 2, float>("f32");
   both<32, 4, float>("f32");
+  both<32, 8, float>("f32");
   both<48, 4, float>("f32");
   both<48, 8, float>("f32");
   both<64, 4, float>("f32");
   both<64, 8, float>("f32");
+  both<64, 16, float>("f32");
   both<96, 8, float>("f32");
   both<96, 16, float>("f32");
   both<128, 4, float>("f32");
   both<128, 8, float>("f32");
+  both<128, 16, float>("f32");
   both<16, 2, double>("f64");
   both<32, 2, double>("f64");
   both<32, 4, double>("f64");
+  both<32, 8, double>("f64");
   both<48, 4, double>("f64");
   both<48, 8, double>("f64");
   both<64, 4, double>("f64");
   both<64, 8, double>("f64");
+  both<64, 16, double>("f64");
   both<96, 4, double>("f64");
   both<96, 8, double>("f64");
   both<96, 16, double>("f64");
