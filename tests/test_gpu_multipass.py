@@ -83,6 +83,54 @@ def test_threshold_skips_inside_every_pass(oracle, alg):
     _check(VF, SN, VFo, SNo)
 
 
+@pytest.mark.parametrize("alg", ["fft", "zncc", "nsqecc"])
+def test_anisotropic_windows_on_a_ragged_image(oracle, alg):
+    a, b = _pair2((203, 171), seed=44)
+    p = setPIVParameters(corr_alg=alg, interSize=(16, 12), searchMargin=(4, 6), step=(8, 6), computeSN=True,
+                         multipass=2)
+    VF, SN = PIV(a, b, p, mode="fp64")
+    op = oracle.make_params(alg, 2, (16, 12), (4, 6), (8, 6), True, a.dtype)
+    VFo, SNo = oracle.piv_multipass(a, b, op, 2, nthreads=8)
+    _check(VF, SN, VFo, SNo)
+
+
+def test_float64_inputs_and_float32_outputs(oracle):
+    a, b = _pair2((144, 160), seed=45)
+    a64, b64 = a.astype(np.float64), b.astype(np.float64)
+    p = setPIVParameters(corr_alg="nsqecc", interSize=16, searchMargin=4, step=8, computeSN=True, multipass=3)
+    VF, SN = PIV(a64, b64, p, mode="fp64", precision=np.float32)
+    assert VF.dtype == np.float32 and SN.dtype == np.float32
+    op = oracle.make_params("nsqecc", 2, 16, 4, 8, True, np.float64)
+    VFo, SNo = oracle.piv_multipass(a64, b64, op, 3, nthreads=8)
+    assert np.abs(VF - VFo).max() <= 1e-5  # FP32 rounding of the output
+    assert np.nanmax(np.abs(SN - SNo) / np.abs(SNo)) <= 1e-6
+
+
+def test_fp32_transforms_follow_the_oracle_at_the_fp32_bar(oracle):
+    """FP32 transforms (mode fp32): the integer shifts of each pass are rounded
+    FP32 results, so a coarse value within ~1e-5 of a .5 boundary may round the
+    other way; the final field agrees to the FP32 bar everywhere else."""
+    a, b = _pair2((200, 168), seed=46)
+    p = setPIVParameters(corr_alg="nsqecc", interSize=16, searchMargin=4, step=8, computeSN=True, multipass=2)
+    VF, SN = PIV(a, b, p, mode="fp32")
+    op = oracle.make_params("nsqecc", 2, 16, 4, 8, True, a.dtype)
+    VFo, SNo = oracle.piv_multipass(a, b, op, 2, nthreads=8)
+    err = np.abs(VF - VFo).max(axis=0)
+    assert (err <= 1e-4).mean() >= 0.99, (err > 1e-4).sum()
+
+
+def test_3d_three_passes_match_oracle(oracle):
+    shape = (52, 44, 40)
+    n = synth.n_particles(shape, 0.01)
+    a, b = synth.particle_pair(shape, n, 1.5, synth.uniform((4.6, -3.7, 2.4)), 47, dtype=torch.float32)
+    a, b = a.numpy(), b.numpy()
+    p = setPIVParameters(corr_alg="nsqecc", interSize=8, searchMargin=1, step=4, computeSN=True, multipass=3)
+    VF, SN = PIV(a, b, p, mode="fp64")
+    op = oracle.make_params("nsqecc", 3, 8, 1, 4, True, a.dtype)
+    VFo, SNo = oracle.piv_multipass(a, b, op, 3, nthreads=8)
+    _check(VF, SN, VFo, SNo)
+
+
 @pytest.mark.parametrize("alg", ["fft", "zncc"])
 def test_3d_multipass_matches_oracle(oracle, alg):
     shape = (40, 36, 32)
