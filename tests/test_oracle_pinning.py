@@ -1,8 +1,8 @@
 """Pins the Float64 oracle against what does not depend on it: integer closed
 forms for the geometry, scipy's direct correlation, exact invariants of the
 similarity measures, analytic surfaces for the peak/subpixel/SN step, and
-delta-particle images with a known integer shift.  (The reference ships no tests
-or fixtures; see DESIGN.md.)"""
+delta-particle images with a known integer shift.  (The reference ships no tests,
+fixtures or golden vectors to pin it against.)"""
 from __future__ import annotations
 
 import numpy as np
@@ -72,7 +72,7 @@ def _good_size_real_py(n: int) -> int:
 
 
 def test_good_size_real(oracle):
-    for n in (64, 96, 48, 32, 128):  # every BASELINE transform size maps to itself
+    for n in (64, 96, 48, 32, 128):  # the benchmark workloads' transform sizes map to themselves
         assert O.good_size_real(n) == n
     assert O.good_size_real(13) == 16  # not 15: the 5-power loop is cut short
     assert O.good_size_real(7) == 8
