@@ -246,7 +246,7 @@ struct Geo {
   int64_t tlf;     // IA start
   int64_t sr;      // clipped search-region start
   int32_t gsz;     // clipped search-region length
-  int32_t goff;    // placement of the search region in the padded buffer (quirk Q1)
+  int32_t goff;    // placement of the search region in the padded buffer (piv_ops.jl:72)
   int32_t off_hi;  // SR_BRB_off
 };
 
@@ -606,7 +606,7 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
   };
 
   // ZNCC means over the IA and the clipped search region (zncc_ops.jl:10,22),
-  // computed in the input element type (quirk Q12) from the loaded window
+  // computed in the input element type, as the reference does, from the loaded window
   float meanF32 = 0.f, meanG32 = 0.f;
   double meanF64 = 0.0, meanG64 = 0.0;
 

@@ -117,7 +117,8 @@ __device__ __forceinline__ double sat_box3(const double* __restrict__ S, int64_t
 }
 
 // ZNCC means of one IA from the integral arrays of both images (sum over the IA
-// and over the clipped search region), rounded like the reference (Q12).
+// and over the clipped search region), rounded in the input element type like
+// the reference (zncc_ops.jl:10, 22).
 struct Means3 {
   float f32, g32;
   double f64, g64;

@@ -13,7 +13,8 @@
  *   orc_good_size_real     src/utils/fft_ops.jl:164-198 (root, good_size_real)
  *   orc_vf_size            src/utils/piv_ops.jl:6-14    (get_vectorfield_size)
  *   orc_coords             src/utils/piv_ops.jl:38-85   (interrogation/search coords,
- *                          including the SR_TLF_off formula of :72, quirk Q1)
+ *                          including the SR_TLF_off formula of :72, which places
+ *                          a clipped search region 2 px early)
  *   load_windows           src/utils/piv_ops.jl:99-127, zncc_ops.jl:3-25,
  *                          masked_nsqecc_ops.jl:19-42   (padded copies)
  *   sat_build/sat_window   src/utils/common_ops.jl:150-180, 203-245
@@ -41,7 +42,7 @@
  *                          vector-field index space) at the IA centre, rounded to
  *                          nearest (ties to even), NaN -> 0, clamped so that the
  *                          IA moved by d stays in the image; the search region is
- *                          the one the reference builds for an IA at TLF + d (Q1
+ *                          the one the reference builds for an IA at TLF + d (:72
  *                          included), the IA stays at TLF, and the pass reports d
  *                          plus its displacement; SN from the last pass.
  *
@@ -151,7 +152,7 @@ void orc_coords_of(const orc_params* p, const int64_t* dims, const int64_t* vf,
     c->brb[d] = brb;
     c->sr_tlf[d] = (tlf - sm > 1) ? tlf - sm : 1;
     c->sr_brb[d] = (brb + sm < dims[d]) ? brb + sm : dims[d];
-    /* abs(min(0, TLF - (SM - 1)))  -- piv_ops.jl:72 (quirk Q1) */
+    /* abs(min(0, TLF - (SM - 1)))  -- piv_ops.jl:72 (2 px short of the true count) */
     int64_t a = tlf - (sm - 1);
     c->off_lo[d] = a < 0 ? -a : 0;
     /* abs(min(0, dim - (BRB + SM))) -- piv_ops.jl:73 */
@@ -275,7 +276,7 @@ static void load_windows(const orc_params* p, const int64_t* dims, const void* i
           sumF += img_at(img1, dt, lin);
         }
   }
-  /* zncc_ops.jl:10: the mean is computed in the input element type (Q12). */
+  /* zncc_ops.jl:10: the mean is computed in the input element type. */
   float meanF32 = (float)sumF / (float)w->nF;
   double meanF64 = sumF / (double)w->nF;
   for (int64_t z = 0; z < w->is[2]; ++z)

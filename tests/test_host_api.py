@@ -52,7 +52,7 @@ def test_parse_corr():
 
 
 def test_unpadded_good_pad_swap():
-    """The outer constructor passes good_pad into the unpadded slot (Q9)."""
+    """The outer constructor passes good_pad into the unpadded slot (piv_parameters.jl:58-59)."""
     p = setPIVParameters(unpadded=True, good_pad=False)
     assert p.unpadded is False and p.good_pad is True
 

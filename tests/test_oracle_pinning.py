@@ -134,7 +134,7 @@ def test_zncc_bounded_and_mask_of_ones_equals_nsqecc(oracle):
 
 
 def test_zncc_uses_search_region_mean(oracle):
-    """zncc_ops.jl:22-24 (Q6): G - mean(G over the clipped SR), zero padding kept.
+    """zncc_ops.jl:22-24: G - mean(G over the whole clipped SR), zero padding kept.
     Adding a constant to image 2 changes nothing only if the mean is SR-wide."""
     a, b = _rand_pair((48, 48), 4)
     p = O.make_params("zncc", 2, 12, 5, 12)
@@ -199,7 +199,7 @@ def test_peak_window_and_refinement_edges(oracle):
 
 def test_sn_window_upper_bound_is_2sm(oracle):
     """compute_SN reads [1+o-, 2SM-o+] (piv_ops.jl:352-364): one less than the
-    peak window on the upper side (Q5)."""
+    peak window on the upper side (piv_ops.jl:354)."""
     S = np.ones((5, 5))
     S[4, :] = 100.0  # row 2SM+1 = 5 is outside the SN window
     S[:, 4] = 100.0
@@ -222,7 +222,7 @@ def _delta_images(shape, d, seed):
 
 def test_integer_shift_expected_peaks_including_border_bias(oracle):
     """Interior IAs peak at SM+1+d; the first row/column (TLF = 1 <= SM-2) peak
-    at SM-1+d because the search region is placed 2 px early (Q1)."""
+    at SM-1+d because the search region is placed 2 px early (piv_ops.jl:72)."""
     d = (3, 2)
     a, b = _delta_images((96, 96), d, 5)
     p = O.make_params("nsqecc", 2, 16, 8, 16)
