@@ -127,8 +127,7 @@ int mqpiv_piv_device(mqpiv_ctx* ctx, const mqpiv_params* p, const int64_t* dims,
  * stays inside the image.  The shifted search region is the one the reference
  * builds for an IA at TLF + d (its SR_TLF_off of piv_ops.jl:72 included); the
  * IA stays at TLF, and the pass reports d + its measured displacement.  VF and
- * SN (computeSN: last pass
- * only) have the multipass = 1 shapes.  Every pass must fit a transform
+ * SN (computeSN: last pass only) have the multipass = 1 shapes.  Every pass must fit a transform
  * (2-D search size <= 128, 3-D <= 64; MQPIV_ERR_UNSUPPORTED otherwise).  The
  * threshold test runs on each pass's own IAs; there is no skip mask.
  * multipass = 1 equals mqpiv_piv. */
