@@ -13,12 +13,14 @@
 #include <stdint.h>
 
 #include "mqpiv_internal.h"
+#include "mqpiv_kernels2d.hpp"
 
 namespace mqpiv {
 
 // Dim-0 pass: one wave per output line (j1, j2); inclusive wave scans over 64
-// pixels with a carried total, four chunks per round with their loads issued
-// together.  Sa receives v^power; Sb (optional) receives v (power 1) from the
+// pixels (DPP row shifts and row broadcasts, the steps of the kernels' group
+// reductions, which leave every lane's prefix in place) with a carried total,
+// four chunks per round with their loads issued together.  Sa receives v^power; Sb (optional) receives v (power 1) from the
 // same pass over the image.  Lines with j1 = 0 or j2 = 0 and the j0 = 0
 // entries are the zero border.
 __global__ __launch_bounds__(64) void sat_dim0_kernel(const void* __restrict__ img, int dt, int power,
@@ -53,15 +55,10 @@ __global__ __launch_bounds__(64) void sat_dim0_kernel(const void* __restrict__ i
       const int64_t i0 = c + 64 * u + lane;
       double a = power == 2 ? v[u] * v[u] : v[u];
       double b = v[u];
-#pragma unroll
-      for (int m = 1; m < 64; m <<= 1) {
-        const double ua = __shfl_up(a, m);
-        if (lane >= m) a += ua;
-        if (outb) {
-          const double ub = __shfl_up(b, m);
-          if (lane >= m) b += ub;
-        }
-      }
+      Group<64>::steps([&](auto mv) {
+        a += mv(0.0, a);
+        if (outb) b += mv(0.0, b);
+      });
       a += ca;
       if (i0 < d0) outa[i0 + 1] = a;
       ca = __shfl(a, 63);
