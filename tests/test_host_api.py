@@ -209,3 +209,17 @@ def test_multipass_argument_checks_without_gpu():
     p.ndims = 2
     with pytest.raises(ValueError):
         to_c_params(p, multipass_ok=True)
+
+
+def test_batch_requests_reject_multipass_without_gpu():
+    """Batches, graphs and the server run single passes; a multipass request is
+    refused when it is built, before any device work."""
+    import torch
+
+    from multi_quickpiv_amd.piv import DeviceFrame, DeviceRequest
+
+    img = torch.rand(64, 64)
+    frame = DeviceFrame(img, img)
+    p = setPIVParameters(interSize=16, searchMargin=4, multipass=2)
+    with pytest.raises(NotImplementedError):
+        DeviceRequest.from_frame(frame, p)
