@@ -238,19 +238,34 @@ static void worker_loop(mqpiv_server* sv) {
       for (auto& sl : sv->slots) any |= sl.busy;
       if (!any) break;
     }
-    // admit pending requests into free slots (round-robin), in arrival order,
-    // while memory lasts
+    // admit pending requests into free slots (round-robin), oldest first, while
+    // memory lasts.  A batch holds requests of one dimensionality: a batch
+    // completes with its longest member, so a 2-D pair does not wait for a 3-D
+    // volume admitted beside it (the next free slot takes it instead).  At most
+    // one slot runs 3-D volumes, one volume at a time: two side by side would
+    // each take about twice as long, so a volume waits for the running one while
+    // the 2-D pairs behind it go ahead (in their own arrival order).
     const size_t ns = sv->slots.size();
+    bool vol_busy = false;
+    for (auto& s2 : sv->slots) vol_busy |= s2.busy && !s2.reqs.empty() && s2.reqs.front()->req.params.ndims == 3;
     for (size_t k = 0; k < ns; ++k) {
       Slot& sl = sv->slots[(sv->next_slot + k) % ns];
       if (sl.busy || sv->pending.empty()) continue;
-      while (!sv->pending.empty() && (int)sl.reqs.size() < sv->max_batch) {
-        Req* r = sv->pending.front();
+      int want = 0;  // dimensionality of this batch, from its first admissible request
+      for (auto it = sv->pending.begin(); it != sv->pending.end() && (int)sl.reqs.size() < sv->max_batch;) {
+        Req* r = *it;
+        const int nd = r->req.params.ndims;
+        if ((vol_busy && nd == 3) || (want && nd != want)) {
+          ++it;
+          continue;
+        }
         const size_t off = sv->pool.alloc(r->d_bytes);
         if (off == SIZE_MAX) break;  // wait for memory to free up
         r->d_off = off;
-        sv->pending.pop_front();
+        it = sv->pending.erase(it);
         sl.reqs.push_back(r);
+        want = nd;
+        if (nd == 3) break;  // a volume runs alone
       }
       if (sl.reqs.empty()) continue;
       // copies and launches run without the lock so that submitters are not held up
