@@ -179,13 +179,15 @@ int mqpiv_profile_end(mqpiv_ctx* ctx, double* kernel_ms, int32_t cap, int32_t* n
 
 /* Continuous-batching request server (no reference counterpart: the reference
  * runs one PIV call at a time on the caller's thread).  A worker thread admits
- * submitted pairs, in arrival order, into one of n_slots execution slots as soon
+ * submitted pairs, oldest first, into one of n_slots execution slots as soon
  * as a slot is free and the pair's device footprint fits in the
- * hbm_budget_bytes pool (up to max_batch pairs per slot).  The footprint is the
- * pair's FP64 integral arrays plus, for host requests, its staged inputs and
- * outputs, each rounded to 256 B; every slot also keeps its own context (3-D
- * transform scratch, at most MQPIV_3D_CHUNK_BYTES, 256 MiB by default) outside
- * the pool.  Each slot copies its host pairs in, runs all its pairs as one
+ * hbm_budget_bytes pool (up to max_batch pairs per slot).  A batch holds pairs
+ * of one dimensionality; a 3-D volume runs alone and one slot at most runs a
+ * volume, so 2-D pairs may complete before a volume submitted ahead of them.
+ * The footprint is the pair's FP64 integral arrays plus, for host requests, its
+ * staged inputs and outputs, each rounded to 256 B; every slot also keeps its
+ * own context (3-D transform scratch, at most MQPIV_3D_CHUNK_BYTES, 2 GiB by
+ * default) outside the pool.  Each slot copies its host pairs in, runs all its pairs as one
  * batched call (as mqpiv_piv_batch_device) on its own stream, and copies
  * VF/SN/peaks back, so batches of different slots overlap on the device.
  *   mqpiv_server_submit: HOST pointers, valid until mqpiv_server_wait reports
