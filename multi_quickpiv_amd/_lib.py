@@ -32,9 +32,10 @@ MQPIV_REPLACE_ZERO = 0
 MQPIV_REPLACE_MEDIAN = 1
 
 MQPIV_PLAIN_FP32 = 0
+MQPIV_SPLIT_MFMA = 1
 MQPIV_FP64 = 2
 MQPIV_AUTO = 3
-PRECISION_MODES = {"fp32": MQPIV_PLAIN_FP32, "fp64": MQPIV_FP64, "auto": MQPIV_AUTO}
+PRECISION_MODES = {"fp32": MQPIV_PLAIN_FP32, "mfma": MQPIV_SPLIT_MFMA, "fp64": MQPIV_FP64, "auto": MQPIV_AUTO}
 
 # Every symbol include/mqpiv.h declares.
 EXPORTS = (

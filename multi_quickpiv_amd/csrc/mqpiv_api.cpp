@@ -23,6 +23,8 @@
 #include "mqpiv_ctx.h"
 #include "mqpiv_internal.h"
 
+static_assert(MQPIV_SPLIT_MFMA == mqpiv::kPrecSplitMfma, "precision mode mirror");
+
 namespace mqpiv {
 int fft_len_2d(int fp64, int ss);
 size_t lds_2d(int fp64, int L, int alg, int ns0, int ns1, int ss0);
@@ -120,9 +122,11 @@ static int check_params(mqpiv_ctx* c, const mqpiv_params* p, const int64_t* dims
   if (p->ndims != 2 && p->ndims != 3) return fail(c, MQPIV_ERR_BAD_PARAMS, "ndims must be 2 or 3 (got %d)", p->ndims);
   if (p->corr_alg < MQPIV_FFTCC || p->corr_alg > MQPIV_MASK_NSQECC)
     return fail(c, MQPIV_ERR_BAD_PARAMS, "unknown corr_alg %d", p->corr_alg);
-  if (p->precision_mode != MQPIV_PLAIN_FP32 && p->precision_mode != MQPIV_FP64 &&
-      p->precision_mode != MQPIV_AUTO)
+  if (p->precision_mode != MQPIV_PLAIN_FP32 && p->precision_mode != MQPIV_SPLIT_MFMA &&
+      p->precision_mode != MQPIV_FP64 && p->precision_mode != MQPIV_AUTO)
     return fail(c, MQPIV_ERR_UNSUPPORTED, "precision_mode %d is not built", p->precision_mode);
+  if (p->precision_mode == MQPIV_SPLIT_MFMA && (p->ndims != 2 || p->corr_alg == MQPIV_MASK_NSQECC))
+    return fail(c, MQPIV_ERR_UNSUPPORTED, "the split-precision matrix-core form runs 2-D FFTCC, ZNCC and NSQECC");
   for (int d = 0; d < p->ndims; ++d) {
     if (dims[d] < 1) return fail(c, MQPIV_ERR_BAD_PARAMS, "dims[%d] = %lld", d, (long long)dims[d]);
     if (p->inter_size[d] < 1) return fail(c, MQPIV_ERR_BAD_PARAMS, "interSize[%d] < 1", d);
@@ -268,6 +272,14 @@ static int choose_2d(mqpiv_ctx* c, const Job& j, int* fp64_out, int* L_out) {
     return L > 0 && lds_2d(fp64, L, j.alg, ns0, ns1, ss0) <= 160 * 1024;
   };
   int fp64 = 0;
+  if (j.precision_mode == MQPIV_SPLIT_MFMA) {
+    if (ss > 64)
+      return fail(c, MQPIV_ERR_UNSUPPORTED, "the split-precision matrix-core form takes 2-D search sizes up to 64 (got %d)",
+                  ss);
+    *fp64_out = 0;
+    *L_out = 64;
+    return MQPIV_OK;
+  }
   if (j.precision_mode == MQPIV_FP64) {
     if (!fits(1))
       return fail(c, MQPIV_ERR_UNSUPPORTED, "FP64 transforms support 2-D search sizes up to 96 (got %d)", ss);
@@ -470,8 +482,10 @@ static int run_multipass(mqpiv_ctx* c, const mqpiv_params* p, int32_t m, const J
   return MQPIV_OK;
 }
 
-static int check_multipass(mqpiv_ctx* c, int32_t m) {
+static int check_multipass(mqpiv_ctx* c, const mqpiv_params* p, int32_t m) {
   if (m < 1 || m > 64) return fail(c, MQPIV_ERR_BAD_PARAMS, "multipass = %d (1..64)", m);
+  if (m > 1 && p && p->precision_mode == MQPIV_SPLIT_MFMA)
+    return fail(c, MQPIV_ERR_UNSUPPORTED, "the split-precision matrix-core form runs single passes");
   return MQPIV_OK;
 }
 
@@ -479,7 +493,7 @@ extern "C" int mqpiv_piv_multipass(mqpiv_ctx* c, const mqpiv_params* p, int32_t 
                                    int32_t in_dtype, const void* img1, const void* img2, const void* mask,
                                    int32_t out_dtype, void* VF, void* SN) {
   if (!c) return MQPIV_ERR_NULL;
-  int rc = check_multipass(c, multipass);
+  int rc = check_multipass(c, p, multipass);
   if (rc) return rc;
   return run_staged(c, p, dims, in_dtype, img1, img2, mask, nullptr, out_dtype, VF, SN, nullptr,
                     [&](Job& j, hipStream_t s) { return run_multipass(c, p, multipass, j, s); });
@@ -491,7 +505,7 @@ extern "C" int mqpiv_piv_multipass_device(mqpiv_ctx* c, const mqpiv_params* p, i
                                           void* d_SN, void* stream) {
   if (!c) return MQPIV_ERR_NULL;
   std::lock_guard<std::mutex> lk(c->mu);
-  int rc = check_multipass(c, multipass);
+  int rc = check_multipass(c, p, multipass);
   if (rc) return rc;
   if (hipSetDevice(c->device) != hipSuccess) return fail(c, MQPIV_ERR_DEVICE, "hipSetDevice failed");
   Job j;
@@ -572,6 +586,8 @@ static int plan_batch(mqpiv_ctx* c, const mqpiv_request* r, int n, BatchPlan& P)
     int rc = build_job(c, &r[i].params, r[i].dims, r[i].in_dtype, r[i].img1, r[i].img2, r[i].mask, r[i].skip_mask,
                        r[i].out_dtype, r[i].VF, r[i].SN, &j);
     if (rc) return fail(c, rc, "request %d: %s", i, c->err.c_str());
+    if (j.precision_mode == MQPIV_SPLIT_MFMA)
+      return fail(c, MQPIV_ERR_UNSUPPORTED, "request %d: the split-precision matrix-core form runs single calls", i);
     j.peaks = r[i].peaks;
     if (j.n_ia == 0) continue;
     if (j.ndims == 3) {

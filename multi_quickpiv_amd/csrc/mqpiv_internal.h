@@ -10,6 +10,8 @@
 namespace mqpiv {
 
 enum Alg : int32_t { ALG_FFTCC = 0, ALG_ZNCC = 1, ALG_NSQECC = 2, ALG_MASK_NSQECC = 3 };
+// Job::precision_mode of the split-precision matrix-core form (MQPIV_SPLIT_MFMA)
+constexpr int32_t kPrecSplitMfma = 1;
 enum DType : int32_t { DT_F32 = 0, DT_F64 = 1 };
 
 // One frame pair (a "job") as the kernels see it.  Geometry follows the
@@ -38,7 +40,7 @@ struct Job {
   int32_t compute_sn;
   int32_t has_threshold;   // unmasked: threshold >= 0; masked: threshold > 0
   int32_t has_mask_threshold;
-  int32_t precision_mode;  // host-side launch choice (MQPIV_PLAIN_FP32 / FP64 / AUTO)
+  int32_t precision_mode;  // launch choice (MQPIV_PLAIN_FP32 / SPLIT_MFMA / FP64 / AUTO)
   double threshold;
   double mask_threshold;
 };

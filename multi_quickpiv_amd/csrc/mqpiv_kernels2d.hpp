@@ -216,6 +216,11 @@ struct OpMaxNaN {
   }
 };
 
+}  // namespace mqpiv
+#include "mqpiv_mfma2d.hpp"
+namespace mqpiv {
+
+
 // (value, linear index) argmax with the reference's tie-break: the first element
 // in column-major order whose value is strictly above everything before it,
 // starting from 0 (common_ops.jl:112-121).  i = INT32_MAX means "none".
@@ -492,7 +497,9 @@ constexpr int min_waves_2d() {
 // One IA per block-slot.  SH: search regions shifted per IA (Job::shift,
 // multipass); a separate instantiation, so that kernels without shifts carry
 // none of its registers.
-template <int L, int ALG, typename T, int LPL, bool SH>
+// MF: the split-precision matrix-core correlation (mqpiv_mfma2d.hpp) in place of
+// the register FFTs; length 64, FP32, four lanes a line, no shifts.
+template <int L, int ALG, typename T, int LPL, bool SH, bool MF = false>
 __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL>())) void piv2d_kernel(Job j0, const Job* __restrict__ jobs,
                                                                       const int64_t* __restrict__ prefix,
                                                                       int n_jobs, int64_t n_ia_total, int ns0_cap,
@@ -532,7 +539,10 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
 
   // LDS layout is uniform across the launch (jobs of a batch may differ in
   // their margins): the host passes the largest ns0, ns1 of the launch
-  const size_t zbytes = region_bytes_2d(L, sizeof(T), ns0_cap, ns1_cap, ss0);
+  static_assert(!MF || (L == kMfL && std::is_same<T, float>::value && LPL == 4 && !SH && ALG != ALG_MASK_NSQECC &&
+                         Cfg2<L, T, LPL>::TPI == 256 && Cfg2<L, T, LPL>::IPB == 1),
+                "matrix-core form: length 64, FP32, 4 lanes a line, unshifted, unmasked");
+  const size_t zbytes = MF ? kMfRegion : region_bytes_2d(L, sizeof(T), ns0_cap, ns1_cap, ss0);
   const size_t c1bytes = (ALG == ALG_MASK_NSQECC) ? sim_bytes(ns0_cap, ns1_cap, sizeof(T)) : 0;
   const size_t per_ia = zbytes + c1bytes;
   unsigned char* base = smem + (size_t)sub * per_ia;
@@ -697,7 +707,8 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
   }
 
   // ---- correlation(s)
-  correlate_2d<L, LPL, T>(x, Z, t, ns0, ns1);
+  if constexpr (MF) correlate_2d_mfma<Group<C::TPI>>(x, Z, t, ns0, ns1, red);
+  else correlate_2d<L, LPL, T>(x, Z, t, ns0, ns1);
   if constexpr (ALG == ALG_MASK_NSQECC) {
     for (int i = t; i < ns0 * ns1; i += C::TPI) C1[i] = S[i];
     // second correlation: mask (x) G^2 (mask_nsqecc.jl:155)
@@ -980,13 +991,13 @@ size_t lds_bytes_2d(int alg, int ns0, int ns1, int ss0) {
   return (z + c1) * C::IPB + (size_t)C::IPB * kRedBytes;
 }
 
-template <int L, int ALG, typename T, int LPL, bool SH>
+template <int L, int ALG, typename T, int LPL, bool SH, bool MF = false>
 hipError_t launch_2d_k(const Job& j0, const Job* d_jobs, const int64_t* d_prefix, int n_jobs, int64_t n_ia_total,
                        int ns0, int ns1, size_t lds, hipStream_t s) {
   using C = Cfg2<L, T, LPL>;
-  allow_full_lds((const void*)piv2d_kernel<L, ALG, T, LPL, SH>);
+  allow_full_lds((const void*)piv2d_kernel<L, ALG, T, LPL, SH, MF>);
   const int64_t nb = (n_ia_total + C::IPB - 1) / C::IPB;
-  hipLaunchKernelGGL((piv2d_kernel<L, ALG, T, LPL, SH>), dim3((unsigned)nb), dim3(C::NT), lds, s, j0, d_jobs,
+  hipLaunchKernelGGL((piv2d_kernel<L, ALG, T, LPL, SH, MF>), dim3((unsigned)nb), dim3(C::NT), lds, s, j0, d_jobs,
                      d_prefix, n_jobs, n_ia_total, ns0, ns1);
   return hipGetLastError();
 }
@@ -996,6 +1007,16 @@ hipError_t launch_2d_k(const Job& j0, const Job* d_jobs, const int64_t* d_prefix
 template <int L, int ALG, typename T, int LPL>
 hipError_t launch_2d_t(const Job& j0, const Job* d_jobs, const int64_t* d_prefix, int n_jobs,
                        int64_t n_ia_total, int ns0, int ns1, int ss0, hipStream_t s) {
+  if (j0.precision_mode == kPrecSplitMfma) {
+    // single unshifted jobs only (the host checks); window up to 64 x 64
+    if constexpr (L == kMfL && std::is_same<T, float>::value && LPL == 4 && ALG != ALG_MASK_NSQECC) {
+      if (d_jobs || j0.shift || ns0 > kMfL || ns1 > kMfL) return hipErrorInvalidValue;
+      return launch_2d_k<L, ALG, T, LPL, false, true>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1,
+                                                      kMfRegion + kRedBytes, s);
+    } else {
+      return hipErrorInvalidValue;
+    }
+  }
   size_t lds = lds_bytes_2d<L, T, LPL>(ALG, ns0, ns1, ss0);
   if (lds > 160 * 1024) return hipErrorInvalidValue;
   if (j0.shift && !d_jobs)

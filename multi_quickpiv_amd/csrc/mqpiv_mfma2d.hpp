@@ -1,0 +1,302 @@
+// Split-precision DFT-matrix correlation of one 2-D IA on the matrix cores.
+//
+// The same computation as correlate_2d (the FFT correlation of fftcc.jl:101-126
+// and fft_ops.jl:95-105) for transforms of length 64, written as dense
+// products with the 64-point DFT matrix W(a, b) = exp(-2 pi i ab / 64):
+//
+//   X = Z W          (along dim 2; Z = A + iB, the packed IA and search window)
+//   Z^ = W X         (along dim 1)
+//   P = conj(A^) B^  (A^, B^ separated from Z^(k) and Z^(-k); columns k1 <= 32)
+//   Y = conj(W) P    (along dim 1, rows s0 of the window)
+//   C = Re(Y conj(W)) over the kept columns, each column k1 weighted by its
+//       multiplicity in the full spectrum (1 for 0 and 32, 2 for 1..31)
+//
+// Every real product runs on v_mfma_f32_16x16x16f16 with split operands: a value
+// v is held as hi = fp16(v) and lo = fp16((v - hi) 2^11), and a product a*b is
+// a_hi b_hi + 2^-11 (a_hi b_lo + a_lo b_hi), each sum accumulated in FP32 (the
+// lo*lo term, about 2^-22 relative, is dropped).  The lo parts carry the 2^11
+// so that they stay in FP16's normal range, as do the operands themselves:
+// every stage is scaled by a power of two from its worst-case growth so that
+// its largest possible value is below 2^15 (the windows to a maximum magnitude
+// in [2^12, 2^13)).  Subnormal FP16 operands would lose their low bits.
+//
+// A 16x16x16 product takes A (16x16) from lane l as row l%16, k = 4(l/16)..+3,
+// B (16x16) as column l%16, k = 4(l/16)..+3, and returns D as row 4(l/16)+v,
+// column l%16 (v < 4).  Operands of the four waves of an IA are staged in LDS as
+// FP16 rows of stride kMfLd; the DFT matrices come from a constant table.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "mqpiv_fft.hpp"
+
+namespace mqpiv {
+
+typedef _Float16 mf_h4 __attribute__((ext_vector_type(4)));
+typedef float mf_f4 __attribute__((ext_vector_type(4)));
+
+constexpr int kMfL = 64;                              // transform length of this form
+constexpr int kMfK1 = 48;                             // kept columns k1 <= 32, padded to whole tiles
+constexpr int kMfLd = 68;                             // FP16 row stride of staged operands
+constexpr size_t kMfPart = (size_t)kMfL * kMfLd * 2;  // bytes of one staged 64-row operand
+// LDS of one IA: two buffers of four staged operands (hi/lo of real and imaginary parts)
+constexpr size_t kMfRegion = 8 * kMfPart;
+
+// DFT tables as hi/lo FP16 pairs.  w[p][a * 64 + b]: p = 0/1 Re W hi/lo, 2/3 Im W,
+// 4/5 -Im W (W is symmetric, so row a serves as row or column).  c[p][s1 * 48 + k1]:
+// p = 0/1 m(k1) Re W(k1, s1), 2/3 m(k1) Im W(k1, s1), with the multiplicity m = 1
+// for k1 = 0 and 32, 2 for 1..31 and 0 for the padding 33..47.
+struct alignas(16) MfTables {
+  _Float16 w[6][kMfL * kMfL];
+  _Float16 c[4][kMfL * kMfK1];
+  static constexpr void split(double v, _Float16& hi, _Float16& lo) {
+    hi = (_Float16)v;
+    lo = (_Float16)((v - (double)hi) * 2048.0);
+  }
+  constexpr MfTables() : w{}, c{} {
+    constexpr Twiddles<kMfL, double> tw{};
+    for (int a = 0; a < kMfL; ++a)
+      for (int b = 0; b < kMfL; ++b) {
+        const int k = (a * b) % kMfL;
+        const double re = tw.c[k], im = -tw.s[k];
+        const int i = a * kMfL + b;
+        split(re, w[0][i], w[1][i]);
+        split(im, w[2][i], w[3][i]);
+        split(-im, w[4][i], w[5][i]);
+      }
+    for (int s1 = 0; s1 < kMfL; ++s1)
+      for (int k1 = 0; k1 < kMfK1; ++k1) {
+        const int k = (k1 * s1) % kMfL;
+        const double m = (k1 == 0 || k1 == kMfL / 2) ? 1.0 : (k1 < kMfL / 2 ? 2.0 : 0.0);
+        const int i = s1 * kMfK1 + k1;
+        split(m * tw.c[k], c[0][i], c[1][i]);
+        split(-m * tw.s[k], c[2][i], c[3][i]);
+      }
+  }
+};
+static __device__ const MfTables kMfTab{};
+
+__device__ __forceinline__ mf_f4 mf_mul(mf_h4 a, mf_h4 b, mf_f4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+}
+// a * b with split operands: hi*hi into c, the cross terms (carrying 2^11) into x
+__device__ __forceinline__ void mf_split(mf_h4 ah, mf_h4 al, mf_h4 bh, mf_h4 bl, mf_f4& c, mf_f4& x) {
+  c = mf_mul(ah, bh, c);
+  x = mf_mul(ah, bl, x);
+  x = mf_mul(al, bh, x);
+}
+__device__ __forceinline__ float mf_sum(float c, float x) { return c + x * (1.0f / 2048.0f); }
+__device__ __forceinline__ void mf_split1(float v, _Float16& hi, _Float16& lo) {
+  hi = (_Float16)v;
+  lo = (_Float16)((v - (float)hi) * 2048.0f);
+}
+__device__ __forceinline__ mf_h4 mf_ld(const _Float16* p) { return *reinterpret_cast<const mf_h4*>(p); }
+
+// Entry and exit as correlate_2d<64, 4, float>: on entry lane t (line = t/4,
+// q = t%4) holds x[k] = Z(line, 4k + q); on exit the window C(s0, s1),
+// s0 < ns0, s1 < ns1, is at the start of `region` as float, column-major.  One
+// IA per 256-thread block; `red` holds the block reductions.
+//
+// Scales (powers of two, exact): Z' = s Z with max |Z'| < 2^13 per window;
+// X' = 2^-6 Z'W (|X'| < 2^14); Z^' = W X'; P' = 2^-28 conj(A^') B^' (< 2^14);
+// Y' = 2^-7 conj(W) P' (< 2^14).  So C = C' 2^47 / (sa sb L^2).
+template <typename Grp>
+__device__ __forceinline__ void correlate_2d_mfma(cpx<float> (&x)[16], void* region, int t, int ns0, int ns1,
+                                                  double* red) {
+  constexpr int P = kMfL * kMfLd;  // halfs per staged operand
+  _Float16* S0 = reinterpret_cast<_Float16*>(region);
+  _Float16* S1 = S0 + 4 * P;
+  const int line = t >> 2, q = t & 3;
+  const int rb = t >> 6;                                  // this wave's 16-row block
+  const int l = t & 63, fr = l & 15, fk = (l >> 4) * 4;  // fragment row / column and k offset
+  const mf_f4 zero4 = {0.f, 0.f, 0.f, 0.f};
+
+  // --- window scales: max |A| and max |B| to [2^12, 2^13)
+  double ma = 0.0, mb = 0.0;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    ma = fmax(ma, (double)fabsf(x[k].x));
+    mb = fmax(mb, (double)fabsf(x[k].y));
+  }
+  Grp::reduce2(ma, mb, OpMax{}, OpMax{}, red, t);
+  int ea = 0, eb = 0;
+  frexp(ma, &ea);
+  frexp(mb, &eb);
+  const float sa = ldexpf(1.0f, 13 - ea), sb = ldexpf(1.0f, 13 - eb);
+
+  // --- stage Z row-major [k0][k1]
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int i = line * kMfLd + 4 * k + q;
+    mf_split1(x[k].x * sa, S0[i], S0[P + i]);
+    mf_split1(x[k].y * sb, S0[2 * P + i], S0[3 * P + i]);
+  }
+  __syncthreads();
+
+  const _Float16* Wrh = kMfTab.w[0];
+  const _Float16* Wrl = kMfTab.w[1];
+  const _Float16* Wih = kMfTab.w[2];
+  const _Float16* Wil = kMfTab.w[3];
+  const _Float16* Wnh = kMfTab.w[4];
+  const _Float16* Wnl = kMfTab.w[5];
+
+  // --- X' = 2^-6 Z' W: rows k0 of block rb; Xr = Zr Wr - Zi Wi, Xi = Zr Wi + Zi Wr.
+  // Staged column-major (XT[k1][k0]) in buffer 1.
+#pragma unroll 1
+  for (int c = 0; c < 4; ++c) {
+    mf_f4 xr = zero4, xr2 = zero4, xi = zero4, xi2 = zero4;
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      const int kk = ks * 16 + fk;
+      const int ai = (16 * rb + fr) * kMfLd + kk;
+      const mf_h4 zrh = mf_ld(S0 + ai), zrl = mf_ld(S0 + P + ai);
+      const mf_h4 zih = mf_ld(S0 + 2 * P + ai), zil = mf_ld(S0 + 3 * P + ai);
+      const int bi = (16 * c + fr) * kMfL + kk;
+      const mf_h4 wrh = mf_ld(Wrh + bi), wrl = mf_ld(Wrl + bi);
+      const mf_h4 wih = mf_ld(Wih + bi), wil = mf_ld(Wil + bi);
+      const mf_h4 wnh = mf_ld(Wnh + bi), wnl = mf_ld(Wnl + bi);
+      mf_split(zrh, zrl, wrh, wrl, xr, xr2);
+      mf_split(zih, zil, wnh, wnl, xr, xr2);
+      mf_split(zrh, zrl, wih, wil, xi, xi2);
+      mf_split(zih, zil, wrh, wrl, xi, xi2);
+    }
+    // this lane holds k0 = 16 rb + fk + v, k1 = 16 c + fr
+    mf_h4 rh, rl, ih, il;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      _Float16 a, b;
+      mf_split1(mf_sum(xr[v], xr2[v]) * (1.0f / 64.0f), a, b);
+      rh[v] = a;
+      rl[v] = b;
+      mf_split1(mf_sum(xi[v], xi2[v]) * (1.0f / 64.0f), a, b);
+      ih[v] = a;
+      il[v] = b;
+    }
+    const int o = (16 * c + fr) * kMfLd + 16 * rb + fk;
+    *reinterpret_cast<mf_h4*>(S1 + o) = rh;
+    *reinterpret_cast<mf_h4*>(S1 + P + o) = rl;
+    *reinterpret_cast<mf_h4*>(S1 + 2 * P + o) = ih;
+    *reinterpret_cast<mf_h4*>(S1 + 3 * P + o) = il;
+  }
+  __syncthreads();
+
+  // --- Z^' = W X': rows k0 of block rb; Z^r = Wr Xr - Wi Xi, Z^i = Wi Xr + Wr Xi.  FP32
+  // complex over buffer 0 (row stride 65), which every wave finished reading above.
+  cpx<float>* Zh = reinterpret_cast<cpx<float>*>(region);
+#pragma unroll 1
+  for (int c = 0; c < 4; ++c) {
+    mf_f4 zr = zero4, zr2 = zero4, zi = zero4, zi2 = zero4;
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      const int kk = ks * 16 + fk;
+      const int ai = (16 * rb + fr) * kMfL + kk;
+      const mf_h4 wrh = mf_ld(Wrh + ai), wrl = mf_ld(Wrl + ai);
+      const mf_h4 wih = mf_ld(Wih + ai), wil = mf_ld(Wil + ai);
+      const mf_h4 wnh = mf_ld(Wnh + ai), wnl = mf_ld(Wnl + ai);
+      const int bi = (16 * c + fr) * kMfLd + kk;
+      const mf_h4 xrh = mf_ld(S1 + bi), xrl = mf_ld(S1 + P + bi);
+      const mf_h4 xih = mf_ld(S1 + 2 * P + bi), xil = mf_ld(S1 + 3 * P + bi);
+      mf_split(wrh, wrl, xrh, xrl, zr, zr2);
+      mf_split(wnh, wnl, xih, xil, zr, zr2);
+      mf_split(wih, wil, xrh, xrl, zi, zi2);
+      mf_split(wrh, wrl, xih, xil, zi, zi2);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      Zh[(16 * rb + fk + v) * (kMfL + 1) + 16 * c + fr] = cpx<float>{mf_sum(zr[v], zr2[v]), mf_sum(zi[v], zi2[v])};
+  }
+  __syncthreads();
+
+  // --- P' = 2^-28 conj(A^') B^' for k1 <= 32 (zero up to 47), staged column-major
+  // (PT[k1][k0]) over buffer 1
+  for (int e = t; e < kMfL * kMfK1; e += 256) {
+    const int k0 = e & (kMfL - 1), k1 = e >> 6;
+    float pr = 0.f, pi = 0.f;
+    if (k1 <= kMfL / 2) {
+      const cpx<float> z = Zh[k0 * (kMfL + 1) + k1];
+      const cpx<float> n = Zh[((kMfL - k0) & (kMfL - 1)) * (kMfL + 1) + ((kMfL - k1) & (kMfL - 1))];
+      // A^ = (z + conj n) / 2 and B^ = (z - conj n) / 2i, with the 1/4 in the scale
+      const float ar = z.x + n.x, ai = z.y - n.y;
+      const float br = z.y + n.y, bi = n.x - z.x;
+      constexpr float s30 = 1.0f / 1073741824.0f;  // 2^-28 / 4
+      pr = (ar * br + ai * bi) * s30;
+      pi = (ar * bi - ai * br) * s30;
+    }
+    const int o = k1 * kMfLd + k0;
+    mf_split1(pr, S1[o], S1[P + o]);
+    mf_split1(pi, S1[2 * P + o], S1[3 * P + o]);
+  }
+  __syncthreads();
+
+  // --- Y' = 2^-7 conj(W) P' for the rows s0 < ns0: Yr = Wr Pr + Wi Pi, Yi = Wr Pi - Wi Pr.
+  // Staged row-major (Y[s0][k1], stride kMfK1 + 4) over buffer 0.
+  constexpr int YLd = kMfK1 + 4;
+  const bool rows = 16 * rb < ns0;
+  if (rows) {
+#pragma unroll 1
+    for (int c = 0; c < 3; ++c) {
+      mf_f4 yr = zero4, yr2 = zero4, yi = zero4, yi2 = zero4;
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) {
+        const int kk = ks * 16 + fk;
+        const int ai = (16 * rb + fr) * kMfL + kk;
+        const mf_h4 wrh = mf_ld(Wrh + ai), wrl = mf_ld(Wrl + ai);
+        const mf_h4 wih = mf_ld(Wih + ai), wil = mf_ld(Wil + ai);
+        const mf_h4 wnh = mf_ld(Wnh + ai), wnl = mf_ld(Wnl + ai);
+        const int bi = (16 * c + fr) * kMfLd + kk;
+        const mf_h4 prh = mf_ld(S1 + bi), prl = mf_ld(S1 + P + bi);
+        const mf_h4 pih = mf_ld(S1 + 2 * P + bi), pil = mf_ld(S1 + 3 * P + bi);
+        mf_split(wrh, wrl, prh, prl, yr, yr2);
+        mf_split(wih, wil, pih, pil, yr, yr2);
+        mf_split(wrh, wrl, pih, pil, yi, yi2);
+        mf_split(wnh, wnl, prh, prl, yi, yi2);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int o = (16 * rb + fk + v) * YLd + 16 * c + fr;
+        mf_split1(mf_sum(yr[v], yr2[v]) * (1.0f / 128.0f), S0[o], S0[P + o]);
+        mf_split1(mf_sum(yi[v], yi2[v]) * (1.0f / 128.0f), S0[2 * P + o], S0[3 * P + o]);
+      }
+    }
+  }
+  __syncthreads();
+
+  // --- C' = Yr' Cr + Yi' Ci over k1 < 48, for s0 < ns0 and s1 < ns1
+  mf_f4 acc[4], acc2[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) acc[c] = acc2[c] = zero4;
+  if (rows) {
+#pragma unroll
+    for (int ks = 0; ks < 3; ++ks) {
+      const int kk = ks * 16 + fk;
+      const int ai = (16 * rb + fr) * YLd + kk;
+      const mf_h4 yrh = mf_ld(S0 + ai), yrl = mf_ld(S0 + P + ai);
+      const mf_h4 yih = mf_ld(S0 + 2 * P + ai), yil = mf_ld(S0 + 3 * P + ai);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        if (16 * c >= ns1) continue;
+        const int bi = (16 * c + fr) * kMfK1 + kk;
+        const mf_h4 crh = mf_ld(kMfTab.c[0] + bi), crl = mf_ld(kMfTab.c[1] + bi);
+        const mf_h4 cih = mf_ld(kMfTab.c[2] + bi), cil = mf_ld(kMfTab.c[3] + bi);
+        mf_split(yrh, yrl, crh, crl, acc[c], acc2[c]);
+        mf_split(yih, yil, cih, cil, acc[c], acc2[c]);
+      }
+    }
+  }
+  __syncthreads();  // every wave has read Y before the window overwrites it
+  // C = C' 2^47 / (sa sb L^2) = C' 2^35 / (sa sb)
+  const double out = 34359738368.0 / ((double)sa * (double)sb);
+  float* S = reinterpret_cast<float*>(region);
+  if (rows) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int s0 = 16 * rb + fk + v, s1 = 16 * c + fr;
+        if (s0 < ns0 && s1 < ns1) S[s1 * ns0 + s0] = (float)((double)mf_sum(acc[c][v], acc2[c][v]) * out);
+      }
+  }
+  __syncthreads();
+}
+
+}  // namespace mqpiv

@@ -34,7 +34,9 @@ def to_c_params(p: PIVParameters, corr_code: Optional[int] = None, mode: str = "
                 multipass_ok: bool = False) -> _lib.MqpivParams:
     """Resolve a PIVParameters into the C struct (step = interSize - overlap when
     ``step`` is ``nothing``, piv_parameters.jl:88).  ``mode`` picks the transform
-    precision: "fp32", "fp64" or "auto" (FP64 where FP32 misses the parity bar).
+    precision: "fp32", "fp64", "auto" (FP64 where FP32 misses the parity bar) or
+    "mfma" (2-D DFTs as matrix-core products with hi + lo FP16 operands; single
+    2-D calls, search sizes up to 64, no masked measure, no multipass).
     ``multipass`` travels separately (mqpiv_piv_multipass); callers that run a
     single pass leave ``multipass_ok`` False and reject multipass > 1."""
     if not (p.unpadded and p.good_pad):
@@ -107,7 +109,7 @@ def PIV(input1: Any, input2: Any, *args: Any, precision: Any = np.float64,  # no
 
     With return_peaks=True also returns the 1-based integer correlation peaks,
     shape (N, nVF...) int32 (0 where no value beats 0 or the IA is skipped).
-    ``mode``: transform precision, "auto" (default), "fp32" or "fp64"."""
+    ``mode``: transform precision, "auto" (default), "fp32", "fp64" or "mfma"."""
     args = list(args)
     pivparams = setPIVParameters()
     if args and isinstance(args[-1], PIVParameters):
