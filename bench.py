@@ -18,6 +18,12 @@ time.  The CPU baseline is the Float64 oracle's FFT mode (a restatement of the
 reference's PIV_CPU loop with per-IA transforms) over the same four runs,
 OpenMP over IAs on the host cores.
 
+"transform_forms_is32" times the two interSize 32 runs once more as single calls
+in both GPU transform forms: FP32 radix transforms (mode "fp32") and the
+split-precision matrix-core form (mode "mfma", DFT-matrix products of hi + lo
+FP16 operands), with the latter's roofline against the dense FP16 matrix peak
+on the dense-DFT flop count.
+
 A batched leg ("batched_masked") runs 16 uneven masked NSQECC pairs (512^2 to
 4096^2, interSize 16/32/64) as one captured HIP graph and as 16 separate calls.
 A 3-D leg ("volume_3d") times FFTCC and ZNCC on FP32 512 x 512 x 256 particle
@@ -54,6 +60,7 @@ SM, STEP, SIZE = 16, 16, 2048
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
+PEAK_FP16_MFMA_TFLOPS = 2500.0  # dense
 
 
 def flop_fft_per_ia(cs: int, ndims: int = 2, transforms: int = 3) -> float:
@@ -63,6 +70,16 @@ def flop_fft_per_ia(cs: int, ndims: int = 2, transforms: int = 3) -> float:
     h = cs // 2 + 1
     Pc = h * cs ** (ndims - 1)
     return transforms * 2.5 * P * math.log2(P) + (transforms / 3) * (6 * Pc + P)
+
+
+def flop_dft_per_ia(cs: int, transforms: int = 3) -> float:
+    """Algorithmic flops of one 2-D IA as dense DFT-matrix products at cs (the
+    unpruned contraction; the hi/lo split does not count):
+    T*[4*h*cs^2 + 8*cs^2*h] + (T/3)*(6*Pc + P), h = cs/2 + 1."""
+    h = cs // 2 + 1
+    P = cs * cs
+    Pc = h * cs
+    return transforms * (4 * h * cs * cs + 8 * cs * cs * h) + (transforms / 3) * (6 * Pc + P)
 
 
 def uses_fp64(mode: str, alg: str, compute_sn: bool, ss: int) -> bool:
@@ -479,6 +496,39 @@ def main() -> int:
     torch.cuda.synchronize()
     wall_calls = time.perf_counter() - t1
 
+    # the interSize 32 runs in the two GPU transform forms, one call each: FP32
+    # radix transforms in registers and LDS, and the split-precision
+    # matrix-core form (the Float64 CPU oracle is the cpu_baseline leg)
+    forms_is32 = {"what": "interSize 32 runs, one call each, correlation-kernel time (events)"}
+    for j in jobs:
+        if j["isz"] != 32:
+            continue
+        res = {}
+        for m in ("fp32", "mfma"):
+            def call(m=m, j=j):
+                piv_device(ctx, j["p"], (SIZE, SIZE), _lib.MQPIV_F32, frame.a.data_ptr(), frame.b.data_ptr(),
+                           None, _lib.MQPIV_F64, j["VF"].data_ptr(), j["SN"].data_ptr(), stream, mode=m)
+            for _ in range(3):
+                call()
+            torch.cuda.synchronize()
+            nc = max(5, min(args.steps, 30))
+            ctx.profile_begin()
+            t2 = time.perf_counter()
+            for _ in range(nc):
+                call()
+            torch.cuda.synchronize()
+            w2 = time.perf_counter() - t2
+            kms = float(np.mean(ctx.profile_end()))
+            res[m] = {"kernel_ms": round(kms, 4), "ia_per_s": round(j["n"] / (kms * 1e-3), 1),
+                      "ia_per_s_per_call": round(j["n"] * nc / w2, 1)}
+        cs32 = 32 + 2 * SM
+        ach = flop_dft_per_ia(cs32) * j["n"] / (res["mfma"]["kernel_ms"] * 1e-3) / 1e12
+        res["mfma"]["roofline"] = {"bound": "mfma", "achieved": round(ach, 3), "peak": PEAK_FP16_MFMA_TFLOPS,
+                                   "unit": "TFLOP/s", "frac": round(ach / PEAK_FP16_MFMA_TFLOPS, 5),
+                                   "flop_per_ia": round(flop_dft_per_ia(cs32))}
+        res["mfma_over_fp32"] = round(res["fp32"]["kernel_ms"] / res["mfma"]["kernel_ms"], 3)
+        forms_is32[j["alg"]] = res
+
     # roofline of the dominant kernel
     kd = int(np.argmax(kern_ms))
     jd = jobs[kd]
@@ -571,6 +621,7 @@ def main() -> int:
         "per_call": {"value": round(n_step * args.steps / wall_calls, 1),
                      "ms_per_step": round(wall_calls * 1e3 / args.steps, 4),
                      "what": "the same four runs as four separate PIV calls per step"},
+        "transform_forms_is32": forms_is32,
         "outputs_finite": finite,
         "volume_3d": volume,
         "batched_masked": batched,

@@ -142,15 +142,21 @@ __device__ __forceinline__ void correlate_2d_mfma(cpx<float> (&x)[16], void* reg
 
   // --- X' = 2^-6 Z' W: rows k0 of block rb; Xr = Zr Wr - Zi Wi, Xi = Zr Wi + Zi Wr.
   // Staged column-major (XT[k1][k0]) in buffer 1.
+  // the block's rows of Z for all k steps (the A operand does not depend on the column tile)
+  mf_h4 za[4][4];
+#pragma unroll
+  for (int ks = 0; ks < 4; ++ks) {
+    const int ai = (16 * rb + fr) * kMfLd + ks * 16 + fk;
+#pragma unroll
+    for (int pp = 0; pp < 4; ++pp) za[ks][pp] = mf_ld(S0 + pp * P + ai);
+  }
 #pragma unroll 1
   for (int c = 0; c < 4; ++c) {
     mf_f4 xr = zero4, xr2 = zero4, xi = zero4, xi2 = zero4;
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) {
       const int kk = ks * 16 + fk;
-      const int ai = (16 * rb + fr) * kMfLd + kk;
-      const mf_h4 zrh = mf_ld(S0 + ai), zrl = mf_ld(S0 + P + ai);
-      const mf_h4 zih = mf_ld(S0 + 2 * P + ai), zil = mf_ld(S0 + 3 * P + ai);
+      const mf_h4 zrh = za[ks][0], zrl = za[ks][1], zih = za[ks][2], zil = za[ks][3];
       const int bi = (16 * c + fr) * kMfL + kk;
       const mf_h4 wrh = mf_ld(Wrh + bi), wrl = mf_ld(Wrl + bi);
       const mf_h4 wih = mf_ld(Wih + bi), wil = mf_ld(Wil + bi);
@@ -183,16 +189,26 @@ __device__ __forceinline__ void correlate_2d_mfma(cpx<float> (&x)[16], void* reg
   // --- Z^' = W X': rows k0 of block rb; Z^r = Wr Xr - Wi Xi, Z^i = Wi Xr + Wr Xi.  FP32
   // complex over buffer 0 (row stride 65), which every wave finished reading above.
   cpx<float>* Zh = reinterpret_cast<cpx<float>*>(region);
+  // this block's rows of W for all k steps (reused by the two inverse-side passes below)
+  mf_h4 wa[4][6];
+#pragma unroll
+  for (int ks = 0; ks < 4; ++ks) {
+    const int ai = (16 * rb + fr) * kMfL + ks * 16 + fk;
+    wa[ks][0] = mf_ld(Wrh + ai);
+    wa[ks][1] = mf_ld(Wrl + ai);
+    wa[ks][2] = mf_ld(Wih + ai);
+    wa[ks][3] = mf_ld(Wil + ai);
+    wa[ks][4] = mf_ld(Wnh + ai);
+    wa[ks][5] = mf_ld(Wnl + ai);
+  }
 #pragma unroll 1
   for (int c = 0; c < 4; ++c) {
     mf_f4 zr = zero4, zr2 = zero4, zi = zero4, zi2 = zero4;
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) {
       const int kk = ks * 16 + fk;
-      const int ai = (16 * rb + fr) * kMfL + kk;
-      const mf_h4 wrh = mf_ld(Wrh + ai), wrl = mf_ld(Wrl + ai);
-      const mf_h4 wih = mf_ld(Wih + ai), wil = mf_ld(Wil + ai);
-      const mf_h4 wnh = mf_ld(Wnh + ai), wnl = mf_ld(Wnl + ai);
+      const mf_h4 wrh = wa[ks][0], wrl = wa[ks][1], wih = wa[ks][2], wil = wa[ks][3], wnh = wa[ks][4],
+                  wnl = wa[ks][5];
       const int bi = (16 * c + fr) * kMfLd + kk;
       const mf_h4 xrh = mf_ld(S1 + bi), xrl = mf_ld(S1 + P + bi);
       const mf_h4 xih = mf_ld(S1 + 2 * P + bi), xil = mf_ld(S1 + 3 * P + bi);
@@ -239,10 +255,8 @@ __device__ __forceinline__ void correlate_2d_mfma(cpx<float> (&x)[16], void* reg
 #pragma unroll
       for (int ks = 0; ks < 4; ++ks) {
         const int kk = ks * 16 + fk;
-        const int ai = (16 * rb + fr) * kMfL + kk;
-        const mf_h4 wrh = mf_ld(Wrh + ai), wrl = mf_ld(Wrl + ai);
-        const mf_h4 wih = mf_ld(Wih + ai), wil = mf_ld(Wil + ai);
-        const mf_h4 wnh = mf_ld(Wnh + ai), wnl = mf_ld(Wnl + ai);
+        const mf_h4 wrh = wa[ks][0], wrl = wa[ks][1], wih = wa[ks][2], wil = wa[ks][3], wnh = wa[ks][4],
+                    wnl = wa[ks][5];
         const int bi = (16 * c + fr) * kMfLd + kk;
         const mf_h4 prh = mf_ld(S1 + bi), prl = mf_ld(S1 + P + bi);
         const mf_h4 pih = mf_ld(S1 + 2 * P + bi), pil = mf_ld(S1 + 3 * P + bi);

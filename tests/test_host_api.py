@@ -141,8 +141,17 @@ def test_vf_size_and_param_errors_without_gpu():
     assert lib.mqpiv_vf_size(C.byref(cp), dims, vf, C.byref(n)) == _lib.MQPIV_ERR_BAD_PARAMS
     cp.ndims, cp.corr_alg = 2, 9
     assert lib.mqpiv_vf_size(C.byref(cp), dims, vf, C.byref(n)) == _lib.MQPIV_ERR_BAD_PARAMS
-    cp.corr_alg, cp.precision_mode = 2, 1  # the split-precision mode is not built
+    cp.corr_alg, cp.precision_mode = 2, 5  # no such mode
     assert lib.mqpiv_vf_size(C.byref(cp), dims, vf, C.byref(n)) == _lib.MQPIV_ERR_UNSUPPORTED
+    # the split-precision matrix-core form: 2-D FFTCC/ZNCC/NSQECC only
+    cp.precision_mode = _lib.MQPIV_SPLIT_MFMA
+    assert lib.mqpiv_vf_size(C.byref(cp), dims, vf, C.byref(n)) == _lib.MQPIV_OK
+    cp.corr_alg = 3
+    assert lib.mqpiv_vf_size(C.byref(cp), dims, vf, C.byref(n)) == _lib.MQPIV_ERR_UNSUPPORTED
+    cp.corr_alg, cp.ndims = 2, 3
+    cp.inter_size[2], cp.step[2] = 1, 1
+    assert lib.mqpiv_vf_size(C.byref(cp), dims, vf, C.byref(n)) == _lib.MQPIV_ERR_UNSUPPORTED
+    cp.ndims = 2
     cp.precision_mode = _lib.MQPIV_AUTO
     cp.step[0] = 0
     assert lib.mqpiv_vf_size(C.byref(cp), dims, vf, C.byref(n)) == _lib.MQPIV_ERR_BAD_PARAMS
