@@ -23,7 +23,8 @@
 // A 16x16x16 product takes A (16x16) from lane l as row l%16, k = 4(l/16)..+3,
 // B (16x16) as column l%16, k = 4(l/16)..+3, and returns D as row 4(l/16)+v,
 // column l%16 (v < 4).  Operands of the four waves of an IA are staged in LDS as
-// FP16 rows of stride kMfLd; the DFT matrices come from a constant table.
+// FP16 rows of stride kMfLd (one buffer, about 34 KB, so that four IAs share a
+// CU); the DFT matrices come from a constant table.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -39,8 +40,10 @@ constexpr int kMfL = 64;                              // transform length of thi
 constexpr int kMfK1 = 48;                             // kept columns k1 <= 32, padded to whole tiles
 constexpr int kMfLd = 68;                             // FP16 row stride of staged operands
 constexpr size_t kMfPart = (size_t)kMfL * kMfLd * 2;  // bytes of one staged 64-row operand
-// LDS of one IA: two buffers of four staged operands (hi/lo of real and imaginary parts)
-constexpr size_t kMfRegion = 8 * kMfPart;
+// LDS of one IA: one buffer of four staged operands (hi/lo of real and imaginary
+// parts).  Each stage keeps its results in registers until every wave has read
+// the stage's operands, then stages them over the same buffer.
+constexpr size_t kMfRegion = 4 * kMfPart;
 
 // DFT tables as hi/lo FP16 pairs.  w[p][a * 64 + b]: p = 0/1 Re W hi/lo, 2/3 Im W,
 // 4/5 -Im W (W is symmetric, so row a serves as row or column).  c[p][s1 * 48 + k1]:
@@ -105,7 +108,6 @@ __device__ __forceinline__ void correlate_2d_mfma(cpx<float> (&x)[16], void* reg
                                                   double* red) {
   constexpr int P = kMfL * kMfLd;  // halfs per staged operand
   _Float16* S0 = reinterpret_cast<_Float16*>(region);
-  _Float16* S1 = S0 + 4 * P;
   const int line = t >> 2, q = t & 3;
   const int rb = t >> 6;                                  // this wave's 16-row block
   const int l = t & 63, fr = l & 15, fk = (l >> 4) * 4;  // fragment row / column and k offset
@@ -141,8 +143,8 @@ __device__ __forceinline__ void correlate_2d_mfma(cpx<float> (&x)[16], void* reg
   const _Float16* Wnl = kMfTab.w[5];
 
   // --- X' = 2^-6 Z' W: rows k0 of block rb; Xr = Zr Wr - Zi Wi, Xi = Zr Wi + Zi Wr.
-  // Staged column-major (XT[k1][k0]) in buffer 1.
-  // the block's rows of Z for all k steps (the A operand does not depend on the column tile)
+  // The block's rows of Z go to registers first; X is staged column-major
+  // (XT[k1][k0]) over them.
   mf_h4 za[4][4];
 #pragma unroll
   for (int ks = 0; ks < 4; ++ks) {
@@ -150,21 +152,21 @@ __device__ __forceinline__ void correlate_2d_mfma(cpx<float> (&x)[16], void* reg
 #pragma unroll
     for (int pp = 0; pp < 4; ++pp) za[ks][pp] = mf_ld(S0 + pp * P + ai);
   }
+  __syncthreads();
 #pragma unroll 1
   for (int c = 0; c < 4; ++c) {
     mf_f4 xr = zero4, xr2 = zero4, xi = zero4, xi2 = zero4;
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) {
       const int kk = ks * 16 + fk;
-      const mf_h4 zrh = za[ks][0], zrl = za[ks][1], zih = za[ks][2], zil = za[ks][3];
       const int bi = (16 * c + fr) * kMfL + kk;
       const mf_h4 wrh = mf_ld(Wrh + bi), wrl = mf_ld(Wrl + bi);
       const mf_h4 wih = mf_ld(Wih + bi), wil = mf_ld(Wil + bi);
       const mf_h4 wnh = mf_ld(Wnh + bi), wnl = mf_ld(Wnl + bi);
-      mf_split(zrh, zrl, wrh, wrl, xr, xr2);
-      mf_split(zih, zil, wnh, wnl, xr, xr2);
-      mf_split(zrh, zrl, wih, wil, xi, xi2);
-      mf_split(zih, zil, wrh, wrl, xi, xi2);
+      mf_split(za[ks][0], za[ks][1], wrh, wrl, xr, xr2);
+      mf_split(za[ks][2], za[ks][3], wnh, wnl, xr, xr2);
+      mf_split(za[ks][0], za[ks][1], wih, wil, xi, xi2);
+      mf_split(za[ks][2], za[ks][3], wrh, wrl, xi, xi2);
     }
     // this lane holds k0 = 16 rb + fk + v, k1 = 16 c + fr
     mf_h4 rh, rl, ih, il;
@@ -179,53 +181,64 @@ __device__ __forceinline__ void correlate_2d_mfma(cpx<float> (&x)[16], void* reg
       il[v] = b;
     }
     const int o = (16 * c + fr) * kMfLd + 16 * rb + fk;
-    *reinterpret_cast<mf_h4*>(S1 + o) = rh;
-    *reinterpret_cast<mf_h4*>(S1 + P + o) = rl;
-    *reinterpret_cast<mf_h4*>(S1 + 2 * P + o) = ih;
-    *reinterpret_cast<mf_h4*>(S1 + 3 * P + o) = il;
+    *reinterpret_cast<mf_h4*>(S0 + o) = rh;
+    *reinterpret_cast<mf_h4*>(S0 + P + o) = rl;
+    *reinterpret_cast<mf_h4*>(S0 + 2 * P + o) = ih;
+    *reinterpret_cast<mf_h4*>(S0 + 3 * P + o) = il;
   }
   __syncthreads();
 
-  // --- Z^' = W X': rows k0 of block rb; Z^r = Wr Xr - Wi Xi, Z^i = Wi Xr + Wr Xi.  FP32
-  // complex over buffer 0 (row stride 65), which every wave finished reading above.
+  // --- Z^' = W X': rows k0 of block rb; Z^r = Wr Xr - Wi Xi, Z^i = Wi Xr + Wr Xi.  Held in
+  // registers until every wave has read X, then written as FP32 complex (row stride 65).
   cpx<float>* Zh = reinterpret_cast<cpx<float>*>(region);
-  // this block's rows of W for all k steps (reused by the two inverse-side passes below)
-  mf_h4 wa[4][6];
-#pragma unroll
-  for (int ks = 0; ks < 4; ++ks) {
-    const int ai = (16 * rb + fr) * kMfL + ks * 16 + fk;
-    wa[ks][0] = mf_ld(Wrh + ai);
-    wa[ks][1] = mf_ld(Wrl + ai);
-    wa[ks][2] = mf_ld(Wih + ai);
-    wa[ks][3] = mf_ld(Wil + ai);
-    wa[ks][4] = mf_ld(Wnh + ai);
-    wa[ks][5] = mf_ld(Wnl + ai);
-  }
-#pragma unroll 1
-  for (int c = 0; c < 4; ++c) {
-    mf_f4 zr = zero4, zr2 = zero4, zi = zero4, zi2 = zero4;
+  mf_f4 zhr[4], zhi[4];
+  {
+    mf_h4 wa[4][6];
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) {
-      const int kk = ks * 16 + fk;
-      const mf_h4 wrh = wa[ks][0], wrl = wa[ks][1], wih = wa[ks][2], wil = wa[ks][3], wnh = wa[ks][4],
-                  wnl = wa[ks][5];
-      const int bi = (16 * c + fr) * kMfLd + kk;
-      const mf_h4 xrh = mf_ld(S1 + bi), xrl = mf_ld(S1 + P + bi);
-      const mf_h4 xih = mf_ld(S1 + 2 * P + bi), xil = mf_ld(S1 + 3 * P + bi);
-      mf_split(wrh, wrl, xrh, xrl, zr, zr2);
-      mf_split(wnh, wnl, xih, xil, zr, zr2);
-      mf_split(wih, wil, xrh, xrl, zi, zi2);
-      mf_split(wrh, wrl, xih, xil, zi, zi2);
+      const int ai = (16 * rb + fr) * kMfL + ks * 16 + fk;
+      wa[ks][0] = mf_ld(Wrh + ai);
+      wa[ks][1] = mf_ld(Wrl + ai);
+      wa[ks][2] = mf_ld(Wih + ai);
+      wa[ks][3] = mf_ld(Wil + ai);
+      wa[ks][4] = mf_ld(Wnh + ai);
+      wa[ks][5] = mf_ld(Wnl + ai);
     }
 #pragma unroll
-    for (int v = 0; v < 4; ++v)
-      Zh[(16 * rb + fk + v) * (kMfL + 1) + 16 * c + fr] = cpx<float>{mf_sum(zr[v], zr2[v]), mf_sum(zi[v], zi2[v])};
+    for (int c = 0; c < 4; ++c) {
+      mf_f4 zr = zero4, zr2 = zero4, zi = zero4, zi2 = zero4;
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) {
+        const int bi = (16 * c + fr) * kMfLd + ks * 16 + fk;
+        const mf_h4 xrh = mf_ld(S0 + bi), xrl = mf_ld(S0 + P + bi);
+        const mf_h4 xih = mf_ld(S0 + 2 * P + bi), xil = mf_ld(S0 + 3 * P + bi);
+        mf_split(wa[ks][0], wa[ks][1], xrh, xrl, zr, zr2);
+        mf_split(wa[ks][4], wa[ks][5], xih, xil, zr, zr2);
+        mf_split(wa[ks][2], wa[ks][3], xrh, xrl, zi, zi2);
+        mf_split(wa[ks][0], wa[ks][1], xih, xil, zi, zi2);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        zhr[c][v] = mf_sum(zr[v], zr2[v]);
+        zhi[c][v] = mf_sum(zi[v], zi2[v]);
+      }
+    }
   }
   __syncthreads();
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Zh[(16 * rb + fk + v) * (kMfL + 1) + 16 * c + fr] = cpx<float>{zhr[c][v], zhi[c][v]};
+  __syncthreads();
 
-  // --- P' = 2^-28 conj(A^') B^' for k1 <= 32 (zero up to 47), staged column-major
-  // (PT[k1][k0]) over buffer 1
-  for (int e = t; e < kMfL * kMfK1; e += 256) {
+  // --- P' = 2^-28 conj(A^') B^' for k1 <= 32 (zero up to 47); this lane's 12 entries
+  // e = t + 256 i (k0 = e % 64, k1 = e / 64) are held until every wave has read Z^,
+  // then staged column-major (PT[k1][k0])
+  constexpr int NPE = kMfL * kMfK1 / 256;
+  float ppr[NPE], ppi[NPE];
+#pragma unroll
+  for (int i = 0; i < NPE; ++i) {
+    const int e = t + 256 * i;
     const int k0 = e & (kMfL - 1), k1 = e >> 6;
     float pr = 0.f, pi = 0.f;
     if (k1 <= kMfL / 2) {
@@ -238,16 +251,26 @@ __device__ __forceinline__ void correlate_2d_mfma(cpx<float> (&x)[16], void* reg
       pr = (ar * br + ai * bi) * s30;
       pi = (ar * bi - ai * br) * s30;
     }
-    const int o = k1 * kMfLd + k0;
-    mf_split1(pr, S1[o], S1[P + o]);
-    mf_split1(pi, S1[2 * P + o], S1[3 * P + o]);
+    ppr[i] = pr;
+    ppi[i] = pi;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < NPE; ++i) {
+    const int e = t + 256 * i;
+    const int o = (e >> 6) * kMfLd + (e & (kMfL - 1));
+    mf_split1(ppr[i], S0[o], S0[P + o]);
+    mf_split1(ppi[i], S0[2 * P + o], S0[3 * P + o]);
   }
   __syncthreads();
 
   // --- Y' = 2^-7 conj(W) P' for the rows s0 < ns0: Yr = Wr Pr + Wi Pi, Yi = Wr Pi - Wi Pr.
-  // Staged row-major (Y[s0][k1], stride kMfK1 + 4) over buffer 0.
+  // Held until every wave has read P, then staged row-major (Y[s0][k1], stride kMfK1 + 4).
   constexpr int YLd = kMfK1 + 4;
   const bool rows = 16 * rb < ns0;
+  mf_f4 yyr[3], yyi[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) yyr[c] = yyi[c] = zero4;
   if (rows) {
 #pragma unroll 1
     for (int c = 0; c < 3; ++c) {
@@ -255,11 +278,13 @@ __device__ __forceinline__ void correlate_2d_mfma(cpx<float> (&x)[16], void* reg
 #pragma unroll
       for (int ks = 0; ks < 4; ++ks) {
         const int kk = ks * 16 + fk;
-        const mf_h4 wrh = wa[ks][0], wrl = wa[ks][1], wih = wa[ks][2], wil = wa[ks][3], wnh = wa[ks][4],
-                    wnl = wa[ks][5];
+        const int ai = (16 * rb + fr) * kMfL + kk;
+        const mf_h4 wrh = mf_ld(Wrh + ai), wrl = mf_ld(Wrl + ai);
+        const mf_h4 wih = mf_ld(Wih + ai), wil = mf_ld(Wil + ai);
+        const mf_h4 wnh = mf_ld(Wnh + ai), wnl = mf_ld(Wnl + ai);
         const int bi = (16 * c + fr) * kMfLd + kk;
-        const mf_h4 prh = mf_ld(S1 + bi), prl = mf_ld(S1 + P + bi);
-        const mf_h4 pih = mf_ld(S1 + 2 * P + bi), pil = mf_ld(S1 + 3 * P + bi);
+        const mf_h4 prh = mf_ld(S0 + bi), prl = mf_ld(S0 + P + bi);
+        const mf_h4 pih = mf_ld(S0 + 2 * P + bi), pil = mf_ld(S0 + 3 * P + bi);
         mf_split(wrh, wrl, prh, prl, yr, yr2);
         mf_split(wih, wil, pih, pil, yr, yr2);
         mf_split(wrh, wrl, pih, pil, yi, yi2);
@@ -267,11 +292,21 @@ __device__ __forceinline__ void correlate_2d_mfma(cpx<float> (&x)[16], void* reg
       }
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int o = (16 * rb + fk + v) * YLd + 16 * c + fr;
-        mf_split1(mf_sum(yr[v], yr2[v]) * (1.0f / 128.0f), S0[o], S0[P + o]);
-        mf_split1(mf_sum(yi[v], yi2[v]) * (1.0f / 128.0f), S0[2 * P + o], S0[3 * P + o]);
+        yyr[c][v] = mf_sum(yr[v], yr2[v]) * (1.0f / 128.0f);
+        yyi[c][v] = mf_sum(yi[v], yi2[v]) * (1.0f / 128.0f);
       }
     }
+  }
+  __syncthreads();
+  if (rows) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int o = (16 * rb + fk + v) * YLd + 16 * c + fr;
+        mf_split1(yyr[c][v], S0[o], S0[P + o]);
+        mf_split1(yyi[c][v], S0[2 * P + o], S0[3 * P + o]);
+      }
   }
   __syncthreads();
 
