@@ -11,7 +11,8 @@
 //   C = Re(Y conj(W)) over the kept columns, each column k1 weighted by its
 //       multiplicity in the full spectrum (1 for 0 and 32, 2 for 1..31)
 //
-// Every real product runs on v_mfma_f32_16x16x16f16 with split operands: a value
+// Every real product runs on v_mfma_f32_16x16x32_f16 (v_mfma_f32_16x16x16f16 for
+// the last 16-deep step) with split operands: a value
 // v is held as hi = fp16(v) and lo = fp16((v - hi) 2^11), and a product a*b is
 // a_hi b_hi + 2^-11 (a_hi b_lo + a_lo b_hi), each sum accumulated in FP32 (the
 // lo*lo term, about 2^-22 relative, is dropped).  The lo parts carry the 2^11
@@ -20,9 +21,9 @@
 // its largest possible value is below 2^15 (the windows to a maximum magnitude
 // in [2^12, 2^13)).  Subnormal FP16 operands would lose their low bits.
 //
-// A 16x16x16 product takes A (16x16) from lane l as row l%16, k = 4(l/16)..+3,
-// B (16x16) as column l%16, k = 4(l/16)..+3, and returns D as row 4(l/16)+v,
-// column l%16 (v < 4).  Operands of the four waves of an IA are staged in LDS as
+// A 16x16xK product (K = 16 or 32, G = K/4 values a lane) takes A (16xK) from
+// lane l as row l%16, k = G(l/16)..+G-1, B (Kx16) as column l%16, the same k,
+// and returns D as row 4(l/16)+v, column l%16 (v < 4).  Operands of the four waves of an IA are staged in LDS as
 // FP16 rows of stride kMfLd (one buffer, about 34 KB, so that four IAs share a
 // CU); the DFT matrices come from a constant table.
 #pragma once
@@ -34,11 +35,12 @@
 namespace mqpiv {
 
 typedef _Float16 mf_h4 __attribute__((ext_vector_type(4)));
+typedef _Float16 mf_h8 __attribute__((ext_vector_type(8)));
 typedef float mf_f4 __attribute__((ext_vector_type(4)));
 
 constexpr int kMfL = 64;                              // transform length of this form
 constexpr int kMfK1 = 48;                             // kept columns k1 <= 32, padded to whole tiles
-constexpr int kMfLd = 68;                             // FP16 row stride of staged operands
+constexpr int kMfLd = 72;                             // FP16 row stride of staged operands (16-byte rows)
 constexpr size_t kMfPart = (size_t)kMfL * kMfLd * 2;  // bytes of one staged 64-row operand
 // LDS of one IA: one buffer of four staged operands (hi/lo of real and imaginary
 // parts).  Each stage keeps its results in registers until every wave has read
@@ -94,6 +96,13 @@ __device__ __forceinline__ void mf_split1(float v, _Float16& hi, _Float16& lo) {
   lo = (_Float16)((v - (float)hi) * 2048.0f);
 }
 __device__ __forceinline__ mf_h4 mf_ld(const _Float16* p) { return *reinterpret_cast<const mf_h4*>(p); }
+__device__ __forceinline__ mf_h8 mf_ld8(const _Float16* p) { return *reinterpret_cast<const mf_h8*>(p); }
+// the same with 16x16x32 products (A: row l%16, k = 8(l/16)..+7; B: column l%16, same k)
+__device__ __forceinline__ void mf_split8(mf_h8 ah, mf_h8 al, mf_h8 bh, mf_h8 bl, mf_f4& c, mf_f4& x) {
+  c = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, c, 0, 0, 0);
+  x = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, x, 0, 0, 0);
+  x = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, x, 0, 0, 0);
+}
 
 // Entry and exit as correlate_2d<64, 4, float>: on entry lane t (line = t/4,
 // q = t%4) holds x[k] = Z(line, 4k + q); on exit the window C(s0, s1),
@@ -111,6 +120,7 @@ __device__ __forceinline__ void correlate_2d_mfma(cpx<float> (&x)[16], void* reg
   const int line = t >> 2, q = t & 3;
   const int rb = t >> 6;                                  // this wave's 16-row block
   const int l = t & 63, fr = l & 15, fk = (l >> 4) * 4;  // fragment row / column and k offset
+  const int fk8 = (l >> 4) * 8;                          // k offset of 32-deep fragments
   const mf_f4 zero4 = {0.f, 0.f, 0.f, 0.f};
 
   // --- window scales: max |A| and max |B| to [2^12, 2^13)
@@ -145,28 +155,27 @@ __device__ __forceinline__ void correlate_2d_mfma(cpx<float> (&x)[16], void* reg
   // --- X' = 2^-6 Z' W: rows k0 of block rb; Xr = Zr Wr - Zi Wi, Xi = Zr Wi + Zi Wr.
   // The block's rows of Z go to registers first; X is staged column-major
   // (XT[k1][k0]) over them.
-  mf_h4 za[4][4];
+  mf_h8 za[2][4];
 #pragma unroll
-  for (int ks = 0; ks < 4; ++ks) {
-    const int ai = (16 * rb + fr) * kMfLd + ks * 16 + fk;
+  for (int ks = 0; ks < 2; ++ks) {
+    const int ai = (16 * rb + fr) * kMfLd + ks * 32 + fk8;
 #pragma unroll
-    for (int pp = 0; pp < 4; ++pp) za[ks][pp] = mf_ld(S0 + pp * P + ai);
+    for (int pp = 0; pp < 4; ++pp) za[ks][pp] = mf_ld8(S0 + pp * P + ai);
   }
   __syncthreads();
 #pragma unroll 1
   for (int c = 0; c < 4; ++c) {
     mf_f4 xr = zero4, xr2 = zero4, xi = zero4, xi2 = zero4;
 #pragma unroll
-    for (int ks = 0; ks < 4; ++ks) {
-      const int kk = ks * 16 + fk;
-      const int bi = (16 * c + fr) * kMfL + kk;
-      const mf_h4 wrh = mf_ld(Wrh + bi), wrl = mf_ld(Wrl + bi);
-      const mf_h4 wih = mf_ld(Wih + bi), wil = mf_ld(Wil + bi);
-      const mf_h4 wnh = mf_ld(Wnh + bi), wnl = mf_ld(Wnl + bi);
-      mf_split(za[ks][0], za[ks][1], wrh, wrl, xr, xr2);
-      mf_split(za[ks][2], za[ks][3], wnh, wnl, xr, xr2);
-      mf_split(za[ks][0], za[ks][1], wih, wil, xi, xi2);
-      mf_split(za[ks][2], za[ks][3], wrh, wrl, xi, xi2);
+    for (int ks = 0; ks < 2; ++ks) {
+      const int bi = (16 * c + fr) * kMfL + ks * 32 + fk8;
+      const mf_h8 wrh = mf_ld8(Wrh + bi), wrl = mf_ld8(Wrl + bi);
+      const mf_h8 wih = mf_ld8(Wih + bi), wil = mf_ld8(Wil + bi);
+      const mf_h8 wnh = mf_ld8(Wnh + bi), wnl = mf_ld8(Wnl + bi);
+      mf_split8(za[ks][0], za[ks][1], wrh, wrl, xr, xr2);
+      mf_split8(za[ks][2], za[ks][3], wnh, wnl, xr, xr2);
+      mf_split8(za[ks][0], za[ks][1], wih, wil, xi, xi2);
+      mf_split8(za[ks][2], za[ks][3], wrh, wrl, xi, xi2);
     }
     // this lane holds k0 = 16 rb + fk + v, k1 = 16 c + fr
     mf_h4 rh, rl, ih, il;
@@ -193,29 +202,29 @@ __device__ __forceinline__ void correlate_2d_mfma(cpx<float> (&x)[16], void* reg
   cpx<float>* Zh = reinterpret_cast<cpx<float>*>(region);
   mf_f4 zhr[4], zhi[4];
   {
-    mf_h4 wa[4][6];
+    mf_h8 wa[2][6];
 #pragma unroll
-    for (int ks = 0; ks < 4; ++ks) {
-      const int ai = (16 * rb + fr) * kMfL + ks * 16 + fk;
-      wa[ks][0] = mf_ld(Wrh + ai);
-      wa[ks][1] = mf_ld(Wrl + ai);
-      wa[ks][2] = mf_ld(Wih + ai);
-      wa[ks][3] = mf_ld(Wil + ai);
-      wa[ks][4] = mf_ld(Wnh + ai);
-      wa[ks][5] = mf_ld(Wnl + ai);
+    for (int ks = 0; ks < 2; ++ks) {
+      const int ai = (16 * rb + fr) * kMfL + ks * 32 + fk8;
+      wa[ks][0] = mf_ld8(Wrh + ai);
+      wa[ks][1] = mf_ld8(Wrl + ai);
+      wa[ks][2] = mf_ld8(Wih + ai);
+      wa[ks][3] = mf_ld8(Wil + ai);
+      wa[ks][4] = mf_ld8(Wnh + ai);
+      wa[ks][5] = mf_ld8(Wnl + ai);
     }
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
       mf_f4 zr = zero4, zr2 = zero4, zi = zero4, zi2 = zero4;
 #pragma unroll
-      for (int ks = 0; ks < 4; ++ks) {
-        const int bi = (16 * c + fr) * kMfLd + ks * 16 + fk;
-        const mf_h4 xrh = mf_ld(S0 + bi), xrl = mf_ld(S0 + P + bi);
-        const mf_h4 xih = mf_ld(S0 + 2 * P + bi), xil = mf_ld(S0 + 3 * P + bi);
-        mf_split(wa[ks][0], wa[ks][1], xrh, xrl, zr, zr2);
-        mf_split(wa[ks][4], wa[ks][5], xih, xil, zr, zr2);
-        mf_split(wa[ks][2], wa[ks][3], xrh, xrl, zi, zi2);
-        mf_split(wa[ks][0], wa[ks][1], xih, xil, zi, zi2);
+      for (int ks = 0; ks < 2; ++ks) {
+        const int bi = (16 * c + fr) * kMfLd + ks * 32 + fk8;
+        const mf_h8 xrh = mf_ld8(S0 + bi), xrl = mf_ld8(S0 + P + bi);
+        const mf_h8 xih = mf_ld8(S0 + 2 * P + bi), xil = mf_ld8(S0 + 3 * P + bi);
+        mf_split8(wa[ks][0], wa[ks][1], xrh, xrl, zr, zr2);
+        mf_split8(wa[ks][4], wa[ks][5], xih, xil, zr, zr2);
+        mf_split8(wa[ks][2], wa[ks][3], xrh, xrl, zi, zi2);
+        mf_split8(wa[ks][0], wa[ks][1], xih, xil, zi, zi2);
       }
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
@@ -266,7 +275,7 @@ __device__ __forceinline__ void correlate_2d_mfma(cpx<float> (&x)[16], void* reg
 
   // --- Y' = 2^-7 conj(W) P' for the rows s0 < ns0: Yr = Wr Pr + Wi Pi, Yi = Wr Pi - Wi Pr.
   // Held until every wave has read P, then staged row-major (Y[s0][k1], stride kMfK1 + 4).
-  constexpr int YLd = kMfK1 + 4;
+  constexpr int YLd = kMfK1 + 8;  // 16-byte rows
   const bool rows = 16 * rb < ns0;
   mf_f4 yyr[3], yyi[3];
 #pragma unroll
@@ -276,19 +285,19 @@ __device__ __forceinline__ void correlate_2d_mfma(cpx<float> (&x)[16], void* reg
     for (int c = 0; c < 3; ++c) {
       mf_f4 yr = zero4, yr2 = zero4, yi = zero4, yi2 = zero4;
 #pragma unroll
-      for (int ks = 0; ks < 4; ++ks) {
-        const int kk = ks * 16 + fk;
+      for (int ks = 0; ks < 2; ++ks) {
+        const int kk = ks * 32 + fk8;
         const int ai = (16 * rb + fr) * kMfL + kk;
-        const mf_h4 wrh = mf_ld(Wrh + ai), wrl = mf_ld(Wrl + ai);
-        const mf_h4 wih = mf_ld(Wih + ai), wil = mf_ld(Wil + ai);
-        const mf_h4 wnh = mf_ld(Wnh + ai), wnl = mf_ld(Wnl + ai);
+        const mf_h8 wrh = mf_ld8(Wrh + ai), wrl = mf_ld8(Wrl + ai);
+        const mf_h8 wih = mf_ld8(Wih + ai), wil = mf_ld8(Wil + ai);
+        const mf_h8 wnh = mf_ld8(Wnh + ai), wnl = mf_ld8(Wnl + ai);
         const int bi = (16 * c + fr) * kMfLd + kk;
-        const mf_h4 prh = mf_ld(S0 + bi), prl = mf_ld(S0 + P + bi);
-        const mf_h4 pih = mf_ld(S0 + 2 * P + bi), pil = mf_ld(S0 + 3 * P + bi);
-        mf_split(wrh, wrl, prh, prl, yr, yr2);
-        mf_split(wih, wil, pih, pil, yr, yr2);
-        mf_split(wrh, wrl, pih, pil, yi, yi2);
-        mf_split(wnh, wnl, prh, prl, yi, yi2);
+        const mf_h8 prh = mf_ld8(S0 + bi), prl = mf_ld8(S0 + P + bi);
+        const mf_h8 pih = mf_ld8(S0 + 2 * P + bi), pil = mf_ld8(S0 + 3 * P + bi);
+        mf_split8(wrh, wrl, prh, prl, yr, yr2);
+        mf_split8(wih, wil, pih, pil, yr, yr2);
+        mf_split8(wrh, wrl, pih, pil, yi, yi2);
+        mf_split8(wnh, wnl, prh, prl, yi, yi2);
       }
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
@@ -315,16 +324,28 @@ __device__ __forceinline__ void correlate_2d_mfma(cpx<float> (&x)[16], void* reg
 #pragma unroll
   for (int c = 0; c < 4; ++c) acc[c] = acc2[c] = zero4;
   if (rows) {
+    {  // k1 < 32: one 32-deep step
+      const int ai = (16 * rb + fr) * YLd + fk8;
+      const mf_h8 yrh = mf_ld8(S0 + ai), yrl = mf_ld8(S0 + P + ai);
+      const mf_h8 yih = mf_ld8(S0 + 2 * P + ai), yil = mf_ld8(S0 + 3 * P + ai);
 #pragma unroll
-    for (int ks = 0; ks < 3; ++ks) {
-      const int kk = ks * 16 + fk;
-      const int ai = (16 * rb + fr) * YLd + kk;
+      for (int c = 0; c < 4; ++c) {
+        if (16 * c >= ns1) continue;
+        const int bi = (16 * c + fr) * kMfK1 + fk8;
+        const mf_h8 crh = mf_ld8(kMfTab.c[0] + bi), crl = mf_ld8(kMfTab.c[1] + bi);
+        const mf_h8 cih = mf_ld8(kMfTab.c[2] + bi), cil = mf_ld8(kMfTab.c[3] + bi);
+        mf_split8(yrh, yrl, crh, crl, acc[c], acc2[c]);
+        mf_split8(yih, yil, cih, cil, acc[c], acc2[c]);
+      }
+    }
+    {  // 32 <= k1 < 48: one 16-deep step
+      const int ai = (16 * rb + fr) * YLd + 32 + fk;
       const mf_h4 yrh = mf_ld(S0 + ai), yrl = mf_ld(S0 + P + ai);
       const mf_h4 yih = mf_ld(S0 + 2 * P + ai), yil = mf_ld(S0 + 3 * P + ai);
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         if (16 * c >= ns1) continue;
-        const int bi = (16 * c + fr) * kMfK1 + kk;
+        const int bi = (16 * c + fr) * kMfK1 + 32 + fk;
         const mf_h4 crh = mf_ld(kMfTab.c[0] + bi), crl = mf_ld(kMfTab.c[1] + bi);
         const mf_h4 cih = mf_ld(kMfTab.c[2] + bi), cil = mf_ld(kMfTab.c[3] + bi);
         mf_split(yrh, yrl, crh, crl, acc[c], acc2[c]);
