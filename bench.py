@@ -18,8 +18,8 @@ time.  The CPU baseline is the Float64 oracle's FFT mode (a restatement of the
 reference's PIV_CPU loop with per-IA transforms) over the same four runs,
 OpenMP over IAs on the host cores.
 
-"transform_forms_is32" times the two interSize 32 runs once more as single calls
-in both GPU transform forms: FP32 radix transforms (mode "fp32") and the
+"transform_forms" times the four runs once more as single calls in both GPU
+transform forms: FP32 radix transforms (mode "fp32") and the
 split-precision matrix-core form (mode "mfma", DFT-matrix products of hi + lo
 FP16 operands), with the latter's roofline against the dense FP16 matrix peak
 on the dense-DFT flop count.
@@ -499,10 +499,8 @@ def main() -> int:
     # the interSize 32 runs in the two GPU transform forms, one call each: FP32
     # radix transforms in registers and LDS, and the split-precision
     # matrix-core form (the Float64 CPU oracle is the cpu_baseline leg)
-    forms_is32 = {"what": "interSize 32 runs, one call each, correlation-kernel time (events)"}
+    forms_is32 = {"what": "the four runs, one call each, correlation-kernel time (events)"}
     for j in jobs:
-        if j["isz"] != 32:
-            continue
         res = {}
         for m in ("fp32", "mfma"):
             def call(m=m, j=j):
@@ -521,13 +519,13 @@ def main() -> int:
             kms = float(np.mean(ctx.profile_end()))
             res[m] = {"kernel_ms": round(kms, 4), "ia_per_s": round(j["n"] / (kms * 1e-3), 1),
                       "ia_per_s_per_call": round(j["n"] * nc / w2, 1)}
-        cs32 = 32 + 2 * SM
+        cs32 = j["isz"] + 2 * SM
         ach = flop_dft_per_ia(cs32) * j["n"] / (res["mfma"]["kernel_ms"] * 1e-3) / 1e12
         res["mfma"]["roofline"] = {"bound": "mfma", "achieved": round(ach, 3), "peak": PEAK_FP16_MFMA_TFLOPS,
                                    "unit": "TFLOP/s", "frac": round(ach / PEAK_FP16_MFMA_TFLOPS, 5),
                                    "flop_per_ia": round(flop_dft_per_ia(cs32))}
         res["mfma_over_fp32"] = round(res["fp32"]["kernel_ms"] / res["mfma"]["kernel_ms"], 3)
-        forms_is32[j["alg"]] = res
+        forms_is32[f'{j["alg"]}{j["isz"]}'] = res
 
     # roofline of the dominant kernel
     kd = int(np.argmax(kern_ms))
@@ -621,7 +619,7 @@ def main() -> int:
         "per_call": {"value": round(n_step * args.steps / wall_calls, 1),
                      "ms_per_step": round(wall_calls * 1e3 / args.steps, 4),
                      "what": "the same four runs as four separate PIV calls per step"},
-        "transform_forms_is32": forms_is32,
+        "transform_forms": forms_is32,
         "outputs_finite": finite,
         "volume_3d": volume,
         "batched_masked": batched,

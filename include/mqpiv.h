@@ -64,7 +64,7 @@ extern "C" {
 #define MQPIV_PLAIN_FP32 0 /* FP32 transforms; FP64 window sums, sum(F^2) and denominators    */
 #define MQPIV_SPLIT_MFMA 1 /* 2-D DFTs as matrix-core products with hi + lo FP16 operands,
                               FP32 accumulation; FP64 sums and denominators as in mode 0.
-                              Single 2-D calls with search sizes up to 64; FFTCC, ZNCC
+                              Single 2-D calls with search sizes up to 96; FFTCC, ZNCC
                               and NSQECC; no multipass                                        */
 #define MQPIV_FP64 2       /* every step in FP64 (2-D search sizes up to 96)                     */
 #define MQPIV_AUTO 3       /* FP64 where FP32 transforms miss the parity bar (masked NSQECC, and

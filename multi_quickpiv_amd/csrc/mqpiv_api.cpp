@@ -273,11 +273,11 @@ static int choose_2d(mqpiv_ctx* c, const Job& j, int* fp64_out, int* L_out) {
   };
   int fp64 = 0;
   if (j.precision_mode == MQPIV_SPLIT_MFMA) {
-    if (ss > 64)
-      return fail(c, MQPIV_ERR_UNSUPPORTED, "the split-precision matrix-core form takes 2-D search sizes up to 64 (got %d)",
+    if (ss > 96)
+      return fail(c, MQPIV_ERR_UNSUPPORTED, "the split-precision matrix-core form takes 2-D search sizes up to 96 (got %d)",
                   ss);
     *fp64_out = 0;
-    *L_out = 64;
+    *L_out = ss <= 64 ? 64 : 96;
     return MQPIV_OK;
   }
   if (j.precision_mode == MQPIV_FP64) {

@@ -220,6 +220,23 @@ struct OpMaxNaN {
 #include "mqpiv_mfma2d.hpp"
 namespace mqpiv {
 
+// The matrix-core form runs FP32 lengths 64 and 96 when one IA's block has a
+// whole number of waves per 16-row block and of product entries per lane.
+template <int L, typename T, int LPL>
+constexpr bool mf_ok() {
+  if constexpr ((L == 64 || L == 96) && std::is_same<T, float>::value) {
+    using C = Cfg2<L, T, LPL>;
+    return C::IPB == 1 && (C::TPI / 64) % (L / 16) == 0 && (L * MfCfg<L>::K1) % C::TPI == 0;
+  } else {
+    return false;
+  }
+}
+template <bool MF, int L>
+constexpr size_t mf_bytes() {
+  if constexpr (MF) return MfCfg<L>::region;
+  else return 0;
+}
+
 
 // (value, linear index) argmax with the reference's tie-break: the first element
 // in column-major order whose value is strictly above everything before it,
@@ -483,8 +500,10 @@ __device__ __forceinline__ double sat_box(const double* __restrict__ S, int64_t 
 #ifndef MQPIV_OCC_CAP
 #define MQPIV_OCC_CAP 8
 #endif
-template <int L, int ALG, typename T, int LPL>
+template <int L, int ALG, typename T, int LPL, bool MF = false>
 constexpr int min_waves_2d() {
+  // the matrix-core form holds many operand fragments: 4 waves per SIMD (128 VGPRs)
+  if constexpr (MF) return 4;
   using C = Cfg2<L, T, LPL>;
   constexpr size_t z = ((size_t)L * (L + 1) * 2 * sizeof(T) + 15) & ~(size_t)15;
   constexpr size_t c1 = ALG == ALG_MASK_NSQECC ? (((size_t)(L / 2 + 1) * (L / 2 + 1) * sizeof(T) + 15) & ~(size_t)15) : 0;
@@ -500,7 +519,7 @@ constexpr int min_waves_2d() {
 // MF: the split-precision matrix-core correlation (mqpiv_mfma2d.hpp) in place of
 // the register FFTs; length 64, FP32, four lanes a line, no shifts.
 template <int L, int ALG, typename T, int LPL, bool SH, bool MF = false>
-__global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL>())) void piv2d_kernel(Job j0, const Job* __restrict__ jobs,
+__global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL, MF>())) void piv2d_kernel(Job j0, const Job* __restrict__ jobs,
                                                                       const int64_t* __restrict__ prefix,
                                                                       int n_jobs, int64_t n_ia_total, int ns0_cap,
                                                                       int ns1_cap) {
@@ -539,10 +558,9 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
 
   // LDS layout is uniform across the launch (jobs of a batch may differ in
   // their margins): the host passes the largest ns0, ns1 of the launch
-  static_assert(!MF || (L == kMfL && std::is_same<T, float>::value && LPL == 4 && !SH && ALG != ALG_MASK_NSQECC &&
-                         Cfg2<L, T, LPL>::TPI == 256 && Cfg2<L, T, LPL>::IPB == 1),
-                "matrix-core form: length 64, FP32, 4 lanes a line, unshifted, unmasked");
-  const size_t zbytes = MF ? kMfRegion : region_bytes_2d(L, sizeof(T), ns0_cap, ns1_cap, ss0);
+  static_assert(!MF || (mf_ok<L, T, LPL>() && !SH && ALG != ALG_MASK_NSQECC),
+                "matrix-core form: FP32 lengths 64 and 96, unshifted, unmasked");
+  const size_t zbytes = MF ? mf_bytes<MF, L>() : region_bytes_2d(L, sizeof(T), ns0_cap, ns1_cap, ss0);
   const size_t c1bytes = (ALG == ALG_MASK_NSQECC) ? sim_bytes(ns0_cap, ns1_cap, sizeof(T)) : 0;
   const size_t per_ia = zbytes + c1bytes;
   unsigned char* base = smem + (size_t)sub * per_ia;
@@ -707,7 +725,7 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
   }
 
   // ---- correlation(s)
-  if constexpr (MF) correlate_2d_mfma<Group<C::TPI>>(x, Z, t, ns0, ns1, red);
+  if constexpr (MF) correlate_2d_mfma<L, LPL, C::TPI, Group<C::TPI>>(x, Z, t, ns0, ns1, red);
   else correlate_2d<L, LPL, T>(x, Z, t, ns0, ns1);
   if constexpr (ALG == ALG_MASK_NSQECC) {
     for (int i = t; i < ns0 * ns1; i += C::TPI) C1[i] = S[i];
@@ -1009,10 +1027,10 @@ hipError_t launch_2d_t(const Job& j0, const Job* d_jobs, const int64_t* d_prefix
                        int64_t n_ia_total, int ns0, int ns1, int ss0, hipStream_t s) {
   if (j0.precision_mode == kPrecSplitMfma) {
     // single unshifted jobs only (the host checks); window up to 64 x 64
-    if constexpr (L == kMfL && std::is_same<T, float>::value && LPL == 4 && ALG != ALG_MASK_NSQECC) {
-      if (d_jobs || j0.shift || ns0 > kMfL || ns1 > kMfL) return hipErrorInvalidValue;
+    if constexpr (mf_ok<L, T, LPL>() && ALG != ALG_MASK_NSQECC) {
+      if (d_jobs || j0.shift || ns0 > L || ns1 > L) return hipErrorInvalidValue;
       return launch_2d_k<L, ALG, T, LPL, false, true>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1,
-                                                      kMfRegion + kRedBytes, s);
+                                                      mf_bytes<true, L>() + kRedBytes, s);
     } else {
       return hipErrorInvalidValue;
     }

@@ -36,7 +36,7 @@ def to_c_params(p: PIVParameters, corr_code: Optional[int] = None, mode: str = "
     ``step`` is ``nothing``, piv_parameters.jl:88).  ``mode`` picks the transform
     precision: "fp32", "fp64", "auto" (FP64 where FP32 misses the parity bar) or
     "mfma" (2-D DFTs as matrix-core products with hi + lo FP16 operands; single
-    2-D calls, search sizes up to 64, no masked measure, no multipass).
+    2-D calls, search sizes up to 96, no masked measure, no multipass).
     ``multipass`` travels separately (mqpiv_piv_multipass); callers that run a
     single pass leave ``multipass_ok`` False and reject multipass > 1."""
     if not (p.unpadded and p.good_pad):

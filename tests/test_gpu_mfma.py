@@ -2,7 +2,7 @@
 Float64 oracle (bar: tests/parity.py), and the calls it refuses.
 
 The form computes the same fully-overlapping correlation as the FFT kernels
-(fftcc.jl:101-126) with the 64-point DFT as dense products of hi + lo FP16
+(fftcc.jl:101-126) with the 64- or 96-point DFT as dense products of hi + lo FP16
 operands accumulated in FP32, so its error is of the order of an FP32
 transform's."""
 from __future__ import annotations
@@ -18,10 +18,12 @@ pytestmark = pytest.mark.gpu
 
 CASES = [
     # shape, interSize, searchMargin, step
-    ((256, 256), 32, 16, 16),       # the 2048^2 workload's IA 32 geometry
+    ((256, 256), 32, 16, 16),       # the 2048^2 workload's IA 32 geometry (length 64)
     ((203, 187), (24, 20), (10, 6), (9, 11)),
     ((96, 112), 16, 8, 8),          # search size 32, run at length 64
     ((150, 130), (40, 30), (12, 17), (20, 15)),
+    ((320, 288), 64, 16, 16),       # the IA 64 geometry (length 96)
+    ((233, 251), (48, 56), (20, 12), (19, 23)),
 ]
 
 
@@ -77,8 +79,8 @@ def test_mfma_refuses_what_it_does_not_run(what):
             PIV(a, a, np.ones_like(a), p, mode="mfma")
     elif what == "search_size":
         a = np.zeros((128, 128), np.float32)
-        p = setPIVParameters(corr_alg="nsqecc", interSize=48, searchMargin=16, step=32)
-        with pytest.raises(RuntimeError, match="up to 64"):
+        p = setPIVParameters(corr_alg="nsqecc", interSize=64, searchMargin=20, step=32)
+        with pytest.raises(RuntimeError, match="up to 96"):
             PIV(a, a, p, mode="mfma")
     else:
         a = np.zeros((128, 128), np.float32)
