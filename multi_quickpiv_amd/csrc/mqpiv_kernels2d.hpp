@@ -503,7 +503,8 @@ __device__ __forceinline__ double sat_box(const double* __restrict__ S, int64_t 
 template <int L, int ALG, typename T, int LPL, bool MF = false>
 constexpr int min_waves_2d() {
   // the matrix-core form holds many operand fragments: 4 waves per SIMD (128 VGPRs)
-  if constexpr (MF) return 4;
+  // at length 64; at 96 one 12-wave IA fills a CU either way, so 3 (168 VGPRs)
+  if constexpr (MF) return L == 96 ? 3 : 4;
   using C = Cfg2<L, T, LPL>;
   constexpr size_t z = ((size_t)L * (L + 1) * 2 * sizeof(T) + 15) & ~(size_t)15;
   constexpr size_t c1 = ALG == ALG_MASK_NSQECC ? (((size_t)(L / 2 + 1) * (L / 2 + 1) * sizeof(T) + 15) & ~(size_t)15) : 0;
