@@ -27,72 +27,17 @@
 
 #include "../../include/mqpiv.h"
 #include "mqpiv_batch.h"
+#include "mqpiv_server_core.h"
 
 namespace {
+
+using mqpiv_srv::al256;
+using mqpiv_srv::Req;
 
 double now_s() {
   using namespace std::chrono;
   return duration<double>(steady_clock::now().time_since_epoch()).count();
 }
-
-size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
-
-// First-fit allocator over one device region, with coalescing on free.
-class Pool {
- public:
-  void init(size_t bytes) {
-    free_.clear();
-    free_[0] = bytes;
-    cap_ = bytes;
-  }
-  size_t alloc(size_t n) {
-    n = al256(n);
-    for (auto it = free_.begin(); it != free_.end(); ++it) {
-      if (it->second >= n) {
-        size_t off = it->first, sz = it->second;
-        free_.erase(it);
-        if (sz > n) free_[off + n] = sz - n;
-        used_ += n;
-        return off;
-      }
-    }
-    return SIZE_MAX;
-  }
-  void release(size_t off, size_t n) {
-    n = al256(n);
-    used_ -= n;
-    auto it = free_.emplace(off, n).first;
-    auto nx = std::next(it);
-    if (nx != free_.end() && it->first + it->second == nx->first) {
-      it->second += nx->second;
-      free_.erase(nx);
-    }
-    if (it != free_.begin()) {
-      auto pv = std::prev(it);
-      if (pv->first + pv->second == it->first) {
-        pv->second += it->second;
-        free_.erase(it);
-      }
-    }
-  }
-  size_t capacity() const { return cap_; }
-
- private:
-  std::map<size_t, size_t> free_;
-  size_t cap_ = 0, used_ = 0;
-};
-
-struct Req {
-  uint64_t id = 0;
-  mqpiv_request req{};  // the client's buffers: host, or device when on_device
-  bool on_device = false;
-  int64_t n_ia = 0;
-  size_t npx = 0, esz = 0, osz = 0;
-  size_t io_bytes = 0, ws_bytes = 0;  // staged inputs/outputs (host requests) + integral arrays
-  size_t d_off = SIZE_MAX, d_bytes = 0;
-  double t_submit = 0, t_done = 0;
-  int status = -1;  // -1 pending/running; >= 0 final status
-};
 
 struct Slot {
   mqpiv_ctx* ctx = nullptr;
@@ -104,10 +49,9 @@ struct Slot {
 
 }  // namespace
 
-struct mqpiv_server {
+struct mqpiv_server : mqpiv_srv::Core {
   int device = 0;
   char* d_pool = nullptr;
-  Pool pool;
   int max_batch = 64;
   std::vector<Slot> slots;
   std::deque<Req*> pending;
@@ -118,12 +62,6 @@ struct mqpiv_server {
   bool stop = false;
   uint64_t next_id = 1;
   size_t next_slot = 0;
-  std::string err;  // message of the last failed slot
-  // statistics
-  std::vector<double> lat;
-  int64_t ia_done = 0;
-  uint64_t pairs_done = 0;
-  double t_first = 0, t_last = 0;
 };
 
 // Device bytes a request holds while admitted: for host requests its staged
@@ -203,35 +141,24 @@ static int launch_slot(mqpiv_server* sv, Slot& sl) {
   return MQPIV_OK;
 }
 
-static void finish(mqpiv_server* sv, Req* r, int status, double t) {
-  r->status = status;
-  r->t_done = t;
-  if (r->d_off != SIZE_MAX) sv->pool.release(r->d_off, r->d_bytes);
-  r->d_off = SIZE_MAX;
-  if (status == MQPIV_OK) {
-    sv->lat.push_back(t - r->t_submit);
-    sv->ia_done += r->n_ia;
-    sv->pairs_done += 1;
-    if (sv->t_first == 0) sv->t_first = r->t_submit;
-    sv->t_last = t;
-  }
-}
+static void finish(mqpiv_server* sv, Req* r, int status, double t) { mqpiv_srv::finish(*sv, r, status, t); }
 
 static void worker_loop(mqpiv_server* sv) {
   hipSetDevice(sv->device);
   std::unique_lock<std::mutex> lk(sv->mu);
   while (true) {
     bool progress = false;
-    // retire finished slots
-    for (auto& sl : sv->slots) {
-      if (sl.busy && hipEventQuery(sl.done) == hipSuccess) {
-        const double t = now_s();
-        for (Req* r : sl.reqs) finish(sv, r, MQPIV_OK, t);
-        sl.reqs.clear();
-        sl.busy = false;
-        progress = true;
-        sv->cv_done.notify_all();
-      }
+    // retire finished slots; a failed query ends the slot's requests with a
+    // device error instead of leaving the slot busy for ever
+    if (mqpiv_srv::retire(*sv, sv->slots, now_s(), [](Slot& sl, std::string& msg) {
+          const hipError_t e = hipEventQuery(sl.done);
+          if (e == hipErrorNotReady) return mqpiv_srv::Poll::kRunning;
+          if (e == hipSuccess) return mqpiv_srv::Poll::kDone;
+          msg = std::string("device error in a running batch: ") + hipGetErrorString(e);
+          return mqpiv_srv::Poll::kFailed;
+        })) {
+      progress = true;
+      sv->cv_done.notify_all();
     }
     if (sv->stop && sv->pending.empty()) {
       bool any = false;

@@ -78,6 +78,22 @@ typedef struct {
   double mask_threshold;
 } orc_params;
 
+/* Planted mistakes.  A test sets one (orc_set_plant) to show that the checks
+ * that pin this restatement detect it, then sets ORC_PLANT_NONE again.  Each
+ * is a plausible misreading of the reference:
+ *   MASK_C1_UNMASKED  C1 = corr(F, G) without the mask product
+ *                     (masked_nsqecc_ops.jl:19-25 multiplies F by the mask);
+ *   MASK_C2_BOXSUM    C2 = the window sum of G^2 instead of corr(mask, G^2)
+ *                     (mask_nsqecc.jl:155);
+ *   MASK_NO_W         sum(F^2) = sum((F*m)^2) without the division by
+ *                     w = m (m > 0) or 1 (mask_nsqecc.jl:144-149);
+ *   ZNCC_RAW_F2       sum(F^2) of the raw IA instead of sum((F - mean)^2)
+ *                     (zncc.jl:131-135 squares the mean-subtracted IA).   */
+enum { ORC_PLANT_NONE = 0, ORC_PLANT_MASK_C1_UNMASKED = 1, ORC_PLANT_MASK_C2_BOXSUM = 2,
+       ORC_PLANT_MASK_NO_W = 3, ORC_PLANT_ZNCC_RAW_F2 = 4 };
+static int orc_plant = ORC_PLANT_NONE;
+void orc_set_plant(int k) { orc_plant = k; }
+
 typedef struct {
   int64_t tlf[3], brb[3];       /* interrogation area corners (1-based)      */
   int64_t sr_tlf[3], sr_brb[3]; /* clipped search region corners (1-based)   */
@@ -196,6 +212,7 @@ typedef struct {
   int nd;
   int64_t is[3], ss[3], ns[3]; /* ns = 2*SM+1 */
   int64_t nF, nG, nS;
+  double meanF; /* ZNCC: the subtracted IA mean (planted-mistake use only) */
   double* F;   /* is^N : F, F-mean, or F*m                        */
   double* M;   /* is^N : mask IA (masked only)                     */
   double* G;   /* ss^N : G or G-mean (quirk-placed, zero padded)   */
@@ -279,6 +296,7 @@ static void load_windows(const orc_params* p, const int64_t* dims, const void* i
   /* zncc_ops.jl:10: the mean is computed in the input element type. */
   float meanF32 = (float)sumF / (float)w->nF;
   double meanF64 = sumF / (double)w->nF;
+  w->meanF = (dt == ORC_F32) ? (double)meanF32 : meanF64;
   for (int64_t z = 0; z < w->is[2]; ++z)
     for (int64_t y = 0; y < w->is[1]; ++y)
       for (int64_t x = 0; x < w->is[0]; ++x) {
@@ -289,8 +307,9 @@ static void load_windows(const orc_params* p, const int64_t* dims, const void* i
                                     : img_at(img1, dt, lin) - meanF64;
         } else if (masked) {
           double m = img_at(mask, dt, lin);
-          w->M[k] = m;
-          w->F[k] = img_at(img1, dt, lin) * m;
+          w->M[k] = (orc_plant == ORC_PLANT_MASK_C2_BOXSUM) ? 1.0 : m;
+          w->F[k] = (orc_plant == ORC_PLANT_MASK_C1_UNMASKED) ? img_at(img1, dt, lin)
+                                                              : img_at(img1, dt, lin) * m;
         } else {
           w->F[k] = img_at(img1, dt, lin);
         }
@@ -586,12 +605,17 @@ static void similarity(const orc_params* p, orc_work* w, int mode) {
     /* mask_nsqecc.jl:144-149 */
     for (int64_t k = 0; k < w->nF; ++k) {
       double m = w->M[k];
-      double wi = (m > 0) ? m : 1.0;
-      sumF2 += w->F[k] * w->F[k] / wi;
+      double wi = (m > 0 && orc_plant != ORC_PLANT_MASK_NO_W) ? m : 1.0;
+      double fm = (orc_plant == ORC_PLANT_MASK_C1_UNMASKED) ? w->F[k] * m : w->F[k];
+      sumF2 += fm * fm / wi;
     }
   } else if (p->alg != ORC_FFTCC) {
     /* zncc.jl:131-135 / nsqecc.jl:125-129 */
-    for (int64_t k = 0; k < w->nF; ++k) sumF2 += w->F[k] * w->F[k];
+    for (int64_t k = 0; k < w->nF; ++k) {
+      double f = w->F[k];
+      if (p->alg == ORC_ZNCC && orc_plant == ORC_PLANT_ZNCC_RAW_F2) f += w->meanF;
+      sumF2 += f * f;
+    }
   }
 
   if (mode == ORC_FFT) corr_fft(w->F, w->G, w, w->S);

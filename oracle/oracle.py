@@ -77,6 +77,8 @@ def load() -> C.CDLL:
         lib.orc_subpixel.argtypes = [vp, C.POINTER(C.c_int64), C.c_int, C.POINTER(C.c_int64),
                                      C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                      C.c_double, C.POINTER(C.c_double)]
+        lib.orc_set_plant.restype = None
+        lib.orc_set_plant.argtypes = [C.c_int]
         lib.orc_sn.restype = C.c_double
         lib.orc_sn.argtypes = [vp, C.POINTER(C.c_int64), C.c_int, C.POINTER(C.c_int64),
                                C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
@@ -103,6 +105,15 @@ def make_params(alg, ndims: int, inter_size, search_margin, step, compute_sn: bo
     p.threshold = threshold
     p.mask_threshold = mask_threshold
     return p
+
+
+# Planted mistakes (orc_set_plant in mqpiv_oracle.c): used by the pinning tests
+# to show that their checks detect each one.
+PLANT_NONE, PLANT_MASK_C1_UNMASKED, PLANT_MASK_C2_BOXSUM, PLANT_MASK_NO_W, PLANT_ZNCC_RAW_F2 = range(5)
+
+
+def set_plant(k: int) -> None:
+    load().orc_set_plant(int(k))
 
 
 def good_size_real(n: int) -> int:
