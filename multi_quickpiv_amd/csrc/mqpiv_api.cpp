@@ -29,7 +29,8 @@ namespace mqpiv {
 int fft_len_2d(int fp64, int ss);
 size_t lds_2d(int fp64, int L, int alg, int ns0, int ns1, int ss0);
 hipError_t launch_piv2d(int fp64, int L, int alg, const Job& j0, const Job* d_jobs, const int64_t* d_prefix,
-                        int n_jobs, int64_t n_ia_total, int ns0, int ns1, int ss0, hipStream_t s);
+                        int n_jobs, int64_t n_ia_total, int ns0, int ns1, int ss0, void* gz, hipStream_t s);
+size_t gz_scratch_2d(int fp64, int L, int64_t n_ia);
 size_t sat_bytes(const int64_t* dims, int nd);
 size_t sat_tmp_bytes(const int64_t* dims, int nd);
 hipError_t build_sat(const void* img, int dt, int power, const int64_t* dims, int nd, double* S, double* tmp,
@@ -104,6 +105,7 @@ extern "C" int mqpiv_ctx_destroy(mqpiv_ctx* c) {
   if (c->h_table) hipHostFree(c->h_table);
   if (c->d_pp) hipFree(c->d_pp);
   if (c->d_mp) hipFree(c->d_mp);
+  if (c->d_gz) hipFree(c->d_gz);
   if (c->table_done) hipEventDestroy(c->table_done);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
@@ -282,7 +284,7 @@ static int choose_2d(mqpiv_ctx* c, const Job& j, int* fp64_out, int* L_out) {
   }
   if (j.precision_mode == MQPIV_FP64) {
     if (!fits(1))
-      return fail(c, MQPIV_ERR_UNSUPPORTED, "FP64 transforms support 2-D search sizes up to 96 (got %d)", ss);
+      return fail(c, MQPIV_ERR_UNSUPPORTED, "FP64 transforms support 2-D search sizes up to 128 (got %d)", ss);
     fp64 = 1;
   } else if (j.precision_mode == MQPIV_AUTO) {
     bool want = j.alg == ALG_MASK_NSQECC || (j.alg == ALG_ZNCC && j.compute_sn);
@@ -305,10 +307,15 @@ static int launch_job(mqpiv_ctx* c, Job& j, hipStream_t s, char* ws = nullptr) {
     if (rc2) return rc2;
     rc2 = prepare_sats(c, j, s, ws);
     if (rc2) return rc2;
+    const size_t gzb = gz_scratch_2d(fp64, L, j.n_ia);
+    if (gzb) {
+      rc2 = ensure(c, &c->d_gz, &c->gz_cap, gzb, s);
+      if (rc2) return rc2;
+    }
     hipError_t e;
     {
       KernelTimer kt(c, s);
-      e = launch_piv2d(fp64, L, j.alg, j, nullptr, nullptr, 1, j.n_ia, ns0, ns1, ss0, s);
+      e = launch_piv2d(fp64, L, j.alg, j, nullptr, nullptr, 1, j.n_ia, ns0, ns1, ss0, c->d_gz, s);
     }
     if (e != hipSuccess) return fail(c, MQPIV_ERR_DEVICE, "2-D launch failed: %s", hipGetErrorString(e));
     return MQPIV_OK;
@@ -568,6 +575,7 @@ struct BatchPlan {
   std::vector<Job> jobs3d;
   std::vector<BatchGroup> groups;
   size_t sat_total = 0, sat_tmp = 0, table_bytes = 0;
+  size_t gz_bytes = 0;  // global transform buffers (shared by the groups, which run in turn)
 };
 
 size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -646,6 +654,7 @@ static int plan_batch(mqpiv_ctx* c, const mqpiv_request* r, int n, BatchPlan& P)
     if (lds_2d(g.fp64, g.L, g.alg, g.ns0cap, g.ns1cap, g.ss0cap) > 160 * 1024)
       return fail(c, MQPIV_ERR_UNSUPPORTED, "batch group (L=%d) does not fit in LDS", g.L);
     g.table_off = P.table_bytes;
+    P.gz_bytes = std::max(P.gz_bytes, gz_scratch_2d(g.fp64, g.L, g.n_ia));
     P.table_bytes += al256(g.members.size() * sizeof(Job) + (g.members.size() + 1) * sizeof(int64_t));
   }
   return MQPIV_OK;
@@ -681,7 +690,7 @@ static void fill_tables(BatchPlan& P, char* sat_base, char* host_table, void* co
 // Integral arrays + one launch per group (device job tables already in place).
 // The prefix-scan temporary follows the arrays: at sat_base + sat_total, or in
 // each request's workspace after its own arrays.
-static int enqueue_2d(mqpiv_ctx* c, BatchPlan& P, char* sat_base, char* d_table, hipStream_t s,
+static int enqueue_2d(mqpiv_ctx* c, BatchPlan& P, char* sat_base, char* d_table, char* gz, hipStream_t s,
                       bool own_ws = false) {
   for (size_t i = 0; i < P.jobs2d.size(); ++i) {
     const Job& j = P.jobs2d[i];
@@ -703,7 +712,7 @@ static int enqueue_2d(mqpiv_ctx* c, BatchPlan& P, char* sat_base, char* d_table,
       // every workgroup would load before it can address its windows
       const bool one = g.members.size() == 1;
       e = launch_piv2d(g.fp64, g.L, g.alg, P.jobs2d[g.members[0]], one ? nullptr : dj, one ? nullptr : dp,
-                       (int)g.members.size(), g.n_ia, g.ns0cap, g.ns1cap, g.ss0cap, s);
+                       (int)g.members.size(), g.n_ia, g.ns0cap, g.ns1cap, g.ss0cap, gz, s);
     }
     if (e != hipSuccess) return fail(c, MQPIV_ERR_DEVICE, "batch launch failed: %s", hipGetErrorString(e));
   }
@@ -725,6 +734,10 @@ int batch_device_ws(mqpiv_ctx* c, const mqpiv_request* reqs, int32_t n, void* st
     }
     rc = ensure(c, &c->d_table, &c->table_cap, P.table_bytes + 256, s);
     if (rc) return rc;
+    if (P.gz_bytes) {
+      rc = ensure(c, &c->d_gz, &c->gz_cap, P.gz_bytes, s);
+      if (rc) return rc;
+    }
     // pinned staging for the job tables; the previous batch's copy must be done
     if (c->table_done) hipEventSynchronize(c->table_done);
     if (P.table_bytes > c->h_table_cap) {
@@ -738,7 +751,7 @@ int batch_device_ws(mqpiv_ctx* c, const mqpiv_request* reqs, int32_t n, void* st
     fill_tables(P, (char*)c->d_bsat, (char*)c->h_table, ws);
     HIP_OR_FAIL(c, hipMemcpyAsync(c->d_table, c->h_table, P.table_bytes, hipMemcpyHostToDevice, s));
     HIP_OR_FAIL(c, hipEventRecord(c->table_done, s));
-    rc = enqueue_2d(c, P, (char*)c->d_bsat, (char*)c->d_table, s, ws != nullptr);
+    rc = enqueue_2d(c, P, (char*)c->d_bsat, (char*)c->d_table, (char*)c->d_gz, s, ws != nullptr);
     if (rc) return rc;
   }
   for (size_t k = 0; k < P.jobs3d.size(); ++k) {
@@ -771,13 +784,15 @@ extern "C" int mqpiv_graph_create(mqpiv_ctx* c, const mqpiv_request* reqs, int32
   mqpiv_graph* g = new mqpiv_graph();
   g->ctx = c;
   const size_t sat_part = al256(P.sat_total + P.sat_tmp + 256);
-  hipError_t e = hipMalloc(&g->d_mem, sat_part + P.table_bytes + 256);
+  const size_t table_part = al256(P.table_bytes + 256);
+  hipError_t e = hipMalloc(&g->d_mem, sat_part + table_part + P.gz_bytes);
   if (e != hipSuccess) {
     delete g;
     return fail(c, MQPIV_ERR_OOM, "graph memory: %s", hipGetErrorString(e));
   }
   char* sat_base = (char*)g->d_mem;
   char* d_table = sat_base + sat_part;
+  char* gz = P.gz_bytes ? d_table + table_part : nullptr;
   std::vector<char> host(P.table_bytes + 1);
   fill_tables(P, sat_base, host.data());
   e = hipMemcpy(d_table, host.data(), P.table_bytes, hipMemcpyHostToDevice);
@@ -789,7 +804,7 @@ extern "C" int mqpiv_graph_create(mqpiv_ctx* c, const mqpiv_request* reqs, int32
   }
   const bool prof = c->profiling;
   c->profiling = false;
-  rc = enqueue_2d(c, P, sat_base, d_table, c->stream);
+  rc = enqueue_2d(c, P, sat_base, d_table, gz, c->stream);
   c->profiling = prof;
   e = hipStreamEndCapture(c->stream, &g->graph);
   if (rc == MQPIV_OK && e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);

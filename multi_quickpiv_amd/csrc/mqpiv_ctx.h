@@ -35,6 +35,9 @@ struct mqpiv_ctx {
   // post-processing filters: reduction partials + a magnitude map
   void* d_pp = nullptr;
   size_t pp_cap = 0;
+  // global transform buffers of the 2-D lengths that do not fit in LDS
+  void* d_gz = nullptr;
+  size_t gz_cap = 0;
   // multipass: two coarse vector fields (FP64) and one pass's search-region shifts
   void* d_mp = nullptr;
   size_t mp_cap = 0;

@@ -5,6 +5,7 @@
 
 #include <mutex>
 #include <set>
+#include <type_traits>
 #include <utility>
 
 namespace mqpiv {
@@ -52,6 +53,33 @@ struct Batch {
   int32_t pad_;
   int64_t n_ia_total;
 };
+
+// 2-D transform lengths whose buffer lives in global memory instead of LDS:
+// FP64 at 128 needs 128 x 129 x 16 B = 258 KB, more than a CU's 160 KB of LDS.
+// Such a launch is a persistent grid of at most kGzBlocksPerCu blocks per CU,
+// one IA per block at a time; each block owns one buffer of gz_stride_2d
+// bytes, which the L2 and the Infinity Cache hold while the block works on it.
+template <int L, typename T>
+constexpr bool gz_len() {
+  return std::is_same<T, double>::value && L == 128;
+}
+template <int L, typename T>
+constexpr size_t gz_stride_2d() {
+  return ((size_t)L * (L + 1) * 2 * sizeof(T) + 4095) & ~(size_t)4095;
+}
+constexpr int kGzBlocksPerCu = 2;
+
+// Compute units of the current device (cached per device).
+inline int device_cus() {
+  static std::mutex mu;
+  static int cus[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!cus[dev] && hipDeviceGetAttribute(&cus[dev], hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    cus[dev] = 256;
+  return cus[dev] > 0 ? cus[dev] : 256;
+}
 
 // Raises a kernel's dynamic-LDS limit to the CU's 160 KB on the current device,
 // once per (kernel, device); safe to call from any thread before a launch.

@@ -1,0 +1,10 @@
+// mqpiv_k2d_f64_128.hip -- instances of the fused 2-D kernel:
+// double transforms of length 128 (lanes per line: mqpiv_lpl2d.h).
+#include "mqpiv_kernels2d.hpp"
+#include "mqpiv_lpl2d.h"
+
+namespace mqpiv {
+template size_t lds_bytes_2d<128, double, Lpl2<double, 128>::v>(int, int, int, int);
+template hipError_t launch_2d_l<128, double, Lpl2<double, 128>::v>(int, const Job&, const Job*, const int64_t*, int, int64_t,
+                                                int, int, int, char*, hipStream_t);
+}  // namespace mqpiv

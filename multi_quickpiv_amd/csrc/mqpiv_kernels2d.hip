@@ -13,46 +13,49 @@ template <int L, typename T, int LPL>
 size_t lds_bytes_2d(int alg, int ns0, int ns1, int ss0);
 template <int L, typename T, int LPL>
 hipError_t launch_2d_l(int alg, const Job& j0, const Job* d_jobs, const int64_t* d_prefix, int n_jobs,
-                       int64_t n_ia_total, int ns0, int ns1, int ss0, hipStream_t s);
+                       int64_t n_ia_total, int ns0, int ns1, int ss0, char* gz, hipStream_t s);
 extern template size_t lds_bytes_2d<16, float, Lpl2<float, 16>::v>(int, int, int, int);
 extern template hipError_t launch_2d_l<16, float, Lpl2<float, 16>::v>(int, const Job&, const Job*, const int64_t*, int,
-                                                    int64_t, int, int, int, hipStream_t);
+                                                    int64_t, int, int, int, char*, hipStream_t);
 extern template size_t lds_bytes_2d<32, float, Lpl2<float, 32>::v>(int, int, int, int);
 extern template hipError_t launch_2d_l<32, float, Lpl2<float, 32>::v>(int, const Job&, const Job*, const int64_t*, int,
-                                                    int64_t, int, int, int, hipStream_t);
+                                                    int64_t, int, int, int, char*, hipStream_t);
 extern template size_t lds_bytes_2d<48, float, Lpl2<float, 48>::v>(int, int, int, int);
 extern template hipError_t launch_2d_l<48, float, Lpl2<float, 48>::v>(int, const Job&, const Job*, const int64_t*, int,
-                                                    int64_t, int, int, int, hipStream_t);
+                                                    int64_t, int, int, int, char*, hipStream_t);
 extern template size_t lds_bytes_2d<64, float, Lpl2<float, 64>::v>(int, int, int, int);
 extern template hipError_t launch_2d_l<64, float, Lpl2<float, 64>::v>(int, const Job&, const Job*, const int64_t*, int,
-                                                    int64_t, int, int, int, hipStream_t);
+                                                    int64_t, int, int, int, char*, hipStream_t);
 extern template size_t lds_bytes_2d<96, float, Lpl2<float, 96>::v>(int, int, int, int);
 extern template hipError_t launch_2d_l<96, float, Lpl2<float, 96>::v>(int, const Job&, const Job*, const int64_t*, int,
-                                                    int64_t, int, int, int, hipStream_t);
+                                                    int64_t, int, int, int, char*, hipStream_t);
 extern template size_t lds_bytes_2d<128, float, Lpl2<float, 128>::v>(int, int, int, int);
 extern template hipError_t launch_2d_l<128, float, Lpl2<float, 128>::v>(int, const Job&, const Job*, const int64_t*, int,
-                                                    int64_t, int, int, int, hipStream_t);
+                                                    int64_t, int, int, int, char*, hipStream_t);
 extern template size_t lds_bytes_2d<16, double, Lpl2<double, 16>::v>(int, int, int, int);
 extern template hipError_t launch_2d_l<16, double, Lpl2<double, 16>::v>(int, const Job&, const Job*, const int64_t*, int,
-                                                    int64_t, int, int, int, hipStream_t);
+                                                    int64_t, int, int, int, char*, hipStream_t);
 extern template size_t lds_bytes_2d<32, double, Lpl2<double, 32>::v>(int, int, int, int);
 extern template hipError_t launch_2d_l<32, double, Lpl2<double, 32>::v>(int, const Job&, const Job*, const int64_t*, int,
-                                                    int64_t, int, int, int, hipStream_t);
+                                                    int64_t, int, int, int, char*, hipStream_t);
 extern template size_t lds_bytes_2d<48, double, Lpl2<double, 48>::v>(int, int, int, int);
 extern template hipError_t launch_2d_l<48, double, Lpl2<double, 48>::v>(int, const Job&, const Job*, const int64_t*, int,
-                                                    int64_t, int, int, int, hipStream_t);
+                                                    int64_t, int, int, int, char*, hipStream_t);
 extern template size_t lds_bytes_2d<64, double, Lpl2<double, 64>::v>(int, int, int, int);
 extern template hipError_t launch_2d_l<64, double, Lpl2<double, 64>::v>(int, const Job&, const Job*, const int64_t*, int,
-                                                    int64_t, int, int, int, hipStream_t);
+                                                    int64_t, int, int, int, char*, hipStream_t);
 extern template size_t lds_bytes_2d<96, double, Lpl2<double, 96>::v>(int, int, int, int);
 extern template hipError_t launch_2d_l<96, double, Lpl2<double, 96>::v>(int, const Job&, const Job*, const int64_t*, int,
-                                                    int64_t, int, int, int, hipStream_t);
+                                                    int64_t, int, int, int, char*, hipStream_t);
+extern template size_t lds_bytes_2d<128, double, Lpl2<double, 128>::v>(int, int, int, int);
+extern template hipError_t launch_2d_l<128, double, Lpl2<double, 128>::v>(int, const Job&, const Job*, const int64_t*, int,
+                                                    int64_t, int, int, int, char*, hipStream_t);
 
 // Transform length for a search size: the smallest supported length >= ss
-// (FP32: up to 128, FP64: up to 96), or -1.
+// (up to 128; FP64 at 128 keeps its transform buffer in global memory), or -1.
 int fft_len_2d(int fp64, int ss) {
   static const int f32[] = {16, 32, 48, 64, 96, 128};
-  static const int f64[] = {16, 32, 48, 64, 96};
+  static const int f64[] = {16, 32, 48, 64, 96, 128};
   if (fp64) {
     for (int l : f64)
       if (l >= ss) return l;
@@ -71,6 +74,7 @@ size_t lds_2d(int fp64, int L, int alg, int ns0, int ns1, int ss0) {
       case 48: return lds_bytes_2d<48, double, Lpl2<double, 48>::v>(alg, ns0, ns1, ss0);
       case 64: return lds_bytes_2d<64, double, Lpl2<double, 64>::v>(alg, ns0, ns1, ss0);
       case 96: return lds_bytes_2d<96, double, Lpl2<double, 96>::v>(alg, ns0, ns1, ss0);
+      case 128: return lds_bytes_2d<128, double, Lpl2<double, 128>::v>(alg, ns0, ns1, ss0);
     }
   } else {
     switch (L) {
@@ -85,25 +89,35 @@ size_t lds_2d(int fp64, int L, int alg, int ns0, int ns1, int ss0) {
   return (size_t)-1;
 }
 
+// Bytes of global transform buffers a launch of n_ia IAs at (fp64, L) needs:
+// one per resident block (at most kGzBlocksPerCu per CU, and no more than the
+// IAs), or 0 for lengths whose buffer is in LDS.
+size_t gz_scratch_2d(int fp64, int L, int64_t n_ia) {
+  if (!(fp64 && gz_len<128, double>() && L == 128) || n_ia <= 0) return 0;
+  const int64_t cap = (int64_t)kGzBlocksPerCu * device_cus();
+  return (size_t)(n_ia < cap ? n_ia : cap) * gz_stride_2d<128, double>();
+}
+
 hipError_t launch_piv2d(int fp64, int L, int alg, const Job& j0, const Job* d_jobs, const int64_t* d_prefix,
-                        int n_jobs, int64_t n_ia_total, int ns0, int ns1, int ss0, hipStream_t s) {
+                        int n_jobs, int64_t n_ia_total, int ns0, int ns1, int ss0, void* gz, hipStream_t s) {
   if (n_ia_total <= 0) return hipSuccess;
   if (fp64) {
     switch (L) {
-      case 16: return launch_2d_l<16, double, Lpl2<double, 16>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, s);
-      case 32: return launch_2d_l<32, double, Lpl2<double, 32>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, s);
-      case 48: return launch_2d_l<48, double, Lpl2<double, 48>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, s);
-      case 64: return launch_2d_l<64, double, Lpl2<double, 64>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, s);
-      case 96: return launch_2d_l<96, double, Lpl2<double, 96>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, s);
+      case 16: return launch_2d_l<16, double, Lpl2<double, 16>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, (char*)gz, s);
+      case 32: return launch_2d_l<32, double, Lpl2<double, 32>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, (char*)gz, s);
+      case 48: return launch_2d_l<48, double, Lpl2<double, 48>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, (char*)gz, s);
+      case 64: return launch_2d_l<64, double, Lpl2<double, 64>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, (char*)gz, s);
+      case 96: return launch_2d_l<96, double, Lpl2<double, 96>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, (char*)gz, s);
+      case 128: return launch_2d_l<128, double, Lpl2<double, 128>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, (char*)gz, s);
     }
   } else {
     switch (L) {
-      case 16: return launch_2d_l<16, float, Lpl2<float, 16>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, s);
-      case 32: return launch_2d_l<32, float, Lpl2<float, 32>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, s);
-      case 48: return launch_2d_l<48, float, Lpl2<float, 48>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, s);
-      case 64: return launch_2d_l<64, float, Lpl2<float, 64>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, s);
-      case 96: return launch_2d_l<96, float, Lpl2<float, 96>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, s);
-      case 128: return launch_2d_l<128, float, Lpl2<float, 128>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, s);
+      case 16: return launch_2d_l<16, float, Lpl2<float, 16>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, (char*)gz, s);
+      case 32: return launch_2d_l<32, float, Lpl2<float, 32>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, (char*)gz, s);
+      case 48: return launch_2d_l<48, float, Lpl2<float, 48>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, (char*)gz, s);
+      case 64: return launch_2d_l<64, float, Lpl2<float, 64>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, (char*)gz, s);
+      case 96: return launch_2d_l<96, float, Lpl2<float, 96>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, (char*)gz, s);
+      case 128: return launch_2d_l<128, float, Lpl2<float, 128>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, (char*)gz, s);
     }
   }
   return hipErrorInvalidValue;

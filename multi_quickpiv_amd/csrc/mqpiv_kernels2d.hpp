@@ -58,6 +58,44 @@ template <typename T>
 __device__ __forceinline__ gptr<T> as_global(const void* p) {
   return (gptr<T>)p;
 }
+// Writable global-address-space pointer: the transform buffer of the lengths
+// whose buffer does not fit in LDS lives in global memory (see piv2d_kernel).
+template <typename T>
+using gmut = T __attribute__((address_space(1)))*;
+// The real view of a complex transform buffer, in the buffer's address space.
+template <typename T>
+__device__ __forceinline__ T* as_real(cpx<T>* p) {
+  return reinterpret_cast<T*>(p);
+}
+template <typename T>
+__device__ __forceinline__ gmut<T> as_real(gmut<cpx<T>> p) {
+  return (gmut<T>)p;
+}
+// Complex loads and stores of a transform buffer: struct copies in LDS, one
+// vector access per value in global memory (where a struct cannot be assigned
+// through the address-space-qualified pointer).
+template <typename T>
+using vec2_t = T __attribute__((ext_vector_type(2)));
+template <typename T>
+__device__ __forceinline__ cpx<T> zld(const cpx<T>* Z, int i) {
+  return Z[i];
+}
+template <typename T>
+__device__ __forceinline__ void zst(cpx<T>* Z, int i, cpx<T> v) {
+  Z[i] = v;
+}
+template <typename T>
+__device__ __forceinline__ cpx<T> zld(gmut<cpx<T>> Z, int i) {
+  const vec2_t<T> v = ((gmut<vec2_t<T>>)Z)[i];
+  return cpx<T>{v.x, v.y};
+}
+template <typename T>
+__device__ __forceinline__ void zst(gmut<cpx<T>> Z, int i, cpx<T> v) {
+  vec2_t<T> w;
+  w.x = v.x;
+  w.y = v.y;
+  ((gmut<vec2_t<T>>)Z)[i] = w;
+}
 
 __device__ __forceinline__ double load_px(const void* img, int dt, int64_t i) {
   return dt == DT_F32 ? (double)((const float*)img)[i] : ((const double*)img)[i];
@@ -374,8 +412,8 @@ __host__ __device__ constexpr int c2r_wide_lanes() {
   return (P <= 16 && P <= TPI && has_plan(L / P)) ? P : c2r_lanes<L, LPL, TPI>();
 }
 
-template <int L, int LPL, typename T>
-__device__ __forceinline__ void correlate_2d(cpx<T> (&x)[L / LPL], cpx<T>* __restrict__ Z, int t, int ns0,
+template <int L, int LPL, typename T, typename ZP>
+__device__ __forceinline__ void correlate_2d(cpx<T> (&x)[L / LPL], ZP __restrict__ Z, int t, int ns0,
                                              int ns1) {
   constexpr int LD = L + 1;
   constexpr int M = L / LPL;
@@ -388,16 +426,16 @@ __device__ __forceinline__ void correlate_2d(cpx<T> (&x)[L / LPL], cpx<T>* __res
   if (lane) {
     line_fwd<L, LPL, T>(x, q);
 #pragma unroll
-    for (int k = 0; k < M; ++k) Z[(cM + k) * LD + line] = x[k];
+    for (int k = 0; k < M; ++k) zst(Z, (cM + k) * LD + line, x[k]);
   }
   __syncthreads();
   // --- forward along dim 1: line = k1
   if (lane) {
 #pragma unroll
-    for (int k = 0; k < M; ++k) x[k] = Z[line * LD + LPL * k + q];
+    for (int k = 0; k < M; ++k) x[k] = zld(Z, line * LD + LPL * k + q);
     line_fwd<L, LPL, T>(x, q);
 #pragma unroll
-    for (int k = 0; k < M; ++k) Z[line * LD + cM + k] = x[k];
+    for (int k = 0; k < M; ++k) zst(Z, line * LD + cM + k, x[k]);
   }
   __syncthreads();
   // The correlation is real, so P is Hermitian: the dim-1 inverse of column
@@ -414,7 +452,7 @@ __device__ __forceinline__ void correlate_2d(cpx<T> (&x)[L / LPL], cpx<T>* __res
       const int k0 = cM + k;
       const int k0n = (L - k0) % L;
       cpx<T> z = x[k];
-      cpx<T> n = Z[k1n * LD + k0n];
+      cpx<T> n = zld(Z, k1n * LD + k0n);
       T ar = T(0.5) * (z.x + n.x), ai = T(0.5) * (z.y - n.y);
       T br = T(0.5) * (z.y + n.y), bi = T(0.5) * (n.x - z.x);
       x[k] = cpx<T>{ar * br + ai * bi, ar * bi - ai * br};
@@ -427,7 +465,7 @@ __device__ __forceinline__ void correlate_2d(cpx<T> (&x)[L / LPL], cpx<T>* __res
 #pragma unroll
     for (int m = 0; m < M; ++m) {
       const int s0 = LPL * m + q;
-      if (s0 < ns0) Z[line * LD + s0] = x[m];
+      if (s0 < ns0) zst(Z, line * LD + s0, x[m]);
     }
   }
   __syncthreads();
@@ -449,8 +487,8 @@ __device__ __forceinline__ void correlate_2d(cpx<T> (&x)[L / LPL], cpx<T>* __res
         const int k1 = cM2 + k;
         const bool mirror = k1 >= KH;
         const int kc = mirror ? L - k1 : k1;
-        const cpx<T> ya = Z[kc * LD + sa];
-        const cpx<T> yb = has_b ? Z[kc * LD + sb] : cpx<T>{T(0), T(0)};
+        const cpx<T> ya = zld(Z, kc * LD + sa);
+        const cpx<T> yb = has_b ? zld(Z, kc * LD + sb) : cpx<T>{T(0), T(0)};
         // Y(s, k1) = conj(Y(s, L - k1)) above L/2
         const T ayi = mirror ? -ya.y : ya.y, byi = mirror ? -yb.y : yb.y;
         y[k] = cpx<T>{ya.x - byi, ayi + yb.x};
@@ -459,7 +497,7 @@ __device__ __forceinline__ void correlate_2d(cpx<T> (&x)[L / LPL], cpx<T>* __res
     }
     __syncthreads();
     if (act) {
-      T* S = reinterpret_cast<T*>(Z);
+      auto S = as_real<T>(Z);
       const T kScale = T(1) / T(L * L);
       const int sa = 2 * line2, sb = 2 * line2 + 1;
 #pragma unroll
@@ -505,6 +543,9 @@ constexpr int min_waves_2d() {
   // the matrix-core form holds many operand fragments: 4 waves per SIMD (128 VGPRs)
   // at length 64; at 96 one 12-wave IA fills a CU either way, so 3 (168 VGPRs)
   if constexpr (MF) return L == 96 ? 3 : 4;
+  // global transform buffer: one IA's block per CU, whose waves all fit the
+  // SIMDs together (16 waves: 4 per SIMD, 128 VGPRs)
+  if constexpr (gz_len<L, T>()) return Cfg2<L, T, LPL>::NT / 256 > 0 ? Cfg2<L, T, LPL>::NT / 256 : 1;
   using C = Cfg2<L, T, LPL>;
   constexpr size_t z = ((size_t)L * (L + 1) * 2 * sizeof(T) + 15) & ~(size_t)15;
   constexpr size_t c1 = ALG == ALG_MASK_NSQECC ? (((size_t)(L / 2 + 1) * (L / 2 + 1) * sizeof(T) + 15) & ~(size_t)15) : 0;
@@ -518,17 +559,26 @@ constexpr int min_waves_2d() {
 // multipass); a separate instantiation, so that kernels without shifts carry
 // none of its registers.
 // MF: the split-precision matrix-core correlation (mqpiv_mfma2d.hpp) in place of
-// the register FFTs; length 64, FP32, four lanes a line, no shifts.
+// the register FFTs; FP32 lengths 64 and 96 (4 or 8 lanes a line, 4 or 12
+// waves an IA, windows up to L x L), no shifts.
+// gz: the global transform buffers of a gz_len launch (persistent grid), else unused.
 template <int L, int ALG, typename T, int LPL, bool SH, bool MF = false>
 __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL, MF>())) void piv2d_kernel(Job j0, const Job* __restrict__ jobs,
                                                                       const int64_t* __restrict__ prefix,
                                                                       int n_jobs, int64_t n_ia_total, int ns0_cap,
-                                                                      int ns1_cap) {
+                                                                      int ns1_cap, char* gz) {
   using C = Cfg2<L, T, LPL>;
   constexpr int H = C::H;
+  constexpr bool GZ = gz_len<L, T>();
+  static_assert(!GZ || (C::IPB == 1 && !MF), "a global transform buffer holds one IA");
   extern __shared__ __align__(16) unsigned char smem[];
 
-  for (;;) {  // runs once (a loop keeps the register allocation of this body as measured)
+  // One pass per block, except with global transform buffers: then the grid is
+  // persistent and block b takes IAs r, r + grid, ... (r the XCD-remapped b), so
+  // that each XCD works through contiguous runs of IAs.  The pass count is
+  // uniform over the block (every barrier below is reached by all its waves).
+  const int32_t n_vb = GZ ? (int32_t)n_ia_total : (int32_t)gridDim.x;
+  for (int32_t vb = xcd_remap32((int32_t)blockIdx.x, (int32_t)gridDim.x); vb < n_vb; vb += (int32_t)gridDim.x) {
   const int tid = (int)threadIdx.x;
   // one IA per block: every IA quantity below is uniform over the block, which
   // lets the compiler keep it in scalar registers
@@ -539,8 +589,7 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
 
   // IA indices in 32-bit arithmetic (the host keeps a launch below 2^31 IAs)
   const int32_t nt = (int32_t)n_ia_total;
-  const int32_t b = xcd_remap32((int32_t)blockIdx.x, (int32_t)gridDim.x);
-  const int32_t ia_g = b * C::IPB + sub;
+  const int32_t ia_g = vb * C::IPB + sub;
   const bool valid = ia_g < nt;
   const int32_t ia_c = valid ? ia_g : nt - 1;
   Job J = j0;
@@ -561,12 +610,15 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
   // their margins): the host passes the largest ns0, ns1 of the launch
   static_assert(!MF || (mf_ok<L, T, LPL>() && !SH && ALG != ALG_MASK_NSQECC),
                 "matrix-core form: FP32 lengths 64 and 96, unshifted, unmasked");
-  const size_t zbytes = MF ? mf_bytes<MF, L>() : region_bytes_2d(L, sizeof(T), ns0_cap, ns1_cap, ss0);
+  const size_t zbytes = GZ ? 0 : (MF ? mf_bytes<MF, L>() : region_bytes_2d(L, sizeof(T), ns0_cap, ns1_cap, ss0));
   const size_t c1bytes = (ALG == ALG_MASK_NSQECC) ? sim_bytes(ns0_cap, ns1_cap, sizeof(T)) : 0;
   const size_t per_ia = zbytes + c1bytes;
   unsigned char* base = smem + (size_t)sub * per_ia;
-  cpx<T>* Z = reinterpret_cast<cpx<T>*>(base);
-  T* S = reinterpret_cast<T*>(base);
+  using ZPtr = std::conditional_t<GZ, gmut<cpx<T>>, cpx<T>*>;
+  ZPtr Z;
+  if constexpr (GZ) Z = (ZPtr)(gz + (size_t)blockIdx.x * gz_stride_2d<L, T>());
+  else Z = reinterpret_cast<cpx<T>*>(base);
+  const auto S = as_real<T>(Z);
   T* C1 = reinterpret_cast<T*>(base + zbytes);
   double* red = reinterpret_cast<double*>(smem + (size_t)C::IPB * per_ia + (size_t)sub * kRedBytes);
 
@@ -643,18 +695,26 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
   cpx<T> x[H];
   double sumF2 = 0.0, mx1 = -INFINITY, mx2 = -INFINITY;
   const bool want_max = J.has_threshold || J.has_mask_threshold;
+  // Values are loaded in batches of LB per lane: all of a lane's columns at
+  // once, except with global transform buffers, whose long lines would hold too
+  // many loaded values in registers (there ZNCC loads its window twice: once
+  // for the means, once for the packed window).
+  constexpr int LB = GZ ? 8 : H;
+  static_assert(H % LB == 0, "whole batches");
   auto load = [&](auto tin) {
     using TIN = decltype(tin);
-    TIN fv[H], gv[H], mv[H];
+    TIN fv[LB], gv[LB], mv[LB];
+    auto fetch = [&](int c0) {
 #pragma unroll
-    for (int k = 0; k < H; ++k) {
-      fv[k] = ld_f(J.img1, k, tin);
-      gv[k] = ld_g(J.img2, k, tin);
-    }
-    if constexpr (ALG == ALG_MASK_NSQECC) {
+      for (int j = 0; j < LB; ++j) {
+        fv[j] = ld_f(J.img1, c0 + j, tin);
+        gv[j] = ld_g(J.img2, c0 + j, tin);
+      }
+      if constexpr (ALG == ALG_MASK_NSQECC) {
 #pragma unroll
-      for (int k = 0; k < H; ++k) mv[k] = ld_f(J.mask, k, tin);
-    }
+        for (int j = 0; j < LB; ++j) mv[j] = ld_f(J.mask, c0 + j, tin);
+      }
+    };
     if constexpr (ALG == ALG_ZNCC) {
       // sums of F, G and F^2 in one reduction; sum((F - mean)^2) follows from
       // them (exact algebra; it differs from summing the squares of the
@@ -662,12 +722,16 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
       // which leaves the peak and SN unchanged)
       double sf = 0.0, sg = 0.0, sff = 0.0;
 #pragma unroll
-      for (int k = 0; k < H; ++k) {
-        const int k1 = k1_of(k);
-        const double f = f_at(k1) ? (double)fv[k] : 0.0;
-        sf += f;
-        sff += f * f;
-        if (g_at(k1)) sg += (double)gv[k];
+      for (int c0 = 0; c0 < H; c0 += LB) {
+        fetch(c0);
+#pragma unroll
+        for (int j = 0; j < LB; ++j) {
+          const int k1 = k1_of(c0 + j);
+          const double f = f_at(k1) ? (double)fv[j] : 0.0;
+          sf += f;
+          sff += f * f;
+          if (g_at(k1)) sg += (double)gv[j];
+        }
       }
       Group<C::TPI>::template reduce_n<3>(sf, sg, sff, OpAdd{}, OpAdd{}, OpAdd{}, red, t);
       const int64_t nF = (int64_t)is0 * is1;
@@ -681,32 +745,37 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
       sumF2 = v > 0.0 ? v : 0.0;
     }
 #pragma unroll
-    for (int k = 0; k < H; ++k) {
-      const int k1 = k1_of(k);
-      double a = 0.0, g = 0.0;
-      if (f_at(k1)) {
-        const TIN f = fv[k];
-        if (want_max) mx1 = fmax(mx1, (double)f);
-        if constexpr (ALG == ALG_ZNCC) {
-          a = std::is_same<TIN, float>::value ? (double)((float)f - meanF32) : (double)f - meanF64;
-        } else if constexpr (ALG == ALG_MASK_NSQECC) {
-          const double m = (double)mv[k];
-          if (want_max) mx2 = fmax(mx2, m);
-          a = (double)f * m;  // masked_nsqecc_ops.jl:19-25
-          const double wi = m > 0 ? m : 1.0;
-          sumF2 += a * a / wi;  // mask_nsqecc.jl:144-149
-        } else {
-          a = (double)f;
-          if constexpr (ALG != ALG_FFTCC) sumF2 += a * a;
+    for (int c0 = 0; c0 < H; c0 += LB) {
+      if (ALG != ALG_ZNCC || LB != H) fetch(c0);  // one batch: ZNCC reuses it
+#pragma unroll
+      for (int j = 0; j < LB; ++j) {
+        const int k = c0 + j;
+        const int k1 = k1_of(k);
+        double a = 0.0, g = 0.0;
+        if (f_at(k1)) {
+          const TIN f = fv[j];
+          if (want_max) mx1 = fmax(mx1, (double)f);
+          if constexpr (ALG == ALG_ZNCC) {
+            a = std::is_same<TIN, float>::value ? (double)((float)f - meanF32) : (double)f - meanF64;
+          } else if constexpr (ALG == ALG_MASK_NSQECC) {
+            const double m = (double)mv[j];
+            if (want_max) mx2 = fmax(mx2, m);
+            a = (double)f * m;  // masked_nsqecc_ops.jl:19-25
+            const double wi = m > 0 ? m : 1.0;
+            sumF2 += a * a / wi;  // mask_nsqecc.jl:144-149
+          } else {
+            a = (double)f;
+            if constexpr (ALG != ALG_FFTCC) sumF2 += a * a;
+          }
         }
+        if (g_at(k1)) {
+          if constexpr (ALG == ALG_ZNCC)
+            g = std::is_same<TIN, float>::value ? (double)((float)gv[j] - meanG32) : (double)gv[j] - meanG64;
+          else
+            g = (double)gv[j];
+        }
+        x[k] = cpx<T>{(T)a, (T)g};
       }
-      if (g_at(k1)) {
-        if constexpr (ALG == ALG_ZNCC)
-          g = std::is_same<TIN, float>::value ? (double)((float)gv[k] - meanG32) : (double)gv[k] - meanG64;
-        else
-          g = (double)gv[k];
-      }
-      x[k] = cpx<T>{(T)a, (T)g};
     }
   };
   if (dt == DT_F32) load(float{});
@@ -733,22 +802,26 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
     // second correlation: mask (x) G^2 (mask_nsqecc.jl:155)
     auto load2 = [&](auto tin) {
       using TIN = decltype(tin);
-      TIN mv[H], gv[H];
 #pragma unroll
-      for (int k = 0; k < H; ++k) {
-        mv[k] = ld_f(J.mask, k, tin);
-        gv[k] = ld_g(J.img2, k, tin);
-      }
+      for (int c0 = 0; c0 < H; c0 += LB) {
+        TIN mv[LB], gv[LB];
 #pragma unroll
-      for (int k = 0; k < H; ++k) {
-        const int k1 = k1_of(k);
-        double m = 0.0, g2 = 0.0;
-        if (f_at(k1)) m = (double)mv[k];
-        if (g_at(k1)) {
-          const double g = (double)gv[k];
-          g2 = g * g;
+        for (int j = 0; j < LB; ++j) {
+          mv[j] = ld_f(J.mask, c0 + j, tin);
+          gv[j] = ld_g(J.img2, c0 + j, tin);
         }
-        x[k] = cpx<T>{(T)m, (T)g2};
+#pragma unroll
+        for (int j = 0; j < LB; ++j) {
+          const int k = c0 + j;
+          const int k1 = k1_of(k);
+          double m = 0.0, g2 = 0.0;
+          if (f_at(k1)) m = (double)mv[j];
+          if (g_at(k1)) {
+            const double g = (double)gv[j];
+            g2 = g * g;
+          }
+          x[k] = cpx<T>{(T)m, (T)g2};
+        }
       }
     };
     if (dt == DT_F32) load2(float{});
@@ -995,7 +1068,10 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
       if (J.sn) ((double*)J.sn)[ia] = sn;
     }
   }
-  break;
+  if constexpr (!GZ) break;
+  // the next pass rewrites this block's buffer and LDS slots: every lane has
+  // finished reading them (the fit above reads the window) before any writes
+  __syncthreads();
   }
 }
 
@@ -1005,19 +1081,31 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
 template <int L, typename T, int LPL>
 size_t lds_bytes_2d(int alg, int ns0, int ns1, int ss0) {
   using C = Cfg2<L, T, LPL>;
-  size_t z = region_bytes_2d(L, sizeof(T), ns0, ns1, ss0);
+  size_t z = gz_len<L, T>() ? 0 : region_bytes_2d(L, sizeof(T), ns0, ns1, ss0);
   size_t c1 = (alg == ALG_MASK_NSQECC) ? sim_bytes(ns0, ns1, sizeof(T)) : 0;
   return (z + c1) * C::IPB + (size_t)C::IPB * kRedBytes;
 }
 
 template <int L, int ALG, typename T, int LPL, bool SH, bool MF = false>
 hipError_t launch_2d_k(const Job& j0, const Job* d_jobs, const int64_t* d_prefix, int n_jobs, int64_t n_ia_total,
-                       int ns0, int ns1, size_t lds, hipStream_t s) {
+                       int ns0, int ns1, size_t lds, char* gz, hipStream_t s) {
   using C = Cfg2<L, T, LPL>;
-  allow_full_lds((const void*)piv2d_kernel<L, ALG, T, LPL, SH, MF>);
-  const int64_t nb = (n_ia_total + C::IPB - 1) / C::IPB;
+  const void* fn = (const void*)piv2d_kernel<L, ALG, T, LPL, SH, MF>;
+  allow_full_lds(fn);
+  int64_t nb = (n_ia_total + C::IPB - 1) / C::IPB;
+  if constexpr (gz_len<L, T>()) {
+    // persistent grid: the blocks that stay resident, within the buffers the
+    // caller provides (gz_scratch_2d)
+    if (!gz) return hipErrorInvalidValue;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, C::NT, lds) != hipSuccess || per_cu < 1)
+      per_cu = 1;
+    per_cu = per_cu < kGzBlocksPerCu ? per_cu : kGzBlocksPerCu;
+    const int64_t cap = (int64_t)per_cu * device_cus();
+    nb = nb < cap ? nb : cap;
+  }
   hipLaunchKernelGGL((piv2d_kernel<L, ALG, T, LPL, SH, MF>), dim3((unsigned)nb), dim3(C::NT), lds, s, j0, d_jobs,
-                     d_prefix, n_jobs, n_ia_total, ns0, ns1);
+                     d_prefix, n_jobs, n_ia_total, ns0, ns1, gz);
   return hipGetLastError();
 }
 
@@ -1025,13 +1113,13 @@ hipError_t launch_2d_k(const Job& j0, const Job* d_jobs, const int64_t* d_prefix
 // instantiation (batches never carry shifts).
 template <int L, int ALG, typename T, int LPL>
 hipError_t launch_2d_t(const Job& j0, const Job* d_jobs, const int64_t* d_prefix, int n_jobs,
-                       int64_t n_ia_total, int ns0, int ns1, int ss0, hipStream_t s) {
+                       int64_t n_ia_total, int ns0, int ns1, int ss0, char* gz, hipStream_t s) {
   if (j0.precision_mode == kPrecSplitMfma) {
     // single unshifted jobs only (the host checks); window up to 64 x 64
     if constexpr (mf_ok<L, T, LPL>() && ALG != ALG_MASK_NSQECC) {
       if (d_jobs || j0.shift || ns0 > L || ns1 > L) return hipErrorInvalidValue;
       return launch_2d_k<L, ALG, T, LPL, false, true>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1,
-                                                      mf_bytes<true, L>() + kRedBytes, s);
+                                                      mf_bytes<true, L>() + kRedBytes, gz, s);
     } else {
       return hipErrorInvalidValue;
     }
@@ -1039,23 +1127,23 @@ hipError_t launch_2d_t(const Job& j0, const Job* d_jobs, const int64_t* d_prefix
   size_t lds = lds_bytes_2d<L, T, LPL>(ALG, ns0, ns1, ss0);
   if (lds > 160 * 1024) return hipErrorInvalidValue;
   if (j0.shift && !d_jobs)
-    return launch_2d_k<L, ALG, T, LPL, true>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, lds, s);
-  return launch_2d_k<L, ALG, T, LPL, false>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, lds, s);
+    return launch_2d_k<L, ALG, T, LPL, true>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, lds, gz, s);
+  return launch_2d_k<L, ALG, T, LPL, false>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, lds, gz, s);
 }
 
 template <int L, typename T, int LPL>
 hipError_t launch_2d_l(int alg, const Job& j0, const Job* d_jobs, const int64_t* d_prefix, int n_jobs,
-                       int64_t n_ia_total, int ns0, int ns1, int ss0, hipStream_t s) {
+                       int64_t n_ia_total, int ns0, int ns1, int ss0, char* gz, hipStream_t s) {
   switch (alg) {
     case ALG_FFTCC:
-      return launch_2d_t<L, ALG_FFTCC, T, LPL>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, s);
+      return launch_2d_t<L, ALG_FFTCC, T, LPL>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, gz, s);
     case ALG_ZNCC:
-      return launch_2d_t<L, ALG_ZNCC, T, LPL>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, s);
+      return launch_2d_t<L, ALG_ZNCC, T, LPL>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, gz, s);
     case ALG_NSQECC:
-      return launch_2d_t<L, ALG_NSQECC, T, LPL>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, s);
+      return launch_2d_t<L, ALG_NSQECC, T, LPL>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, gz, s);
     case ALG_MASK_NSQECC:
       return launch_2d_t<L, ALG_MASK_NSQECC, T, LPL>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1,
-                                                    ss0, s);
+                                                    ss0, gz, s);
   }
   return hipErrorInvalidValue;
 }

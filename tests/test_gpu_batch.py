@@ -29,6 +29,8 @@ def _mixed_requests():
         ((384, 320), "mask_nsqecc", 32, 16, 16, "mask"),
         ((256, 288), "mask_nsqecc", 16, 8, 8, "mask"),
         ((320, 256), "nsqecc", 64, 32, 32, None),     # L=128
+        ((320, 288), "mask_nsqecc", 64, 32, 32, "mask"),  # FP64 L=128: global transform buffers
+        ((288, 320), "zncc", 64, 32, 16, None),       # FP64 L=128 (ZNCC with SN)
     ]
     for i, (shape, alg, isz, sm, st, mk) in enumerate(specs):
         a, b = _pair(shape, (1.5 + 0.3 * i, -2.0 + 0.5 * i), 100 + i)

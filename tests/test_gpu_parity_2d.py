@@ -56,9 +56,8 @@ def test_algorithms_all_lengths(oracle, alg, isz, sm, st, mode):
     full bar for FFTCC and NSQECC; for ZNCC their SN is not (the window sum of
     ZNCC values nearly cancels, so SN is ill-conditioned) and for masked NSQECC
     the num = sum(F^2) + C2 - 2 C1 cancellation flips peaks -- "auto" therefore
-    runs those two in FP64."""
-    if mode == "fp64" and isz + 2 * sm > 96:
-        pytest.skip("FP64 transforms cover search sizes up to 96")
+    runs those two in FP64, at search size 128 (IA 64, margin 32) with the
+    transform buffer in global memory."""
     if mode == "fp32" and alg == "mask_nsqecc":
         pytest.skip("masked NSQECC is not at the parity bar with FP32 transforms (auto uses FP64)")
     shape = (192, 160) if isz < 64 else (256, 224)
@@ -69,11 +68,11 @@ def test_algorithms_all_lengths(oracle, alg, isz, sm, st, mode):
     rep, *_ = _run(oracle, a, b, alg, isz, sm, st, mask=mask, mode=mode)
     if mode == "fp32" and alg == "zncc":
         assert rep.peak_mismatch_clear == 0 and rep.vf_maxerr <= 0.01, rep
-    elif alg == "mask_nsqecc" and isz + 2 * sm > 96:
-        # auto falls back to FP32 transforms above the FP64 LDS limit
-        assert rep.vf_maxerr <= 0.01 and rep.peak_mismatch_clear <= max(1, rep.n // 100), rep
     else:
         assert rep.ok(), rep
+    if mode == "fp64" or (mode == "auto" and alg in ("zncc", "mask_nsqecc")):
+        # FP64 transforms: far inside the displacement bar
+        assert rep.vf_maxerr <= 1e-6, rep
 
 
 @pytest.mark.parametrize("alg", ["fft", "zncc", "nsqecc"])

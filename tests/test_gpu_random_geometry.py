@@ -22,6 +22,8 @@ def _case(i: int):
     alg = ALGS[i % len(ALGS)]
     mode = MODES[(i // len(ALGS)) % len(MODES)]
     lim = 96 if mode == "fp64" or (mode == "auto" and alg in ("zncc", "mask_nsqecc")) else 128
+    # (the draws keep the limit they were made with; search size 128 in FP64 is
+    # covered by test_gpu_len128.py)
     while True:
         isz = tuple(int(v) for v in rng.integers(6, 41, 2))
         sm = tuple(int(v) for v in rng.integers(1, 13, 2))
