@@ -397,6 +397,7 @@ def main() -> int:
     ap.add_argument("--no-3d", action="store_true", help="skip the 3-D leg of the default run")
     ap.add_argument("--only-3d", action="store_true", help="run only the 3-D leg (no CPU sample)")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched masked leg of the default run")
+    ap.add_argument("--only-batched", action="store_true", help="run only the batched masked leg")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-batch", type=int, default=32)
     args = ap.parse_args()
@@ -413,6 +414,11 @@ def main() -> int:
     if args.only_3d:
         torch.cuda.set_device(0)
         print(json.dumps({"volume_3d": volume_bench(args.mode, args.steps, False, 1)}))
+        return 0
+
+    if args.only_batched:
+        torch.cuda.set_device(0)
+        print(json.dumps({"batched_masked": batched_bench(args.mode, args.steps)}))
         return 0
 
     if args.serve:

@@ -73,6 +73,16 @@ int fail(mqpiv_ctx* c, int code, const char* fmt, ...) {
   return code;
 }
 
+// The context's global transform buffers (gz_scratch_2d bytes): a new
+// allocation gets its slot flags cleared (launches leave them clear).
+static int ensure_gz(mqpiv_ctx* c, size_t need, hipStream_t s) {
+  const size_t cap0 = c->gz_cap;
+  int rc = ensure(c, &c->d_gz, &c->gz_cap, need, s);
+  if (rc) return rc;
+  if (c->gz_cap != cap0) HIP_OR_FAIL(c, hipMemsetAsync(c->d_gz, 0, kGzFlagBytes, s));
+  return MQPIV_OK;
+}
+
 extern "C" const char* mqpiv_version(void) { return "mqpiv 0.1.0 (gfx950)"; }
 
 extern "C" int mqpiv_ctx_create(int device, uint64_t hbm_budget_bytes, mqpiv_ctx** out) {
@@ -309,7 +319,7 @@ static int launch_job(mqpiv_ctx* c, Job& j, hipStream_t s, char* ws = nullptr) {
     if (rc2) return rc2;
     const size_t gzb = gz_scratch_2d(fp64, L, j.n_ia);
     if (gzb) {
-      rc2 = ensure(c, &c->d_gz, &c->gz_cap, gzb, s);
+      rc2 = ensure_gz(c, gzb, s);
       if (rc2) return rc2;
     }
     hipError_t e;
@@ -735,7 +745,7 @@ int batch_device_ws(mqpiv_ctx* c, const mqpiv_request* reqs, int32_t n, void* st
     rc = ensure(c, &c->d_table, &c->table_cap, P.table_bytes + 256, s);
     if (rc) return rc;
     if (P.gz_bytes) {
-      rc = ensure(c, &c->d_gz, &c->gz_cap, P.gz_bytes, s);
+      rc = ensure_gz(c, P.gz_bytes, s);
       if (rc) return rc;
     }
     // pinned staging for the job tables; the previous batch's copy must be done
@@ -796,6 +806,7 @@ extern "C" int mqpiv_graph_create(mqpiv_ctx* c, const mqpiv_request* reqs, int32
   std::vector<char> host(P.table_bytes + 1);
   fill_tables(P, sat_base, host.data());
   e = hipMemcpy(d_table, host.data(), P.table_bytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess && gz) e = hipMemset(gz, 0, kGzFlagBytes);  // slot flags, before capture
   if (e == hipSuccess) e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
   if (e != hipSuccess) {
     hipFree(g->d_mem);

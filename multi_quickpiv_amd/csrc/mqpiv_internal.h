@@ -54,11 +54,14 @@ struct Batch {
   int64_t n_ia_total;
 };
 
-// 2-D transform lengths whose buffer lives in global memory instead of LDS:
-// FP64 at 128 needs 128 x 129 x 16 B = 258 KB, more than a CU's 160 KB of LDS.
-// Such a launch is a persistent grid of at most kGzBlocksPerCu blocks per CU,
-// one IA per block at a time; each block owns one buffer of gz_stride_2d
-// bytes, which the L2 and the Infinity Cache hold while the block works on it.
+// 2-D transform lengths whose buffer does not fit in LDS: FP64 at 128 needs
+// 128 x 129 x 16 B = 258 KB, more than a CU's 160 KB.  Its buffer lives in
+// global memory instead: one of gz_slots_2d buffers of gz_stride_2d bytes,
+// which a block claims for its IA (an int32 flag per slot, 0 = free, in the
+// kGzFlagBytes before the buffers) and frees at its end; the L2 holds the
+// buffer while the block works on it.  At most kGzBlocksPerCu blocks of such a
+// kernel are resident per CU (registers), so that many slots per CU (or one
+// per block, if fewer) leave every claiming block a free slot.
 template <int L, typename T>
 constexpr bool gz_len() {
   return std::is_same<T, double>::value && L == 128;
@@ -68,6 +71,9 @@ constexpr size_t gz_stride_2d() {
   return ((size_t)L * (L + 1) * 2 * sizeof(T) + 4095) & ~(size_t)4095;
 }
 constexpr int kGzBlocksPerCu = 2;
+constexpr int kGzMaxSlots = 1024;
+constexpr size_t kGzFlagBytes = kGzMaxSlots * sizeof(int32_t);
+constexpr uint32_t kGzProbeLimit = 1u << 24;  // probes before a block gives up (NaN outputs)
 
 // Compute units of the current device (cached per device).
 inline int device_cus() {
@@ -81,6 +87,13 @@ inline int device_cus() {
   return cus[dev] > 0 ? cus[dev] : 256;
 }
 
+// Global transform-buffer slots for a launch of n_ia IAs.
+inline int gz_slots_2d(int64_t n_ia) {
+  int64_t c = (int64_t)kGzBlocksPerCu * device_cus();
+  if (c > kGzMaxSlots) c = kGzMaxSlots;
+  return (int)(n_ia < c ? n_ia : c);
+}
+
 // Raises a kernel's dynamic-LDS limit to the CU's 160 KB on the current device,
 // once per (kernel, device); safe to call from any thread before a launch.
 inline void allow_full_lds(const void* fn) {
@@ -89,8 +102,9 @@ inline void allow_full_lds(const void* fn) {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return;
   std::lock_guard<std::mutex> lock(mu);
-  if (done.insert({fn, dev}).second)
-    (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  if (done.insert({fn, dev}).second &&
+      hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) != hipSuccess)
+    (void)hipGetLastError();  // a kernel with static LDS keeps its default; clear the error for the launch check
 }
 
 }  // namespace mqpiv

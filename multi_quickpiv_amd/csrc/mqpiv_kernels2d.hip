@@ -89,13 +89,12 @@ size_t lds_2d(int fp64, int L, int alg, int ns0, int ns1, int ss0) {
   return (size_t)-1;
 }
 
-// Bytes of global transform buffers a launch of n_ia IAs at (fp64, L) needs:
-// one per resident block (at most kGzBlocksPerCu per CU, and no more than the
-// IAs), or 0 for lengths whose buffer is in LDS.
+// Bytes of global transform buffers (slot flags, then the buffers) a launch of
+// n_ia IAs at (fp64, L) needs, or 0 for lengths whose buffer is in LDS.  The
+// flags must be zero when a launch starts; every launch leaves them zero.
 size_t gz_scratch_2d(int fp64, int L, int64_t n_ia) {
   if (!(fp64 && gz_len<128, double>() && L == 128) || n_ia <= 0) return 0;
-  const int64_t cap = (int64_t)kGzBlocksPerCu * device_cus();
-  return (size_t)(n_ia < cap ? n_ia : cap) * gz_stride_2d<128, double>();
+  return kGzFlagBytes + (size_t)gz_slots_2d(n_ia) * gz_stride_2d<128, double>();
 }
 
 hipError_t launch_piv2d(int fp64, int L, int alg, const Job& j0, const Job* d_jobs, const int64_t* d_prefix,

@@ -58,44 +58,37 @@ template <typename T>
 __device__ __forceinline__ gptr<T> as_global(const void* p) {
   return (gptr<T>)p;
 }
-// Writable global-address-space pointer: the transform buffer of the lengths
-// whose buffer does not fit in LDS lives in global memory (see piv2d_kernel).
+// Writable global-address-space pointer (the global part of a transform buffer
+// that does not fit in LDS).
 template <typename T>
 using gmut = T __attribute__((address_space(1)))*;
-// The real view of a complex transform buffer, in the buffer's address space.
-template <typename T>
-__device__ __forceinline__ T* as_real(cpx<T>* p) {
-  return reinterpret_cast<T*>(p);
-}
-template <typename T>
-__device__ __forceinline__ gmut<T> as_real(gmut<cpx<T>> p) {
-  return (gmut<T>)p;
-}
-// Complex loads and stores of a transform buffer: struct copies in LDS, one
-// vector access per value in global memory (where a struct cannot be assigned
-// through the address-space-qualified pointer).
+// The transform buffer of one IA, addressed by (column, row) with row stride
+// L + 1: in LDS (ZLds) or, for the lengths whose buffer does not fit (gz_len),
+// in global memory (ZGlob).
 template <typename T>
 using vec2_t = T __attribute__((ext_vector_type(2)));
-template <typename T>
-__device__ __forceinline__ cpx<T> zld(const cpx<T>* Z, int i) {
-  return Z[i];
-}
-template <typename T>
-__device__ __forceinline__ void zst(cpx<T>* Z, int i, cpx<T> v) {
-  Z[i] = v;
-}
-template <typename T>
-__device__ __forceinline__ cpx<T> zld(gmut<cpx<T>> Z, int i) {
-  const vec2_t<T> v = ((gmut<vec2_t<T>>)Z)[i];
-  return cpx<T>{v.x, v.y};
-}
-template <typename T>
-__device__ __forceinline__ void zst(gmut<cpx<T>> Z, int i, cpx<T> v) {
-  vec2_t<T> w;
-  w.x = v.x;
-  w.y = v.y;
-  ((gmut<vec2_t<T>>)Z)[i] = w;
-}
+template <typename T, int LD>
+struct ZLds {
+  cpx<T>* z;
+  __device__ __forceinline__ cpx<T> ld(int col, int row) const { return z[col * LD + row]; }
+  __device__ __forceinline__ void st(int col, int row, cpx<T> v) const { z[col * LD + row] = v; }
+  __device__ __forceinline__ T* real() const { return reinterpret_cast<T*>(z); }
+};
+template <typename T, int LD>
+struct ZGlob {
+  gmut<cpx<T>> g;  // one vector access per value
+  __device__ __forceinline__ cpx<T> ld(int col, int row) const {
+    const vec2_t<T> v = ((gmut<vec2_t<T>>)g)[col * LD + row];
+    return cpx<T>{v.x, v.y};
+  }
+  __device__ __forceinline__ void st(int col, int row, cpx<T> v) const {
+    vec2_t<T> w;
+    w.x = v.x;
+    w.y = v.y;
+    ((gmut<vec2_t<T>>)g)[col * LD + row] = w;
+  }
+  __device__ __forceinline__ gmut<T> real() const { return (gmut<T>)g; }
+};
 
 __device__ __forceinline__ double load_px(const void* img, int dt, int64_t i) {
   return dt == DT_F32 ? (double)((const float*)img)[i] : ((const double*)img)[i];
@@ -412,9 +405,8 @@ __host__ __device__ constexpr int c2r_wide_lanes() {
   return (P <= 16 && P <= TPI && has_plan(L / P)) ? P : c2r_lanes<L, LPL, TPI>();
 }
 
-template <int L, int LPL, typename T, typename ZP>
-__device__ __forceinline__ void correlate_2d(cpx<T> (&x)[L / LPL], ZP __restrict__ Z, int t, int ns0,
-                                             int ns1) {
+template <int L, int LPL, typename T, typename ZB>
+__device__ __forceinline__ void correlate_2d(cpx<T> (&x)[L / LPL], const ZB& Z, int t, int ns0, int ns1) {
   constexpr int LD = L + 1;
   constexpr int M = L / LPL;
   const int line = t / LPL;
@@ -426,16 +418,16 @@ __device__ __forceinline__ void correlate_2d(cpx<T> (&x)[L / LPL], ZP __restrict
   if (lane) {
     line_fwd<L, LPL, T>(x, q);
 #pragma unroll
-    for (int k = 0; k < M; ++k) zst(Z, (cM + k) * LD + line, x[k]);
+    for (int k = 0; k < M; ++k) Z.st(cM + k, line, x[k]);
   }
   __syncthreads();
   // --- forward along dim 1: line = k1
   if (lane) {
 #pragma unroll
-    for (int k = 0; k < M; ++k) x[k] = zld(Z, line * LD + LPL * k + q);
+    for (int k = 0; k < M; ++k) x[k] = Z.ld(line, LPL * k + q);
     line_fwd<L, LPL, T>(x, q);
 #pragma unroll
-    for (int k = 0; k < M; ++k) zst(Z, line * LD + cM + k, x[k]);
+    for (int k = 0; k < M; ++k) Z.st(line, cM + k, x[k]);
   }
   __syncthreads();
   // The correlation is real, so P is Hermitian: the dim-1 inverse of column
@@ -452,7 +444,7 @@ __device__ __forceinline__ void correlate_2d(cpx<T> (&x)[L / LPL], ZP __restrict
       const int k0 = cM + k;
       const int k0n = (L - k0) % L;
       cpx<T> z = x[k];
-      cpx<T> n = zld(Z, k1n * LD + k0n);
+      cpx<T> n = Z.ld(k1n, k0n);
       T ar = T(0.5) * (z.x + n.x), ai = T(0.5) * (z.y - n.y);
       T br = T(0.5) * (z.y + n.y), bi = T(0.5) * (n.x - z.x);
       x[k] = cpx<T>{ar * br + ai * bi, ar * bi - ai * br};
@@ -465,7 +457,7 @@ __device__ __forceinline__ void correlate_2d(cpx<T> (&x)[L / LPL], ZP __restrict
 #pragma unroll
     for (int m = 0; m < M; ++m) {
       const int s0 = LPL * m + q;
-      if (s0 < ns0) zst(Z, line * LD + s0, x[m]);
+      if (s0 < ns0) Z.st(line, s0, x[m]);
     }
   }
   __syncthreads();
@@ -487,8 +479,8 @@ __device__ __forceinline__ void correlate_2d(cpx<T> (&x)[L / LPL], ZP __restrict
         const int k1 = cM2 + k;
         const bool mirror = k1 >= KH;
         const int kc = mirror ? L - k1 : k1;
-        const cpx<T> ya = zld(Z, kc * LD + sa);
-        const cpx<T> yb = has_b ? zld(Z, kc * LD + sb) : cpx<T>{T(0), T(0)};
+        const cpx<T> ya = Z.ld(kc, sa);
+        const cpx<T> yb = has_b ? Z.ld(kc, sb) : cpx<T>{T(0), T(0)};
         // Y(s, k1) = conj(Y(s, L - k1)) above L/2
         const T ayi = mirror ? -ya.y : ya.y, byi = mirror ? -yb.y : yb.y;
         y[k] = cpx<T>{ya.x - byi, ayi + yb.x};
@@ -497,7 +489,7 @@ __device__ __forceinline__ void correlate_2d(cpx<T> (&x)[L / LPL], ZP __restrict
     }
     __syncthreads();
     if (act) {
-      auto S = as_real<T>(Z);
+      const auto S = Z.real();
       const T kScale = T(1) / T(L * L);
       const int sa = 2 * line2, sb = 2 * line2 + 1;
 #pragma unroll
@@ -555,30 +547,19 @@ constexpr int min_waves_2d() {
   return waves < 1 ? 1 : (waves > MQPIV_OCC_CAP ? MQPIV_OCC_CAP : waves);
 }
 
-// One IA per block-slot.  SH: search regions shifted per IA (Job::shift,
-// multipass); a separate instantiation, so that kernels without shifts carry
-// none of its registers.
-// MF: the split-precision matrix-core correlation (mqpiv_mfma2d.hpp) in place of
-// the register FFTs; FP32 lengths 64 and 96 (4 or 8 lanes a line, 4 or 12
-// waves an IA, windows up to L x L), no shifts.
-// gz: the global transform buffers of a gz_len launch (persistent grid), else unused.
-template <int L, int ALG, typename T, int LPL, bool SH, bool MF = false>
-__global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL, MF>())) void piv2d_kernel(Job j0, const Job* __restrict__ jobs,
-                                                                      const int64_t* __restrict__ prefix,
-                                                                      int n_jobs, int64_t n_ia_total, int ns0_cap,
-                                                                      int ns1_cap, char* gz) {
+// The work of one block of piv2d_kernel; vb is its block index after the XCD
+// remap.
+template <int L, int ALG, typename T, int LPL, bool SH, bool MF>
+__device__ __forceinline__ void piv2d_pass(const Job& j0, const Job* __restrict__ jobs,
+                                           const int64_t* __restrict__ prefix, int n_jobs, int64_t n_ia_total,
+                                           int ns0_cap, int ns1_cap, char* gz, int gz_slots, int32_t vb) {
   using C = Cfg2<L, T, LPL>;
   constexpr int H = C::H;
   constexpr bool GZ = gz_len<L, T>();
   static_assert(!GZ || (C::IPB == 1 && !MF), "a global transform buffer holds one IA");
   extern __shared__ __align__(16) unsigned char smem[];
 
-  // One pass per block, except with global transform buffers: then the grid is
-  // persistent and block b takes IAs r, r + grid, ... (r the XCD-remapped b), so
-  // that each XCD works through contiguous runs of IAs.  The pass count is
-  // uniform over the block (every barrier below is reached by all its waves).
-  const int32_t n_vb = GZ ? (int32_t)n_ia_total : (int32_t)gridDim.x;
-  for (int32_t vb = xcd_remap32((int32_t)blockIdx.x, (int32_t)gridDim.x); vb < n_vb; vb += (int32_t)gridDim.x) {
+  for (;;) {  // runs once (a loop keeps the register allocation of this body as measured)
   const int tid = (int)threadIdx.x;
   // one IA per block: every IA quantity below is uniform over the block, which
   // lets the compiler keep it in scalar registers
@@ -610,17 +591,57 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
   // their margins): the host passes the largest ns0, ns1 of the launch
   static_assert(!MF || (mf_ok<L, T, LPL>() && !SH && ALG != ALG_MASK_NSQECC),
                 "matrix-core form: FP32 lengths 64 and 96, unshifted, unmasked");
+  constexpr int LDc = L + 1;
   const size_t zbytes = GZ ? 0 : (MF ? mf_bytes<MF, L>() : region_bytes_2d(L, sizeof(T), ns0_cap, ns1_cap, ss0));
   const size_t c1bytes = (ALG == ALG_MASK_NSQECC) ? sim_bytes(ns0_cap, ns1_cap, sizeof(T)) : 0;
   const size_t per_ia = zbytes + c1bytes;
   unsigned char* base = smem + (size_t)sub * per_ia;
-  using ZPtr = std::conditional_t<GZ, gmut<cpx<T>>, cpx<T>*>;
-  ZPtr Z;
-  if constexpr (GZ) Z = (ZPtr)(gz + (size_t)blockIdx.x * gz_stride_2d<L, T>());
-  else Z = reinterpret_cast<cpx<T>*>(base);
-  const auto S = as_real<T>(Z);
+  cpx<T>* Z = reinterpret_cast<cpx<T>*>(base);
+  using ZB = std::conditional_t<GZ, ZGlob<T, LDc>, ZLds<T, LDc>>;
+  ZB zb;
+  if constexpr (!GZ) zb.z = Z;
   T* C1 = reinterpret_cast<T*>(base + zbytes);
   double* red = reinterpret_cast<double*>(smem + (size_t)C::IPB * per_ia + (size_t)sub * kRedBytes);
+  // Global part of the buffer: a slot claimed for this block.  The slot flags
+  // (0 free, 1 held) precede the buffers; the host provides twice as many slots
+  // as blocks can be resident, so a free one always exists and the probe
+  // (bounded all the same) ends at once in practice.  The slot is released
+  // after the block's last access (end of this function).
+  int32_t gz_slot = 0;
+  if constexpr (GZ) {
+    // the claim is passed on through the first reduction slot (every thread
+    // reads it before a reduction writes the slots again, behind a barrier)
+    int32_t& claimed = *reinterpret_cast<int32_t*>(red);
+    int32_t* flags = reinterpret_cast<int32_t*>(gz);
+    if (tid == 0) {
+      int32_t got = -1;
+      const uint32_t n = (uint32_t)gz_slots;
+      for (uint32_t i = 0; i < kGzProbeLimit && got < 0; ++i) {
+        const uint32_t s = ((uint32_t)blockIdx.x + i) % n;
+        if (atomicCAS(&flags[s], 0, 1) == 0) got = (int32_t)s;
+      }
+      __threadfence();
+      claimed = got;
+    }
+    __syncthreads();
+    gz_slot = claimed;
+    if (gz_slot < 0) {  // no slot (cannot happen with the host's slot count): NaN outputs
+      if (valid && t == 0) {
+        if (J.out_dtype == DT_F32) {
+          ((float*)J.vf)[2 * ia32] = NAN;
+          ((float*)J.vf)[2 * ia32 + 1] = NAN;
+          if (J.sn) ((float*)J.sn)[ia32] = NAN;
+        } else {
+          ((double*)J.vf)[2 * ia32] = NAN;
+          ((double*)J.vf)[2 * ia32 + 1] = NAN;
+          if (J.sn) ((double*)J.sn)[ia32] = NAN;
+        }
+      }
+      return;
+    }
+    zb.g = (gmut<cpx<T>>)(gz + kGzFlagBytes + (size_t)gz_slot * gz_stride_2d<L, T>());
+  }
+  const auto S = zb.real();
 
   const uint32_t vn0 = (uint32_t)J.vf_n[0];
   const int32_t v1 = (int32_t)((uint32_t)ia32 / vn0);
@@ -796,7 +817,7 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
 
   // ---- correlation(s)
   if constexpr (MF) correlate_2d_mfma<L, LPL, C::TPI, Group<C::TPI>>(x, Z, t, ns0, ns1, red);
-  else correlate_2d<L, LPL, T>(x, Z, t, ns0, ns1);
+  else correlate_2d<L, LPL, T>(x, zb, t, ns0, ns1);
   if constexpr (ALG == ALG_MASK_NSQECC) {
     for (int i = t; i < ns0 * ns1; i += C::TPI) C1[i] = S[i];
     // second correlation: mask (x) G^2 (mask_nsqecc.jl:155)
@@ -827,7 +848,7 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
     if (dt == DT_F32) load2(float{});
     else load2(double{});
     __syncthreads();
-    correlate_2d<L, LPL, T>(x, Z, t, ns0, ns1);
+    correlate_2d<L, LPL, T>(x, zb, t, ns0, ns1);
   }
 
   // ---- normalisation, peak and SN in one pass.  Each lane normalises its
@@ -1068,11 +1089,32 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
       if (J.sn) ((double*)J.sn)[ia] = sn;
     }
   }
-  if constexpr (!GZ) break;
-  // the next pass rewrites this block's buffer and LDS slots: every lane has
-  // finished reading them (the fit above reads the window) before any writes
-  __syncthreads();
+  if constexpr (GZ) {
+    __syncthreads();  // every wave is done with the buffer
+    if (tid == 0) {
+      __threadfence();
+      atomicExch(reinterpret_cast<int32_t*>(gz) + gz_slot, 0);
+    }
   }
+  break;
+  }
+}
+
+// One IA per block-slot.  SH: search regions shifted per IA (Job::shift,
+// multipass); a separate instantiation, so that kernels without shifts carry
+// none of its registers.
+// MF: the split-precision matrix-core correlation (mqpiv_mfma2d.hpp) in place of
+// the register FFTs; FP32 lengths 64 and 96 (4 or 8 lanes a line, 4 or 12
+// waves an IA, windows up to L x L), no shifts.
+// gz / gz_slots: the global parts of the transform buffers of a gz_len launch
+// (gz_slots of them after the slot flags), else unused.
+template <int L, int ALG, typename T, int LPL, bool SH, bool MF = false>
+__global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL, MF>())) void piv2d_kernel(Job j0, const Job* __restrict__ jobs,
+                                                                      const int64_t* __restrict__ prefix,
+                                                                      int n_jobs, int64_t n_ia_total, int ns0_cap,
+                                                                      int ns1_cap, char* gz, int gz_slots) {
+  piv2d_pass<L, ALG, T, LPL, SH, MF>(j0, jobs, prefix, n_jobs, n_ia_total, ns0_cap, ns1_cap, gz, gz_slots,
+                                     xcd_remap32((int32_t)blockIdx.x, (int32_t)gridDim.x));
 }
 
 // ---------------------------------------------------------------------------
@@ -1092,20 +1134,14 @@ hipError_t launch_2d_k(const Job& j0, const Job* d_jobs, const int64_t* d_prefix
   using C = Cfg2<L, T, LPL>;
   const void* fn = (const void*)piv2d_kernel<L, ALG, T, LPL, SH, MF>;
   allow_full_lds(fn);
-  int64_t nb = (n_ia_total + C::IPB - 1) / C::IPB;
+  const int64_t nb = (n_ia_total + C::IPB - 1) / C::IPB;
+  int slots = 0;
   if constexpr (gz_len<L, T>()) {
-    // persistent grid: the blocks that stay resident, within the buffers the
-    // caller provides (gz_scratch_2d)
-    if (!gz) return hipErrorInvalidValue;
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, C::NT, lds) != hipSuccess || per_cu < 1)
-      per_cu = 1;
-    per_cu = per_cu < kGzBlocksPerCu ? per_cu : kGzBlocksPerCu;
-    const int64_t cap = (int64_t)per_cu * device_cus();
-    nb = nb < cap ? nb : cap;
+    if (!gz) return hipErrorInvalidValue;  // the caller provides gz_scratch_2d bytes
+    slots = gz_slots_2d(n_ia_total);
   }
   hipLaunchKernelGGL((piv2d_kernel<L, ALG, T, LPL, SH, MF>), dim3((unsigned)nb), dim3(C::NT), lds, s, j0, d_jobs,
-                     d_prefix, n_jobs, n_ia_total, ns0, ns1, gz);
+                     d_prefix, n_jobs, n_ia_total, ns0, ns1, gz, slots);
   return hipGetLastError();
 }
 

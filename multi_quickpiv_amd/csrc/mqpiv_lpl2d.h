@@ -52,7 +52,7 @@ template <> struct Lpl2<double, 64> { static constexpr int v = MQPIV_LPL_F64_64;
 #endif
 template <> struct Lpl2<double, 96> { static constexpr int v = MQPIV_LPL_F64_96; };
 #ifndef MQPIV_LPL_F64_128
-#define MQPIV_LPL_F64_128 8
+#define MQPIV_LPL_F64_128 4
 #endif
 template <> struct Lpl2<double, 128> { static constexpr int v = MQPIV_LPL_F64_128; };
 

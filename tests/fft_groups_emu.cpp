@@ -123,6 +123,7 @@ int main() {
   both<96, 4, double>("f64");
   both<96, 8, double>("f64");
   both<96, 16, double>("f64");
+  both<128, 4, double>("f64");
   both<128, 8, double>("f64");
   both<128, 16, double>("f64");
   return 0;

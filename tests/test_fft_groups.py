@@ -37,7 +37,7 @@ def emu_results(tmp_path_factory):
 
 
 def test_every_group_matches_the_dft(emu_results):
-    assert len(emu_results) == 56
+    assert len(emu_results) == 58
     for (L, P, tn, d), err in emu_results.items():
         # inputs U(-1, 1); rounding grows slowly with L
         tol = 5e-6 if tn == "f32" else 5e-14
@@ -48,5 +48,5 @@ def test_a_wrong_partner_is_detected(tmp_path_factory):
     """Planted mistake: every exchange reads the lane next to its partner.  The
     same comparison must then fail in every case (errors O(1), not rounding)."""
     bad = _run(tmp_path_factory, "-DEMU_WRONG_PARTNER")
-    assert len(bad) == 56
+    assert len(bad) == 58
     assert min(bad.values()) > 1e-2, min(bad.items(), key=lambda kv: kv[1])
