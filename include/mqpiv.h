@@ -96,6 +96,24 @@ const char* mqpiv_last_error(const mqpiv_ctx* ctx);
 /* nVF per dim and the IA count for an image of size dims[0..ndims-1]. */
 int mqpiv_vf_size(const mqpiv_params* p, const int64_t* dims, int64_t* vf_out3, int64_t* n_ia);
 
+/* Allocation-free form of mqpiv_piv_device.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  It calls no
+ * hipMalloc or hipFree: the integral arrays of ZNCC and NSQECC live in a static
+ * per-device array of MQPIV_NOALLOC_SCRATCH_BYTES (96 MiB: a 2048 x 2048 pair with
+ * ZNCC fits), which each GPU holds from library load until the process ends;
+ * the context must have been created on that device.  2-D pairs only, and not
+ * FP64 transforms at search size 128 (MQPIV_ERR_UNSUPPORTED); a pair whose
+ * integral arrays exceed the array is rejected with MQPIV_ERR_OOM.  Every call
+ * on a device shares its array, which keeps the last call's contents: calls
+ * that overlap on one device (other streams, other contexts) race.  Results
+ * equal mqpiv_piv_device's. */
+#define MQPIV_NOALLOC_SCRATCH_BYTES (96ull << 20)
+int mqpiv_piv_device_noalloc(mqpiv_ctx* ctx, const mqpiv_params* p, const int64_t* dims, int32_t in_dtype,
+                             const void* d_img1, const void* d_img2, const void* d_mask,
+                             const uint8_t* d_skip_mask, int32_t out_dtype, void* d_VF, void* d_SN,
+                             void* stream);
+
 /* Host buffers.  img1/img2 (and mask) have in_dtype elements; VF/SN have
  * out_dtype elements and are fully written (SN may be NULL; it is required
  * when compute_sn is set).  skip_mask (nullable, one byte per IA) overrides the

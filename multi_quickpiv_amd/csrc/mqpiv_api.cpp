@@ -73,6 +73,30 @@ int fail(mqpiv_ctx* c, int code, const char* fmt, ...) {
   return code;
 }
 
+// ---------------------------------------------------------------------------
+// Static scratch of the allocation-free entry point (mqpiv_piv_device_noalloc):
+// the integral arrays of one 2-D call, in a file-scope device array that each
+// GPU holds from library load to process exit.  Host code never uses the
+// array's name as an address: its per-device address comes from
+// hipGetSymbolAddress at context creation (with that device current).  The
+// integral-array build writes every entry, its zero border included, so
+// nothing in it has to start at zero.
+static __device__ __align__(256) unsigned char scratch[MQPIV_NOALLOC_SCRATCH_BYTES];
+static constexpr int kMaxDevices = 64;
+static void* g_scratch_ptr[kMaxDevices] = {};
+static std::string g_scratch_err[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+static void noalloc_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e == hipSuccess && ptr) g_scratch_ptr[device] = ptr;
+  else g_scratch_err[device] = std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e);
+}
+
 // The context's global transform buffers (gz_scratch_2d bytes): a new
 // allocation gets its slot flags cleared (launches leave them clear).
 static int ensure_gz(mqpiv_ctx* c, size_t need, hipStream_t s) {
@@ -92,6 +116,7 @@ extern "C" int mqpiv_ctx_create(int device, uint64_t hbm_budget_bytes, mqpiv_ctx
   if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return MQPIV_ERR_DEVICE;
   if (device < 0 || device >= n) return MQPIV_ERR_BAD_PARAMS;
   if (hipSetDevice(device) != hipSuccess) return MQPIV_ERR_DEVICE;
+  noalloc_setup(device);  // the allocation-free entry point's scratch address on this device
   mqpiv_ctx* c = new mqpiv_ctx();
   c->device = device;
   c->budget = hbm_budget_bytes;
@@ -234,6 +259,15 @@ static int sat_count(int alg, int ndims) {
   return alg == ALG_ZNCC ? (ndims == 3 ? 3 : 2) : (alg == ALG_NSQECC ? 1 : 0);
 }
 
+// Workspace a job's integral arrays take (as request_ws_bytes).
+static size_t job_ws_bytes(const Job& j) {
+  const int nsat = sat_count(j.alg, j.ndims);
+  if (!nsat) return 0;
+  const size_t one = (sat_bytes(j.dims, j.ndims) + 255) & ~(size_t)255;
+  const size_t tmp = (sat_tmp_bytes(j.dims, j.ndims) + 255) & ~(size_t)255;
+  return nsat * one + tmp;
+}
+
 size_t request_ws_bytes(const mqpiv_request& r) {
   const int nd = r.params.ndims;
   const int nsat = sat_count(r.params.corr_alg, nd);
@@ -363,6 +397,40 @@ extern "C" int mqpiv_piv_device(mqpiv_ctx* c, const mqpiv_params* p, const int64
   if (rc) return rc;
   hipStream_t s = stream ? (hipStream_t)stream : c->stream;
   return launch_job(c, j, s);
+}
+
+extern "C" int mqpiv_piv_device_noalloc(mqpiv_ctx* c, const mqpiv_params* p, const int64_t* dims,
+                                        int32_t in_dtype, const void* d_img1, const void* d_img2,
+                                        const void* d_mask, const uint8_t* d_skip, int32_t out_dtype, void* d_VF,
+                                        void* d_SN, void* stream) {
+  if (!c) return MQPIV_ERR_NULL;
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, MQPIV_ERR_DEVICE, "hipSetDevice failed");
+  Job j;
+  int rc = build_job(c, p, dims, in_dtype, d_img1, d_img2, d_mask, d_skip, out_dtype, d_VF, d_SN, &j);
+  if (rc) return rc;
+  if (j.ndims != 2)
+    return fail(c, MQPIV_ERR_UNSUPPORTED, "the allocation-free entry point runs 2-D pairs (3-D needs transform scratch)");
+  int fp64 = 0, L = 0;
+  rc = choose_2d(c, j, &fp64, &L);
+  if (rc) return rc;
+  if (gz_scratch_2d(fp64, L, j.n_ia))
+    return fail(c, MQPIV_ERR_UNSUPPORTED,
+                "the allocation-free entry point does not run FP64 transforms at search size 128 (global transform buffers)");
+  void* ptr = nullptr;
+  {
+    std::lock_guard<std::mutex> lk2(g_scratch_mu);
+    if (c->device >= 0 && c->device < kMaxDevices) ptr = g_scratch_ptr[c->device];
+    if (!ptr)
+      return fail(c, MQPIV_ERR_DEVICE, "no static scratch on device %d: %s", c->device,
+                  c->device >= 0 && c->device < kMaxDevices ? g_scratch_err[c->device].c_str() : "device index");
+  }
+  const size_t need = job_ws_bytes(j);
+  if (need > MQPIV_NOALLOC_SCRATCH_BYTES)
+    return fail(c, MQPIV_ERR_OOM, "the pair needs %zu bytes of integral arrays; the static scratch holds %llu", need,
+                (unsigned long long)MQPIV_NOALLOC_SCRATCH_BYTES);
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  return launch_job(c, j, s, need ? (char*)ptr : nullptr);
 }
 
 extern "C" int mqpiv_piv(mqpiv_ctx* c, const mqpiv_params* p, const int64_t* dims, int32_t in_dtype,
