@@ -715,6 +715,10 @@ __device__ __forceinline__ void piv2d_pass(const Job& j0, const Job* __restrict_
   // ---- load the packed window into registers; sum(F^2) and the IA maxima on the way
   cpx<T> x[H];
   double sumF2 = 0.0, mx1 = -INFINITY, mx2 = -INFINITY;
+  // FP64 ZNCC forms its window energies and sum(F'^2) from the rounded
+  // differences G - mean, F - mean themselves (below), as the reference does
+  constexpr bool kLocalW = ALG == ALG_ZNCC && std::is_same<T, double>::value;
+  double sa2 = 0.0;
   const bool want_max = J.has_threshold || J.has_mask_threshold;
   // Values are loaded in batches of LB per lane: all of a lane's columns at
   // once, except with global transform buffers, whose long lines would hold too
@@ -778,6 +782,7 @@ __device__ __forceinline__ void piv2d_pass(const Job& j0, const Job* __restrict_
           if (want_max) mx1 = fmax(mx1, (double)f);
           if constexpr (ALG == ALG_ZNCC) {
             a = std::is_same<TIN, float>::value ? (double)((float)f - meanF32) : (double)f - meanF64;
+            if constexpr (kLocalW) sa2 += a * a;
           } else if constexpr (ALG == ALG_MASK_NSQECC) {
             const double m = (double)mv[j];
             if (want_max) mx2 = fmax(mx2, m);
@@ -802,6 +807,8 @@ __device__ __forceinline__ void piv2d_pass(const Job& j0, const Job* __restrict_
   if (dt == DT_F32) load(float{});
   else load(double{});
   if constexpr (ALG == ALG_NSQECC || ALG == ALG_MASK_NSQECC) sumF2 = Group<C::TPI>::reduce(sumF2, OpAdd{}, red, t);
+  // FP64 ZNCC: sum(F'^2) of the rounded differences themselves, as zncc.jl:131-135
+  if constexpr (kLocalW) sumF2 = Group<C::TPI>::reduce(sa2, OpAdd{}, red, t);
 
   // ---- skip test (piv_ops.jl:134-157)
   int skip = 0;
@@ -884,7 +891,76 @@ __device__ __forceinline__ void piv2d_pass(const Job& j0, const Job* __restrict_
       smv += dv;
     }
   };
-  if constexpr (ALG == ALG_ZNCC || ALG == ALG_NSQECC) {
+  if constexpr (kLocalW) {
+    // W(s) = sum of G'^2 over the search-buffer window [s, s + is), with G' the
+    // padded search window minus its mean, rounded in the input element type
+    // (zncc_ops.jl:22-24), from a summed-area table of G'^2 built here in LDS
+    // (integralArraySQ!, common_ops.jl:142-180; window sums as common_ops.jl:
+    // 203-245).  Forming it from image-wide integral arrays of G and G^2
+    // instead changes each window by the rounding of G - mean, which the near
+    // cancellation of ZNCC's SN sum can amplify past its bar.
+    const int ss1 = is1 + 2 * sm1;
+    const int Dl = ss0 + 1;
+    double* satl;
+    if constexpr (GZ) satl = reinterpret_cast<double*>(smem + (size_t)C::IPB * per_ia + (size_t)C::IPB * kRedBytes);
+    else satl = reinterpret_cast<double*>(base + sim_bytes(ns0_cap, ns1_cap, sizeof(T)));
+    const int nsat = Dl * (ss1 + 1);
+    for (int e = t; e < nsat; e += C::TPI) {
+      const int i = e % Dl, j = e / Dl;
+      double v = 0.0;
+      const int x0 = i - 1 - g0.goff, x1 = j - 1 - g1.goff;
+      if (i > 0 && j > 0 && x0 >= 0 && x0 < g0.gsz && x1 >= 0 && x1 < g1.gsz) {
+        const int64_t px = (g0.sr + x0) + d0 * (g1.sr + x1);
+        const double gg = dt == DT_F32 ? (double)(((const float*)J.img2)[px] - meanG32)
+                                       : ((const double*)J.img2)[px] - meanG64;
+        v = gg * gg;
+      }
+      satl[e] = v;
+    }
+    __syncthreads();
+    // prefix sums along dim 1 (one column per lane), then along dim 2 (one row
+    // per lane), eight entries loaded at a time
+    auto scan = [&](int lines, int len, int line_step, int elem_step) {
+      for (int c = t; c < lines; c += C::TPI) {
+        double* col = satl + (size_t)(c + 1) * line_step + elem_step;
+        double acc = 0.0;
+        int k = 0;
+        for (; k + 8 <= len; k += 8) {
+          double v[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) v[u] = col[(k + u) * elem_step];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            acc += v[u];
+            col[(k + u) * elem_step] = acc;
+          }
+        }
+        for (; k < len; ++k) {
+          acc += col[k * elem_step];
+          col[k * elem_step] = acc;
+        }
+      }
+      __syncthreads();
+    };
+    scan(ss1, ss0, Dl, 1);
+    scan(ss0, ss1, 1, Dl);
+    const double sqF = sqrt(sumF2);
+    for (int e = t; e < nE; e += C::TPI) {
+      const int s0 = w0, s1 = w1;
+      step();
+      // sat_window_add order (1-based corners T = s0 + 1, L = s1 + 1)
+      double W = 0.0;
+      W += satl[(s0 + is0) + Dl * (s1 + is1)];
+      W += satl[s0 + Dl * s1];
+      W -= satl[s0 + Dl * (s1 + is1)];
+      W -= satl[(s0 + is0) + Dl * s1];
+      double den = sqrt(W);
+      den *= sqF;
+      const double r = (double)S[e] / den;  // zncc_ops.jl:39-48
+      S[e] = (T)r;
+      visit(e, s0, s1, (T)r);
+    }
+  } else if constexpr (ALG == ALG_ZNCC || ALG == ALG_NSQECC) {
     // W(s) = sum of G'^2 over the search-buffer window [s, s + is) (the SAT
     // window sums of common_ops.jl:203-245), from the image's FP64 integral
     // arrays after clipping the window to the part that holds image data.
@@ -1124,8 +1200,10 @@ template <int L, typename T, int LPL>
 size_t lds_bytes_2d(int alg, int ns0, int ns1, int ss0) {
   using C = Cfg2<L, T, LPL>;
   size_t z = gz_len<L, T>() ? 0 : region_bytes_2d(L, sizeof(T), ns0, ns1, ss0);
+  // FP64 ZNCC with a global transform buffer keeps its window-energy table in LDS
+  const size_t zw = (gz_len<L, T>() && alg == ALG_ZNCC) ? (size_t)(L + 1) * (L + 1) * sizeof(double) : 0;
   size_t c1 = (alg == ALG_MASK_NSQECC) ? sim_bytes(ns0, ns1, sizeof(T)) : 0;
-  return (z + c1) * C::IPB + (size_t)C::IPB * kRedBytes;
+  return (z + c1) * C::IPB + (size_t)C::IPB * kRedBytes + zw;
 }
 
 template <int L, int ALG, typename T, int LPL, bool SH, bool MF = false>

@@ -183,7 +183,11 @@ def piv_device(ctx: "_lib.Context", pivparams: PIVParameters, shape: Tuple[int, 
                mode: str = "auto") -> None:
     """Asynchronous PIV over device-resident buffers (raw device pointers, e.g.
     ``tensor.data_ptr()``; tensors laid out column-major as in ``PIV``).
-    ``pivparams.multipass`` > 1 runs the coarse-to-fine passes."""
+    ``pivparams.multipass`` > 1 runs the coarse-to-fine passes.  With
+    ``stream`` None the work runs on the context's own stream, which is not
+    ordered with torch's streams, so the device is synchronised first (the
+    inputs and outputs may still be being written by torch)."""
+    _order_before_library_stream(stream)
     cp = to_c_params(pivparams, mask_NSQECC.code if masked else None, mode, multipass_ok=True)
     dims = (C.c_int64 * 3)(*(list(shape) + [1] * (3 - len(shape))))
     if pivparams.multipass > 1:
@@ -282,9 +286,21 @@ class DeviceRequest:
         return VF, SN
 
 
+def _order_before_library_stream(stream: Optional[int]) -> None:
+    """The library's own stream (stream None) is a non-blocking stream: work
+    queued by torch on the device (rendering inputs, zero-filling outputs) is
+    not ordered before it, so wait for it."""
+    if stream is None:
+        import torch
+
+        torch.cuda.synchronize()
+
+
 def piv_batch(reqs, ctx=None, stream: Optional[int] = None) -> None:
     """All requests in one batched call (one kernel launch per algorithm /
-    precision / transform-length group); asynchronous on ``stream``."""
+    precision / transform-length group); asynchronous on ``stream`` (None: the
+    context's stream, after a device synchronisation)."""
+    _order_before_library_stream(stream)
     ctx = ctx or _lib.context(reqs[0].a.device.index or 0)
     arr = (_lib.MqpivRequest * len(reqs))(*[r.c_request() for r in reqs])
     ctx.check(ctx.lib.mqpiv_piv_batch_device(ctx.handle, arr, len(reqs), stream))
@@ -303,6 +319,7 @@ class BatchGraph:
         self.handle = h
 
     def launch(self, stream: Optional[int] = None) -> None:
+        _order_before_library_stream(stream)
         self.ctx.check(self.ctx.lib.mqpiv_graph_launch(self.handle, stream))
 
     def close(self) -> None:
