@@ -251,17 +251,19 @@ static int build_job(mqpiv_ctx* c, const mqpiv_params* p, const int64_t* dims, i
   return MQPIV_OK;
 }
 
-// Integral arrays the normalisation reads: NSQECC needs sum(G^2), ZNCC also sum(G).
-// (3-D ZNCC also reads the means from integral arrays: sum(F) of image 1.)
-// Integral arrays a job needs: count, bytes of one (256-aligned) and of the
-// prefix-scan temporary.
-static int sat_count(int alg, int ndims) {
-  return alg == ALG_ZNCC ? (ndims == 3 ? 3 : 2) : (alg == ALG_NSQECC ? 1 : 0);
+// Integral arrays the normalisation reads: NSQECC needs sum(G^2), ZNCC also
+// sum(G) (3-D ZNCC also the means from sum(F) of image 1).  2-D ZNCC with FP64
+// transforms needs none: it forms its window sums in the kernel.  Integral
+// arrays a job needs: count, bytes of one (256-aligned) and of the prefix-scan
+// temporary.
+static int sat_count(int alg, int ndims, int fp64 = 0) {
+  if (alg == ALG_ZNCC) return ndims == 3 ? 3 : (fp64 ? 0 : 2);
+  return alg == ALG_NSQECC ? 1 : 0;
 }
 
 // Workspace a job's integral arrays take (as request_ws_bytes).
-static size_t job_ws_bytes(const Job& j) {
-  const int nsat = sat_count(j.alg, j.ndims);
+static size_t job_ws_bytes(const Job& j, int fp64) {
+  const int nsat = sat_count(j.alg, j.ndims, fp64);
   if (!nsat) return 0;
   const size_t one = (sat_bytes(j.dims, j.ndims) + 255) & ~(size_t)255;
   const size_t tmp = (sat_tmp_bytes(j.dims, j.ndims) + 255) & ~(size_t)255;
@@ -279,12 +281,12 @@ size_t request_ws_bytes(const mqpiv_request& r) {
 
 // Builds the job's integral arrays in ws (request_ws_bytes of it) or, when ws is
 // NULL, in the context's grow-only buffer.
-static int prepare_sats(mqpiv_ctx* c, Job& j, hipStream_t s, char* ws = nullptr) {
+static int prepare_sats(mqpiv_ctx* c, Job& j, hipStream_t s, char* ws = nullptr, int fp64 = 0) {
   j.sat1 = j.sat2 = j.sat1f = nullptr;
-  if (j.alg != ALG_ZNCC && j.alg != ALG_NSQECC) return MQPIV_OK;
+  const int nsat = sat_count(j.alg, j.ndims, fp64);
+  if (!nsat) return MQPIV_OK;
   const size_t one = (sat_bytes(j.dims, j.ndims) + 255) & ~(size_t)255;
   const size_t tmp = (sat_tmp_bytes(j.dims, j.ndims) + 255) & ~(size_t)255;
-  const int nsat = sat_count(j.alg, j.ndims);
   if (!ws) {
     int rc = ensure(c, &c->d_sat, &c->sat_cap, nsat * one + tmp, s);
     if (rc) return rc;
@@ -349,7 +351,7 @@ static int launch_job(mqpiv_ctx* c, Job& j, hipStream_t s, char* ws = nullptr) {
     int fp64 = 0, L = 0;
     int rc2 = choose_2d(c, j, &fp64, &L);
     if (rc2) return rc2;
-    rc2 = prepare_sats(c, j, s, ws);
+    rc2 = prepare_sats(c, j, s, ws, fp64);
     if (rc2) return rc2;
     const size_t gzb = gz_scratch_2d(fp64, L, j.n_ia);
     if (gzb) {
@@ -425,7 +427,7 @@ extern "C" int mqpiv_piv_device_noalloc(mqpiv_ctx* c, const mqpiv_params* p, con
       return fail(c, MQPIV_ERR_DEVICE, "no static scratch on device %d: %s", c->device,
                   c->device >= 0 && c->device < kMaxDevices ? g_scratch_err[c->device].c_str() : "device index");
   }
-  const size_t need = job_ws_bytes(j);
+  const size_t need = job_ws_bytes(j, fp64);
   if (need > MQPIV_NOALLOC_SCRATCH_BYTES)
     return fail(c, MQPIV_ERR_OOM, "the pair needs %zu bytes of integral arrays; the static scratch holds %llu", need,
                 (unsigned long long)MQPIV_NOALLOC_SCRATCH_BYTES);
@@ -684,7 +686,7 @@ static int plan_batch(mqpiv_ctx* c, const mqpiv_request* r, int n, BatchPlan& P)
     int fp64 = 0, L = 0;
     rc = choose_2d(c, j, &fp64, &L);
     if (rc) return fail(c, rc, "request %d: %s", i, c->err.c_str());
-    const int ns = j.alg == ALG_ZNCC ? 2 : (j.alg == ALG_NSQECC ? 1 : 0);
+    const int ns = sat_count(j.alg, 2, fp64);
     const int idx = (int)P.jobs2d.size();
     P.jobs2d.push_back(j);
     P.req2d.push_back(i);

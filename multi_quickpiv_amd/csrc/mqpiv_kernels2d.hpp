@@ -547,12 +547,20 @@ constexpr int min_waves_2d() {
   return waves < 1 ? 1 : (waves > MQPIV_OCC_CAP ? MQPIV_OCC_CAP : waves);
 }
 
-// The work of one block of piv2d_kernel; vb is its block index after the XCD
-// remap.
-template <int L, int ALG, typename T, int LPL, bool SH, bool MF>
-__device__ __forceinline__ void piv2d_pass(const Job& j0, const Job* __restrict__ jobs,
-                                           const int64_t* __restrict__ prefix, int n_jobs, int64_t n_ia_total,
-                                           int ns0_cap, int ns1_cap, char* gz, int gz_slots, int32_t vb) {
+// One IA per block-slot.  SH: search regions shifted per IA (Job::shift,
+// multipass); a separate instantiation, so that kernels without shifts carry
+// none of its registers.
+// MF: the split-precision matrix-core correlation (mqpiv_mfma2d.hpp) in place of
+// the register FFTs; FP32 lengths 64 and 96 (4 or 8 lanes a line, 4 or 12
+// waves an IA, windows up to L x L), no shifts.
+// gz / gz_slots: the global parts of the transform buffers of a gz_len launch
+// (gz_slots of them after the slot flags), else unused.
+template <int L, int ALG, typename T, int LPL, bool SH, bool MF = false>
+__global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL, MF>())) void piv2d_kernel(Job j0, const Job* __restrict__ jobs,
+                                                                      const int64_t* __restrict__ prefix,
+                                                                      int n_jobs, int64_t n_ia_total, int ns0_cap,
+                                                                      int ns1_cap, char* gz, int gz_slots) {
+  const int32_t vb = xcd_remap32((int32_t)blockIdx.x, (int32_t)gridDim.x);
   using C = Cfg2<L, T, LPL>;
   constexpr int H = C::H;
   constexpr bool GZ = gz_len<L, T>();
@@ -719,6 +727,10 @@ __device__ __forceinline__ void piv2d_pass(const Job& j0, const Job* __restrict_
   // differences G - mean, F - mean themselves (below), as the reference does
   constexpr bool kLocalW = ALG == ALG_ZNCC && std::is_same<T, double>::value;
   double sa2 = 0.0;
+  // G'^2 of the lane's window samples, kept through the correlation for the
+  // table (lengths whose buffer is in LDS; the others read the image again)
+  constexpr bool kKeepG2 = kLocalW && !GZ;
+  double gq[kKeepG2 ? H : 1];
   const bool want_max = J.has_threshold || J.has_mask_threshold;
   // Values are loaded in batches of LB per lane: all of a lane's columns at
   // once, except with global transform buffers, whose long lines would hold too
@@ -800,6 +812,7 @@ __device__ __forceinline__ void piv2d_pass(const Job& j0, const Job* __restrict_
           else
             g = (double)gv[j];
         }
+        if constexpr (kKeepG2) gq[k] = g * g;
         x[k] = cpx<T>{(T)a, (T)g};
       }
     }
@@ -904,46 +917,114 @@ __device__ __forceinline__ void piv2d_pass(const Job& j0, const Job* __restrict_
     double* satl;
     if constexpr (GZ) satl = reinterpret_cast<double*>(smem + (size_t)C::IPB * per_ia + (size_t)C::IPB * kRedBytes);
     else satl = reinterpret_cast<double*>(base + sim_bytes(ns0_cap, ns1_cap, sizeof(T)));
-    const int nsat = Dl * (ss1 + 1);
-    for (int e = t; e < nsat; e += C::TPI) {
-      const int i = e % Dl, j = e / Dl;
-      double v = 0.0;
-      const int x0 = i - 1 - g0.goff, x1 = j - 1 - g1.goff;
-      if (i > 0 && j > 0 && x0 >= 0 && x0 < g0.gsz && x1 >= 0 && x1 < g1.gsz) {
-        const int64_t px = (g0.sr + x0) + d0 * (g1.sr + x1);
-        const double gg = dt == DT_F32 ? (double)(((const float*)J.img2)[px] - meanG32)
-                                       : ((const double*)J.img2)[px] - meanG64;
-        v = gg * gg;
+    // G'^2 into the table (entry (i, j) = pixel (i - 1, j - 1) of the padded
+    // window; border entries, index 0 in either dim, are zero): consecutive
+    // lanes take consecutive pixels down a column, each lane all its pixels at
+    // once.
+    for (int i = t; i <= ss0; i += C::TPI) satl[i] = 0.0;         // column 0
+    for (int j = t + 1; j <= ss1; j += C::TPI) satl[Dl * j] = 0.0; // row 0
+    if constexpr (kKeepG2) {
+      // lane (line k0, h) holds column k1 = LPL*k + h of row k0
+      if (line < ss0) {
+#pragma unroll
+        for (int k = 0; k < H; ++k) {
+          const int k1 = LPL * k + h;
+          if (k1 < ss1) satl[(line + 1) + Dl * (k1 + 1)] = gq[k];
+        }
       }
-      satl[e] = v;
-    }
-    __syncthreads();
-    // prefix sums along dim 1 (one column per lane), then along dim 2 (one row
-    // per lane), eight entries loaded at a time
-    auto scan = [&](int lines, int len, int line_step, int elem_step) {
-      for (int c = t; c < lines; c += C::TPI) {
-        double* col = satl + (size_t)(c + 1) * line_step + elem_step;
-        double acc = 0.0;
-        int k = 0;
-        for (; k + 8 <= len; k += 8) {
-          double v[8];
+    } else {
+      constexpr int KF = (L * L + C::TPI - 1) / C::TPI;  // pixels a lane takes, at most
+      const int npx = ss0 * ss1;
+      const int st0 = C::TPI % ss0, st1 = C::TPI / ss0;
+      int i0 = t % ss0, j0 = t / ss0;
+      static_assert(KF <= 32, "one data bit per pixel");
+      double v[KF];
+      int dst[KF];
+      uint32_t data = 0;  // bit k: pixel k holds image data
+      // every load reads an in-image pixel (positions outside the data are
+      // clamped to it and masked afterwards), so that no load is predicated
+      // and a lane's loads all go out before it waits
+      const int32_t id0 = (int32_t)d0;
+      auto fill = [&](auto tin) {
+        using TIN = decltype(tin);
+        const gptr<TIN> org = as_global<TIN>(J.img2) + (g0.sr + d0 * g1.sr);
+        TIN gv[KF];
 #pragma unroll
-          for (int u = 0; u < 8; ++u) v[u] = col[(k + u) * elem_step];
-#pragma unroll
-          for (int u = 0; u < 8; ++u) {
-            acc += v[u];
-            col[(k + u) * elem_step] = acc;
+        for (int k = 0; k < KF; ++k) {
+          const int e = t + k * C::TPI;
+          const int x0 = i0 - g0.goff, x1 = j0 - g1.goff;
+          const bool in = e < npx && x0 >= 0 && x0 < g0.gsz && x1 >= 0 && x1 < g1.gsz;
+          data |= (in ? 1u : 0u) << k;
+          dst[k] = e < npx ? (i0 + 1) + Dl * (j0 + 1) : Dl * (ss1 + 1);  // past the table: a spare entry
+          const int32_t cx0 = min(max(x0, 0), g0.gsz - 1), cx1 = min(max(x1, 0), g1.gsz - 1);
+          gv[k] = org[(uint32_t)(cx0 + id0 * cx1)];
+          i0 += st0;
+          j0 += st1;
+          if (i0 >= ss0) {
+            i0 -= ss0;
+            ++j0;
           }
         }
-        for (; k < len; ++k) {
-          acc += col[k * elem_step];
-          col[k * elem_step] = acc;
+#pragma unroll
+        for (int k = 0; k < KF; ++k) {
+          const double gg = std::is_same<TIN, float>::value ? (double)((float)gv[k] - meanG32)
+                                                            : (double)gv[k] - meanG64;
+          v[k] = ((data >> k) & 1u) ? gg * gg : 0.0;  // outside the data: 0, not -mean (zncc_ops.jl:22-24)
         }
+      };
+      if (dt == DT_F32) fill(float{});
+      else fill(double{});
+#pragma unroll
+      for (int k = 0; k < KF; ++k) satl[dst[k]] = v[k];
+    }
+    __syncthreads();
+    // Prefix sums down the columns, then along the rows: lines of at most L
+    // entries split into PS segments of RS entries; lane t scans segment
+    // t % PS of line t / PS in registers and adds the totals of the line's
+    // earlier segments, taken from the lanes beside it.  A lane writes back
+    // only the entries it read, so each pass needs no barrier inside.
+    constexpr int PS = C::TPI / L > 1 ? (C::TPI / L > 8 ? 8 : C::TPI / L) : 1;
+    constexpr int RS = (L + PS - 1) / PS;
+    const int seg = t % PS, ln = t / PS;
+    auto seg_offset = [&](double total) {
+      double inc = total;
+#pragma unroll
+      for (int d = 1; d < PS; d <<= 1) {
+        const double o = __shfl_up(inc, d, PS);
+        if (seg >= d) inc += o;
+      }
+      return inc - total;
+    };
+    auto pass = [&](int lines, int len, int line_stride, int elem_stride) {
+      for (int lb = 0; lb < lines; lb += C::TPI / PS) {
+        const int l = lb + ln;
+        // entries past the line (or of lines past the last) read and write
+        // the spare entry after the table; their values are never used
+        const int spare = Dl * (ss1 + 1);
+        const int lbase = (l + 1) * line_stride;
+        double w[RS];
+        int at[RS];
+#pragma unroll
+        for (int k = 0; k < RS; ++k) {
+          const int m = seg * RS + k;
+          at[k] = (l < lines && m < len) ? lbase + (m + 1) * elem_stride : spare;
+          w[k] = satl[at[k]];
+        }
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < RS; ++k) {
+          const int m = seg * RS + k;
+          acc += m < len ? w[k] : 0.0;
+          w[k] = acc;
+        }
+        const double off = seg_offset(acc);
+#pragma unroll
+        for (int k = 0; k < RS; ++k) satl[at[k]] = w[k] + off;
       }
       __syncthreads();
     };
-    scan(ss1, ss0, Dl, 1);
-    scan(ss0, ss1, 1, Dl);
+    pass(ss1, ss0, Dl, 1);  // down each column (dim 1)
+    pass(ss0, ss1, 1, Dl);  // along each row (dim 2)
     const double sqF = sqrt(sumF2);
     for (int e = t; e < nE; e += C::TPI) {
       const int s0 = w0, s1 = w1;
@@ -1176,22 +1257,6 @@ __device__ __forceinline__ void piv2d_pass(const Job& j0, const Job* __restrict_
   }
 }
 
-// One IA per block-slot.  SH: search regions shifted per IA (Job::shift,
-// multipass); a separate instantiation, so that kernels without shifts carry
-// none of its registers.
-// MF: the split-precision matrix-core correlation (mqpiv_mfma2d.hpp) in place of
-// the register FFTs; FP32 lengths 64 and 96 (4 or 8 lanes a line, 4 or 12
-// waves an IA, windows up to L x L), no shifts.
-// gz / gz_slots: the global parts of the transform buffers of a gz_len launch
-// (gz_slots of them after the slot flags), else unused.
-template <int L, int ALG, typename T, int LPL, bool SH, bool MF = false>
-__global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL, MF>())) void piv2d_kernel(Job j0, const Job* __restrict__ jobs,
-                                                                      const int64_t* __restrict__ prefix,
-                                                                      int n_jobs, int64_t n_ia_total, int ns0_cap,
-                                                                      int ns1_cap, char* gz, int gz_slots) {
-  piv2d_pass<L, ALG, T, LPL, SH, MF>(j0, jobs, prefix, n_jobs, n_ia_total, ns0_cap, ns1_cap, gz, gz_slots,
-                                     xcd_remap32((int32_t)blockIdx.x, (int32_t)gridDim.x));
-}
 
 // ---------------------------------------------------------------------------
 // Host-side launch helpers (explicitly instantiated per (T, L) in mqpiv_k2d_*.hip).
@@ -1201,7 +1266,7 @@ size_t lds_bytes_2d(int alg, int ns0, int ns1, int ss0) {
   using C = Cfg2<L, T, LPL>;
   size_t z = gz_len<L, T>() ? 0 : region_bytes_2d(L, sizeof(T), ns0, ns1, ss0);
   // FP64 ZNCC with a global transform buffer keeps its window-energy table in LDS
-  const size_t zw = (gz_len<L, T>() && alg == ALG_ZNCC) ? (size_t)(L + 1) * (L + 1) * sizeof(double) : 0;
+  const size_t zw = (gz_len<L, T>() && alg == ALG_ZNCC) ? ((size_t)(L + 1) * (L + 1) + 1) * sizeof(double) : 0;
   size_t c1 = (alg == ALG_MASK_NSQECC) ? sim_bytes(ns0, ns1, sizeof(T)) : 0;
   return (z + c1) * C::IPB + (size_t)C::IPB * kRedBytes + zw;
 }
