@@ -44,6 +44,7 @@ import json
 import math
 import os
 import sys
+import subprocess
 import time
 from pathlib import Path
 
@@ -309,14 +310,20 @@ def volume_bench(mode: str, steps: int, cpu: bool, nthreads: int) -> dict:
         from oracle import oracle as O
 
         ha, hb = a.cpu().numpy(), b.cpu().numpy()
-        n_s = 1500
+        n_s = 40 * nthreads
+        lo = 7200 - n_s // 2  # IAs from the middle of the volume
         op = O.make_params("fft", 3, 32, 8, 16, False, np.float32)
-        t0 = time.perf_counter()
-        O.piv(ha, hb, op, mode=O.FFT, nthreads=nthreads, ia_begin=0, ia_end=n_s)
-        dt = time.perf_counter() - t0
+        ts = []
+        for r in range(4):  # one warm-up, then the median of three
+            t0 = time.perf_counter()
+            O.piv(ha, hb, op, mode=O.FFT, nthreads=nthreads, ia_begin=lo, ia_end=lo + n_s)
+            if r:
+                ts.append(time.perf_counter() - t0)
+        dt = float(np.median(ts))
         out["cpu_baseline"] = {"value": round(n_s / dt, 1), "unit": "IA/s", "cores": nthreads, "kind": "port",
-                               "sample": f"FFTCC, the first {n_s} IAs of the workload, Float64 oracle FFT mode "
-                                         f"(48^3 transforms), OpenMP over IAs, {dt:.1f} s"}
+                               "sample": f"FFTCC, {n_s} IAs from the middle of the workload, Float64 oracle FFT "
+                                         f"mode (48^3 transforms), OpenMP over IAs; median of 3 runs after a "
+                                         f"warm-up ({dt:.2f} s each)"}
     return out
 
 
@@ -376,6 +383,70 @@ def batched_bench(mode: str, steps: int, seed: int = 4000) -> dict:
             "graph": {"ms_per_set": round(t_graph * 1e3, 3), "ia_per_s": round(n_ia / t_graph, 1)},
             "plain": {"ms_per_set": round(t_plain * 1e3, 3), "ia_per_s": round(n_ia / t_plain, 1)},
             "speedup": round(t_plain / t_graph, 3)}
+
+
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _mfma_executed_flop_per_ia(length):
+    """MFMA flops one IA executes in the split-precision form (the hi/lo terms
+    included), as rocprof's SQ_INSTS_VALU_MFMA_MOPS_F16 counted them."""
+    f = ROOT / "profiles" / "mfma_split_2048_nsqecc_zncc.md"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        if line.startswith(f"mfma_flop_per_ia[{length}]"):
+            return float(line.split("=")[1].split()[0])
+    return None
+
+
+def cpu_baseline_2d(ha, hb, jobs, nthreads: int, reps: int = 5) -> dict:
+    """The restated reference CPU path (the Float64 oracle in FFT mode: the
+    reference's per-IA structure, transforms at cs = good_size_real(ss)) on a
+    bounded sample of the headline step: the same contiguous run of IAs from
+    the middle of each of the four runs.  Timed single-threaded (the reference's
+    serial IA loop, multi_quickPIV.jl:63) and OpenMP over IAs on the host's
+    cores, each as the median of `reps` runs after one warm-up."""
+    from oracle import oracle as O
+
+    def timed(per_run: int, threads: int):
+        ts = []
+        for r in range(reps + 1):
+            t = 0.0
+            n = 0
+            for j in jobs:
+                op = O.make_params(j["alg"], 2, j["isz"], SM, STEP, True, np.float32)
+                lo = j["n"] // 2 - per_run // 2
+                t1 = time.perf_counter()
+                O.piv(ha, hb, op, mode=O.FFT, nthreads=threads, ia_begin=lo, ia_end=lo + per_run)
+                t += time.perf_counter() - t1
+                n += per_run
+            if r:  # the first run warms up
+                ts.append(t)
+        t_med = float(np.median(ts))
+        return n / t_med, t_med, n
+
+    one, t1, n1 = timed(200, 1)
+    allc, ta, na = timed(200 * nthreads, nthreads)
+    try:
+        nproc = int(subprocess.run(["nproc"], capture_output=True, text=True).stdout.strip())
+    except (OSError, ValueError):
+        nproc = os.cpu_count() or 0
+    return {"value": round(allc, 1), "unit": "IA/s", "cores": nthreads, "kind": "port",
+            "sample": f"{na} IAs ({na // 4} from the middle of each of the four runs of the step), "
+                      f"Float64 oracle FFT mode (per-IA transforms at the reference size), OpenMP over IAs on "
+                      f"{nthreads} threads; median of {reps} runs after a warm-up ({ta:.2f} s each)",
+            "single_thread": {"value": round(one, 1), "unit": "IA/s", "cores": 1,
+                              "sample": f"{n1} IAs ({n1 // 4} per run), median of {reps} runs after a warm-up "
+                                        f"({t1:.2f} s each)"},
+            "nproc": nproc, "cpu_model": _cpu_model()}
 
 
 def main() -> int:
@@ -530,6 +601,11 @@ def main() -> int:
         res["mfma"]["roofline"] = {"bound": "mfma", "achieved": round(ach, 3), "peak": PEAK_FP16_MFMA_TFLOPS,
                                    "unit": "TFLOP/s", "frac": round(ach / PEAK_FP16_MFMA_TFLOPS, 5),
                                    "flop_per_ia": round(flop_dft_per_ia(cs32))}
+        ex = _mfma_executed_flop_per_ia(cs32)
+        if ex is not None:
+            exr = ex * j["n"] / (res["mfma"]["kernel_ms"] * 1e-3) / 1e12
+            res["mfma"]["executed"] = {"flop_per_ia": ex, "tflops": round(exr, 2),
+                                       "frac": round(exr / PEAK_FP16_MFMA_TFLOPS, 4)}
         res["mfma_over_fp32"] = round(res["fp32"]["kernel_ms"] / res["mfma"]["kernel_ms"], 3)
         forms_is32[f'{j["alg"]}{j["isz"]}'] = res
 
@@ -575,21 +651,7 @@ def main() -> int:
 
     cpu = None
     if not args.no_cpu:
-        from oracle import oracle as O
-
-        ha = a.cpu().numpy()
-        hb = b.cpu().numpy()
-        t_cpu = 0.0
-        n_cpu = 0
-        for j in jobs:
-            op = O.make_params(j["alg"], 2, j["isz"], SM, STEP, True, np.float32)
-            t1 = time.perf_counter()
-            O.piv(ha, hb, op, mode=O.FFT, nthreads=nthreads)
-            t_cpu += time.perf_counter() - t1
-            n_cpu += j["n"]
-        cpu = {"value": round(n_cpu / t_cpu, 1), "unit": "IA/s", "cores": nthreads, "kind": "port",
-               "sample": f"the full step ({n_cpu} IAs: the four runs once), Float64 oracle FFT mode "
-                         f"(per-IA transforms at the reference size), OpenMP over IAs, {t_cpu:.1f} s"}
+        cpu = cpu_baseline_2d(a.cpu().numpy(), b.cpu().numpy(), jobs, nthreads)
 
     dtype = "f32+f64" if any(j["fp64"] for j in jobs) and not all(j["fp64"] for j in jobs) else (
         "f64" if all(j["fp64"] for j in jobs) else "f32")
