@@ -136,6 +136,21 @@ int mqpiv_piv_device(mqpiv_ctx* ctx, const mqpiv_params* p, const int64_t* dims,
                      const uint8_t* d_skip_mask, int32_t out_dtype, void* d_VF, void* d_SN,
                      void* stream);
 
+/* PIV(img1, img2, mask_VF, mask_IA, pivparams) (src/multi_quickPIV.jl:155-213):
+ * masked NSQECC (corr_alg must be MQPIV_MASK_NSQECC, 2-D) correlating with
+ * mask_IA, while the skip filter of skip_inter_region (src/utils/piv_ops.jl:
+ * 148-157, filtFun = mask_filtFun = maxval) reads image 1 and mask_VF.  The skip
+ * flags are computed on the device.  Host buffers (mqpiv_piv_masks; peaks as in
+ * mqpiv_piv_ex, nullable) or device buffers enqueued on `stream`
+ * (mqpiv_piv_masks_device; NULL = the context's stream).  With one mask passed
+ * as both, results equal mqpiv_piv's. */
+int mqpiv_piv_masks(mqpiv_ctx* ctx, const mqpiv_params* p, const int64_t* dims, int32_t in_dtype,
+                    const void* img1, const void* img2, const void* mask_vf, const void* mask_ia,
+                    int32_t out_dtype, void* VF, void* SN, int32_t* peaks);
+int mqpiv_piv_masks_device(mqpiv_ctx* ctx, const mqpiv_params* p, const int64_t* dims, int32_t in_dtype,
+                           const void* d_img1, const void* d_img2, const void* d_mask_vf,
+                           const void* d_mask_ia, int32_t out_dtype, void* d_VF, void* d_SN, void* stream);
+
 /* Multipass PIV: the coarse-to-fine loop `for scale in multipass:-1:1` of
  * PIV_CPU (src/multi_quickPIV.jl:57-88), with the search-region offset hook of
  * get_search_coordinates(..., displacement) (src/utils/piv_ops.jl:59-76) wired

@@ -47,6 +47,8 @@ EXPORTS = (
     "mqpiv_piv_ex",
     "mqpiv_piv_device",
     "mqpiv_piv_device_noalloc",
+    "mqpiv_piv_masks",
+    "mqpiv_piv_masks_device",
     "mqpiv_piv_multipass",
     "mqpiv_piv_multipass_device",
     "mqpiv_version",
@@ -167,6 +169,12 @@ def load() -> C.CDLL:
         lib.mqpiv_piv_device_noalloc.restype = C.c_int
         lib.mqpiv_piv_device_noalloc.argtypes = [vp, C.POINTER(MqpivParams), C.POINTER(C.c_int64), C.c_int32,
                                                  vp, vp, vp, vp, C.c_int32, vp, vp, vp]
+        lib.mqpiv_piv_masks.restype = C.c_int
+        lib.mqpiv_piv_masks.argtypes = [vp, C.POINTER(MqpivParams), C.POINTER(C.c_int64), C.c_int32,
+                                        vp, vp, vp, vp, C.c_int32, vp, vp, vp]
+        lib.mqpiv_piv_masks_device.restype = C.c_int
+        lib.mqpiv_piv_masks_device.argtypes = [vp, C.POINTER(MqpivParams), C.POINTER(C.c_int64), C.c_int32,
+                                               vp, vp, vp, vp, C.c_int32, vp, vp, vp]
         lib.mqpiv_piv_multipass.restype = C.c_int
         lib.mqpiv_piv_multipass.argtypes = [vp, C.POINTER(MqpivParams), C.c_int32, C.POINTER(C.c_int64),
                                             C.c_int32, vp, vp, vp, C.c_int32, vp, vp]

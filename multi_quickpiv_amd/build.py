@@ -59,6 +59,7 @@ SOURCES = [
     "mqpiv_kernels3d.hip",
     "mqpiv_sat.hip",
     "mqpiv_multipass.hip",
+    "mqpiv_skip.hip",
     "mqpiv_api.cpp",
     "mqpiv_server.cpp",
     "mqpiv_post.hip",

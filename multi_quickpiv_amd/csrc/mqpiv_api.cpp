@@ -40,6 +40,7 @@ hipError_t build_sat_pair(const void* img, int dt, const int64_t* dims, int nd, 
 int fft_len_3d(int fp64, int ss);
 size_t scratch_3d_bytes(int fp64, int L, const Job& j);
 hipError_t launch_piv3d(int fp64, int L, const Job& j, void* scratch, const double* sat1_img1, hipStream_t s);
+hipError_t launch_skip_flags_2d(const Job& j, const void* mask_vf, uint8_t* flags, hipStream_t s);
 hipError_t launch_coarse_to_shift(const double* vfc, const int32_t* nc, const int32_t* isc, const int32_t* stc,
                                   const int32_t* nf, const int32_t* isf, const int32_t* stf, int nd,
                                   int64_t n_fine, int32_t* out, hipStream_t s);
@@ -141,6 +142,7 @@ extern "C" int mqpiv_ctx_destroy(mqpiv_ctx* c) {
   if (c->d_pp) hipFree(c->d_pp);
   if (c->d_mp) hipFree(c->d_mp);
   if (c->d_gz) hipFree(c->d_gz);
+  if (c->d_flags) hipFree(c->d_flags);
   if (c->table_done) hipEventDestroy(c->table_done);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
@@ -444,10 +446,13 @@ extern "C" int mqpiv_piv(mqpiv_ctx* c, const mqpiv_params* p, const int64_t* dim
 // Host buffers: validates the request, copies the inputs into the context's
 // staging buffer, runs `run` on the job with device pointers, copies the outputs
 // back and synchronises.
+// mask_vf (double-mask calls): the skip filter's mask, staged beside the
+// others; the flags it gives (one byte per IA) become the job's skip mask.
 template <typename Run>
 static int run_staged(mqpiv_ctx* c, const mqpiv_params* p, const int64_t* dims, int32_t in_dtype,
                       const void* img1, const void* img2, const void* mask, const uint8_t* skip,
-                      int32_t out_dtype, void* VF, void* SN, int32_t* peaks, Run&& run) {
+                      int32_t out_dtype, void* VF, void* SN, int32_t* peaks, Run&& run,
+                      const void* mask_vf = nullptr) {
   if (!c) return MQPIV_ERR_NULL;
   std::lock_guard<std::mutex> lk(c->mu);
   if (hipSetDevice(c->device) != hipSuccess) return fail(c, MQPIV_ERR_DEVICE, "hipSetDevice failed");
@@ -460,11 +465,12 @@ static int run_staged(mqpiv_ctx* c, const mqpiv_params* p, const int64_t* dims, 
   auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
   const size_t b_img = al(npx * esz);
   const size_t b_mask = (p->corr_alg == MQPIV_MASK_NSQECC) ? b_img : 0;
-  const size_t b_skip = skip ? al(j.n_ia) : 0;
+  const size_t b_skip = (skip || mask_vf) ? al(j.n_ia) : 0;
+  const size_t b_mvf = mask_vf ? b_img : 0;
   const size_t b_vf = al(j.n_ia * j.ndims * osz);
   const size_t b_sn = j.sn ? al(j.n_ia * osz) : 0;
   const size_t b_pk = peaks ? al(j.n_ia * j.ndims * sizeof(int32_t)) : 0;
-  const size_t total = 2 * b_img + b_mask + b_skip + b_vf + b_sn + b_pk;
+  const size_t total = 2 * b_img + b_mask + b_mvf + b_skip + b_vf + b_sn + b_pk;
   if (c->budget && total > c->budget)
     return fail(c, MQPIV_ERR_OOM, "request needs %zu bytes, over the HBM budget %llu", total,
                 (unsigned long long)c->budget);
@@ -482,7 +488,8 @@ static int run_staged(mqpiv_ctx* c, const mqpiv_params* p, const int64_t* dims, 
   char* d1 = b;
   char* d2 = d1 + b_img;
   char* dm = d2 + b_img;
-  char* dk = dm + b_mask;
+  char* dmv = dm + b_mask;
+  char* dk = dmv + b_mvf;
   char* dv = dk + b_skip;
   char* ds = dv + b_vf;
   char* dp = ds + b_sn;
@@ -490,11 +497,16 @@ static int run_staged(mqpiv_ctx* c, const mqpiv_params* p, const int64_t* dims, 
   HIP_OR_FAIL(c, hipMemcpyAsync(d1, img1, npx * esz, hipMemcpyHostToDevice, s));
   HIP_OR_FAIL(c, hipMemcpyAsync(d2, img2, npx * esz, hipMemcpyHostToDevice, s));
   if (b_mask) HIP_OR_FAIL(c, hipMemcpyAsync(dm, mask, npx * esz, hipMemcpyHostToDevice, s));
-  if (b_skip) HIP_OR_FAIL(c, hipMemcpyAsync(dk, skip, j.n_ia, hipMemcpyHostToDevice, s));
+  if (b_mvf) HIP_OR_FAIL(c, hipMemcpyAsync(dmv, mask_vf, npx * esz, hipMemcpyHostToDevice, s));
+  if (skip) HIP_OR_FAIL(c, hipMemcpyAsync(dk, skip, j.n_ia, hipMemcpyHostToDevice, s));
   j.img1 = d1;
   j.img2 = d2;
   j.mask = b_mask ? dm : nullptr;
-  j.skip = b_skip ? (const uint8_t*)dk : nullptr;
+  j.skip = skip ? (const uint8_t*)dk : nullptr;
+  if (mask_vf && (j.has_threshold || j.has_mask_threshold)) {
+    HIP_OR_FAIL(c, launch_skip_flags_2d(j, dmv, (uint8_t*)dk, s));
+    j.skip = (const uint8_t*)dk;
+  }
   j.vf = dv;
   j.sn = b_sn ? ds : nullptr;
   j.peaks = b_pk ? (int32_t*)dp : nullptr;
@@ -515,6 +527,49 @@ extern "C" int mqpiv_piv_ex(mqpiv_ctx* c, const mqpiv_params* p, const int64_t* 
                             int32_t out_dtype, void* VF, void* SN, int32_t* peaks) {
   return run_staged(c, p, dims, in_dtype, img1, img2, mask, skip, out_dtype, VF, SN, peaks,
                     [&](Job& j, hipStream_t s) { return launch_job(c, j, s); });
+}
+
+// Checks of the double-mask form: the masked measure, 2-D, both masks given.
+static int check_masks(mqpiv_ctx* c, const mqpiv_params* p, const void* mask_vf, const void* mask_ia) {
+  if (!p) return fail(c, MQPIV_ERR_NULL, "params is NULL");
+  if (p->corr_alg != MQPIV_MASK_NSQECC)
+    return fail(c, MQPIV_ERR_BAD_PARAMS, "the two-mask form runs the masked NSQECC measure");
+  if (p->ndims != 2) return fail(c, MQPIV_ERR_UNSUPPORTED, "masked NSQECC is built for 2-D only");
+  if (!mask_vf || !mask_ia) return fail(c, MQPIV_ERR_NULL, "a mask pointer is NULL");
+  return MQPIV_OK;
+}
+
+extern "C" int mqpiv_piv_masks(mqpiv_ctx* c, const mqpiv_params* p, const int64_t* dims, int32_t in_dtype,
+                               const void* img1, const void* img2, const void* mask_vf, const void* mask_ia,
+                               int32_t out_dtype, void* VF, void* SN, int32_t* peaks) {
+  if (!c) return MQPIV_ERR_NULL;
+  int rc = check_masks(c, p, mask_vf, mask_ia);
+  if (rc) return rc;
+  return run_staged(
+      c, p, dims, in_dtype, img1, img2, mask_ia, nullptr, out_dtype, VF, SN, peaks,
+      [&](Job& j, hipStream_t s) { return launch_job(c, j, s); }, mask_vf);
+}
+
+extern "C" int mqpiv_piv_masks_device(mqpiv_ctx* c, const mqpiv_params* p, const int64_t* dims, int32_t in_dtype,
+                                      const void* d_img1, const void* d_img2, const void* d_mask_vf,
+                                      const void* d_mask_ia, int32_t out_dtype, void* d_VF, void* d_SN,
+                                      void* stream) {
+  if (!c) return MQPIV_ERR_NULL;
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, MQPIV_ERR_DEVICE, "hipSetDevice failed");
+  int rc = check_masks(c, p, d_mask_vf, d_mask_ia);
+  if (rc) return rc;
+  Job j;
+  rc = build_job(c, p, dims, in_dtype, d_img1, d_img2, d_mask_ia, nullptr, out_dtype, d_VF, d_SN, &j);
+  if (rc) return rc;
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  if (j.n_ia > 0 && (j.has_threshold || j.has_mask_threshold)) {
+    rc = ensure(c, &c->d_flags, &c->flags_cap, (size_t)j.n_ia, s);
+    if (rc) return rc;
+    HIP_OR_FAIL(c, launch_skip_flags_2d(j, d_mask_vf, (uint8_t*)c->d_flags, s));
+    j.skip = (const uint8_t*)c->d_flags;
+  }
+  return launch_job(c, j, s);
 }
 
 // ---------------------------------------------------------------------------

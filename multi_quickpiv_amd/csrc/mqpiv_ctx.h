@@ -41,6 +41,9 @@ struct mqpiv_ctx {
   // multipass: two coarse vector fields (FP64) and one pass's search-region shifts
   void* d_mp = nullptr;
   size_t mp_cap = 0;
+  // double-mask calls: per-IA skip flags computed on the device
+  void* d_flags = nullptr;
+  size_t flags_cap = 0;
   // measurement hooks
   bool profiling = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> events;

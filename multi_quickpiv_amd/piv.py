@@ -81,9 +81,9 @@ def _ia_slices(vf_idx: Tuple[int, ...], p: PIVParameters) -> Tuple[slice, ...]:
 
 def host_skip_mask(img1: np.ndarray, mask: Optional[np.ndarray], p: PIVParameters,
                    masked: bool) -> Optional[np.ndarray]:
-    """Per-IA skip flags for filters the device does not evaluate itself:
-    a user ``filtFun``/``mask_filtFun`` other than ``maxval`` (piv_ops.jl:134-157),
-    or the double-mask variant where the skip mask differs from the IA mask."""
+    """Per-IA skip flags for filters the device does not evaluate itself: a user
+    ``filtFun``/``mask_filtFun`` other than ``maxval`` (piv_ops.jl:134-157).  In
+    the double-mask form ``mask`` is mask_VF."""
     vf = get_vectorfield_size(img1.shape, p)
     n = int(np.prod(vf))
     out = np.zeros(n, dtype=np.uint8)
@@ -149,7 +149,7 @@ def PIV(input1: Any, input2: Any, *args: Any, precision: Any = np.float64,  # no
     mp = int(pivparams.multipass)
     if mp > 1 and (custom or len(masks) == 2 or return_peaks):
         raise NotImplementedError("multipass > 1 takes the default filtFun, one mask and no peaks output")
-    if custom or len(masks) == 2:
+    if custom:
         skip = host_skip_mask(img1, m_vf, pivparams, masked)
     if int(np.prod(vf)) == 0:
         return (VF, SN, PK) if return_peaks else (VF, SN)
@@ -165,6 +165,16 @@ def PIV(input1: Any, input2: Any, *args: Any, precision: Any = np.float64,  # no
             out_code, VF.ctypes.data, SN.ctypes.data if SN is not None else None)
         ctx.check(rc)
         return VF, SN
+    if len(masks) == 2 and skip is None:
+        # multi_quickPIV.jl:155-213: skip filter on mask_VF, correlation on mask_IA;
+        # the flags come from the device
+        mv = np.asfortranarray(m_vf, dtype=dt)
+        rc = ctx.lib.mqpiv_piv_masks(
+            ctx.handle, C.byref(cp), dims, in_code, a1.ctypes.data, a2.ctypes.data, mv.ctypes.data,
+            m_ia.ctypes.data, out_code, VF.ctypes.data, SN.ctypes.data if SN is not None else None,
+            PK.ctypes.data if PK is not None else None)
+        ctx.check(rc)
+        return (VF, SN, PK) if return_peaks else (VF, SN)
     rc = ctx.lib.mqpiv_piv_ex(
         ctx.handle, C.byref(cp), dims, in_code,
         a1.ctypes.data, a2.ctypes.data,

@@ -860,6 +860,25 @@ int orc_piv(const orc_params* p, const int64_t* dims, const void* img1, const vo
                      VF, SN, aux);
 }
 
+/* Skip flags (one byte per IA, 1 = skipped) of skip_inter_region for every IA
+ * of the image (piv_ops.jl:134-157).  For the masked measure `mask` is the
+ * mask the filter reads: PIV_CPU_masked with two masks passes mask_VF here
+ * (multi_quickPIV.jl:186) and mask_IA to the correlation (:189), so
+ * orc_piv(mask = mask_IA, skip_mask = these flags) restates that driver. */
+int orc_skip_flags(const orc_params* p, const int64_t* dims, const void* img1, const void* mask,
+                   uint8_t* flags) {
+  if (p->ndims < 2 || p->ndims > 3) return -1;
+  if (p->alg == ORC_MASK_NSQECC && p->mask_threshold > 0 && !mask) return -1;
+  int64_t vf[3];
+  const int64_t n = orc_vf_size(p, dims, vf);
+  for (int64_t i = 0; i < n; ++i) {
+    orc_coords c;
+    orc_coords_of(p, dims, vf, i + 1, &c);
+    flags[i] = (uint8_t)skip_ia(p, dims, img1, mask, &c);
+  }
+  return 0;
+}
+
 /* orc_piv with per-IA search-region shifts (3 int32 per IA; NULL = none). */
 static int piv_shifted(const orc_params* p, const int64_t* dims, const void* img1, const void* img2,
                        const void* mask, const uint8_t* skip_mask, const int32_t* shift, int mode,

@@ -52,6 +52,8 @@ def load() -> C.CDLL:
         lib.orc_piv.restype = C.c_int
         lib.orc_piv.argtypes = [C.POINTER(OrcParams), C.POINTER(C.c_int64), vp, vp, vp, vp, C.c_int,
                                 C.c_int, C.c_int64, C.c_int64, vp, vp, vp]
+        lib.orc_skip_flags.restype = C.c_int
+        lib.orc_skip_flags.argtypes = [C.POINTER(OrcParams), C.POINTER(C.c_int64), vp, vp, vp]
         lib.orc_piv_multipass.restype = C.c_int
         lib.orc_piv_multipass.argtypes = [C.POINTER(OrcParams), C.c_int, C.POINTER(C.c_int64), vp, vp, vp,
                                           C.c_int, C.c_int, vp, vp]
@@ -157,6 +159,22 @@ def piv(img1, img2, p: OrcParams, mask=None, skip_mask=None, mode: int = DIRECT,
         if rc != 0:
             raise ValueError(f"orc_piv failed ({rc})")
     return VF, SN, aux
+
+
+def skip_flags(img1, p: OrcParams, mask=None) -> np.ndarray:
+    """Skip flags of skip_inter_region for every IA (uint8, 1 = skipped, IA order)
+    read from image 1 and, for the masked measure, ``mask`` (mask_VF in the
+    two-mask driver)."""
+    lib = load()
+    dt = np.float32 if p.in_dtype == 0 else np.float64
+    a1, am = _prep(img1, dt), _prep(mask, dt)
+    n = int(np.prod(vf_size(p, a1.shape)))
+    out = np.zeros(max(n, 1), dtype=np.uint8)
+    rc = lib.orc_skip_flags(C.byref(p), _i3(a1.shape, 1), a1.ctypes.data,
+                            am.ctypes.data if am is not None else None, out.ctypes.data)
+    if rc != 0:
+        raise ValueError(f"orc_skip_flags failed ({rc})")
+    return out[:n]
 
 
 def piv_multipass(img1, img2, p: OrcParams, multipass: int, mask=None, mode: int = DIRECT,

@@ -109,3 +109,82 @@ def test_masked_fully_excluded_windows_give_nan(oracle):
     assert VF[0, 0, 0] == -9 and VF[1, 0, 0] == -9
     assert np.isnan(SN[0, 0])
     assert rep.ok(), rep
+
+
+def _masks(shape):
+    """An ellipse IA mask and a vector-field mask that excludes a band of rows
+    and fades elsewhere, so that its maxval filter skips some IAs."""
+    m_ia = synth.ellipse_mask(shape, n=6, cover=0.3, seed=5).numpy()
+    m_vf = np.full(shape, 0.8, dtype=np.float32)
+    m_vf[: shape[0] // 3, :] = 0.2
+    m_vf[:, -40:] = 0.0
+    return m_ia, m_vf
+
+
+def test_mask_threshold_on_device(oracle):
+    """maskThreshold > 0 with one mask: the device evaluates maxval of the IA mask
+    (piv_ops.jl:148-157)."""
+    a, b = _pair((192, 160), seed=17)
+    m_ia, _ = _masks(a.shape)
+    rep, VF, SN, PK, VFo, SNo, aux = _run(oracle, a, b, "mask_nsqecc", 16, 8, 8, mask=m_ia,
+                                          mask_threshold=0.5)
+    skipped = (VFo[0] == 0) & (VFo[1] == 0) & (SNo == 0)
+    assert skipped.any() and (~skipped).any()
+    assert np.all(VF[:, skipped] == 0) and np.all(SN[skipped] == 0)
+    assert rep.ok(), rep
+
+
+@pytest.mark.parametrize("thr,mthr", [(-1.0, 0.5), (0.3, 0.5), (0.3, -1.0)])
+def test_two_masks_match_oracle(oracle, thr, mthr):
+    """PIV(img1, img2, mask_VF, mask_IA, p) (multi_quickPIV.jl:155-213): skip flags
+    from image 1 and mask_VF on the device, correlation on mask_IA."""
+    a, b = _pair((192, 160), seed=19)
+    a[:, :24] *= 0.25  # a dim band whose maxima fall below the image threshold
+    m_ia, m_vf = _masks(a.shape)
+    p = setPIVParameters(corr_alg="nsqecc", interSize=16, searchMargin=8, step=8, computeSN=True,
+                         threshold=thr, mask_threshold=mthr)
+    print(f"case two masks thr={thr} mthr={mthr}", flush=True)
+    VF, SN, PK = PIV(a, b, m_vf, m_ia, p, return_peaks=True)
+    op = oracle.make_params("mask_nsqecc", 2, 16, 8, 8, True, a.dtype, thr, mthr)
+    fl = oracle.skip_flags(a, op, mask=m_vf)
+    VFo, SNo, aux = oracle.piv(a, b, op, mask=m_ia, skip_mask=fl, mode=oracle.DIRECT, nthreads=8,
+                               want_aux=True)
+    rep = compare(VF, SN, PK, VFo, SNo, aux)
+    print(rep, flush=True)
+    skipped = fl.reshape(VF.shape[1:], order="F").astype(bool)
+    assert skipped.any() and (~skipped).any()
+    assert np.all(VF[:, skipped] == 0) and np.all(SN[skipped] == 0)
+    assert rep.ok(), rep
+
+
+def test_two_masks_device_entry_equals_host_entry():
+    """mqpiv_piv_masks_device on resident tensors gives the host entry's outputs."""
+    import ctypes as C
+
+    from multi_quickpiv_amd import _lib
+    from multi_quickpiv_amd.piv import to_c_params
+
+    a, b = _pair((192, 160), seed=23)
+    m_ia, m_vf = _masks(a.shape)
+    p = setPIVParameters(corr_alg="nsqecc", interSize=16, searchMargin=8, step=8, computeSN=True,
+                         threshold=0.3, mask_threshold=0.5)
+    VF, SN = PIV(a, b, m_vf, m_ia, p)
+    dev = [torch.from_numpy(np.asfortranarray(x).ravel(order="F")).cuda() for x in (a, b, m_vf, m_ia)]
+    n = int(np.prod(SN.shape))
+    dVF = torch.zeros(2 * n, dtype=torch.float64, device="cuda")
+    dSN = torch.zeros(n, dtype=torch.float64, device="cuda")
+    from multi_quickpiv_amd.params import mask_NSQECC
+
+    cp = to_c_params(p, mask_NSQECC.code, "auto")
+    dims = (C.c_int64 * 3)(a.shape[0], a.shape[1], 1)
+    ctx = _lib.context(0)
+    torch.cuda.synchronize()
+    rc = ctx.lib.mqpiv_piv_masks_device(ctx.handle, C.byref(cp), dims, _lib.MQPIV_F32,
+                                        *[t.data_ptr() for t in dev], _lib.MQPIV_F64, dVF.data_ptr(),
+                                        dSN.data_ptr(), None)
+    ctx.check(rc)
+    torch.cuda.synchronize()
+    got_vf = dVF.cpu().numpy().reshape(VF.shape, order="F")
+    got_sn = dSN.cpu().numpy().reshape(SN.shape, order="F")
+    np.testing.assert_array_equal(got_vf, VF)
+    np.testing.assert_array_equal(got_sn, SN)

@@ -206,3 +206,59 @@ def test_zncc_peak_is_one_for_an_exact_copy_with_offset(oracle):
     Z, co = O.ia_similarity(a, b, p, 6)
     assert co[4][0] == 0 and co[4][1] == 0
     assert Z[6 + 3, 6 + 2] == pytest.approx(1.0, abs=1e-12)
+
+
+def _skip_def(img1, mask, p, masked):
+    """skip_inter_region by brute force (piv_ops.jl:134-157, maxval filter)."""
+    vf = O.vf_size(p, img1.shape)
+    out = []
+    for lin in range(int(np.prod(vf))):
+        v = np.unravel_index(lin, vf, order="F")
+        sl = tuple(slice(v[d] * p.st[d], v[d] * p.st[d] + p.is_[d]) for d in range(2))
+        if masked:
+            s = p.threshold > 0 and img1[sl].max() < p.threshold
+            s = s or (p.mask_threshold > 0 and mask[sl].max() < p.mask_threshold)
+        else:
+            s = p.threshold >= 0 and img1[sl].max() < p.threshold
+        out.append(int(s))
+    return np.array(out, dtype=np.uint8)
+
+
+@pytest.mark.parametrize("masked,thr,mthr", [(True, 0.5, 0.5), (True, -1.0, 0.5), (True, 0.5, -1.0),
+                                             (False, 0.5, -1.0), (False, 0.0, -1.0)])
+def test_skip_flags_match_definition(oracle, masked, thr, mthr):
+    """Per-IA skip flags, with IA maxima placed exactly at the thresholds (strict <)."""
+    r = np.random.default_rng(31)
+    a = r.uniform(0.0, 0.45, (40, 36))
+    m = r.uniform(0.0, 0.45, (40, 36))
+    a[:, 18:] += 0.5  # image 1 passes the threshold on the right, the mask at the bottom
+    m[20:, :] += 0.5
+    a[3, 4] = 0.5     # the first IA of image 1 reaches the threshold exactly: kept by it
+    m[5, 25] = 0.5    # a top-right IA of the mask reaches it exactly
+    a[24:, :10] -= 1.0  # negative maxima: skipped at threshold 0 too
+    p = O.make_params("mask_nsqecc" if masked else "nsqecc", 2, (8, 6), (2, 2), (5, 4), True,
+                      np.float64, thr, mthr)
+    got = O.skip_flags(a, p, mask=m if masked else None)
+    ref = _skip_def(a, m, p, masked)
+    np.testing.assert_array_equal(got, ref)
+    assert got.any() and not got.all()
+
+
+def test_two_mask_driver_reads_mask_vf_for_skips(oracle):
+    """The two-mask restatement: flags from mask_VF change which IAs run; with
+    mask_VF = mask_IA it equals the one-mask call."""
+    a, b = _pair((48, 44), 41)
+    m_ia = _binary_mask(a.shape, 42)
+    m_vf = np.ones_like(m_ia)
+    m_vf[:20, :] = 0.0
+    p = O.make_params("mask_nsqecc", 2, (12, 10), (5, 4), (8, 7), True, np.float64, -1.0, 0.5)
+    one, sn1, _ = O.piv(a, b, p, mask=m_ia)
+    same, sn2, _ = O.piv(a, b, p, mask=m_ia, skip_mask=O.skip_flags(a, p, mask=m_ia))
+    np.testing.assert_array_equal(one, same)
+    fl = O.skip_flags(a, p, mask=m_vf)
+    two, _, _ = O.piv(a, b, p, mask=m_ia, skip_mask=fl)
+    vf = O.vf_size(p, a.shape)
+    skipped = fl.reshape(vf, order="F").astype(bool)
+    assert skipped.any() and (~skipped).any()
+    assert np.all(two[:, skipped] == 0)
+    np.testing.assert_array_equal(two[:, ~skipped], one[:, ~skipped])
