@@ -35,6 +35,7 @@ DEVICE_FLAGS = [
 
 SOURCES = [
     # longest transforms first so the parallel build finishes sooner
+    "mqpiv_k2d_wave.hip",
     "mqpiv_k2d_f64_128.hip",
     "mqpiv_k2d_f32_128.hip",
     "mqpiv_k2d_f32_96.hip",

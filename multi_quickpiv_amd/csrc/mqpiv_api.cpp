@@ -31,6 +31,7 @@ size_t lds_2d(int fp64, int L, int alg, int ns0, int ns1, int ss0);
 hipError_t launch_piv2d(int fp64, int L, int alg, const Job& j0, const Job* d_jobs, const int64_t* d_prefix,
                         int n_jobs, int64_t n_ia_total, int ns0, int ns1, int ss0, void* gz, hipStream_t s);
 size_t gz_scratch_2d(int fp64, int L, int64_t n_ia);
+int wave2d_len(int alg, int is0, int is1, int ss, int ns0, int ns1);
 size_t sat_bytes(const int64_t* dims, int nd);
 size_t sat_tmp_bytes(const int64_t* dims, int nd);
 hipError_t build_sat(const void* img, int dt, int power, const int64_t* dims, int nd, double* S, double* tmp,
@@ -259,7 +260,7 @@ static int build_job(mqpiv_ctx* c, const mqpiv_params* p, const int64_t* dims, i
 // arrays a job needs: count, bytes of one (256-aligned) and of the prefix-scan
 // temporary.
 static int sat_count(int alg, int ndims, int fp64 = 0) {
-  if (alg == ALG_ZNCC) return ndims == 3 ? 3 : (fp64 ? 0 : 2);
+  if (alg == ALG_ZNCC) return ndims == 3 ? 3 : (fp64 == 1 ? 0 : 2);
   return alg == ALG_NSQECC ? 1 : 0;
 }
 
@@ -323,11 +324,14 @@ static int choose_2d(mqpiv_ctx* c, const Job& j, int* fp64_out, int* L_out) {
   };
   int fp64 = 0;
   if (j.precision_mode == MQPIV_SPLIT_MFMA) {
-    if (ss > 96)
-      return fail(c, MQPIV_ERR_UNSUPPORTED, "the split-precision matrix-core form takes 2-D search sizes up to 96 (got %d)",
-                  ss);
-    *fp64_out = 0;
-    *L_out = ss <= 64 ? 64 : 96;
+    const int L = wave2d_len(j.alg, j.is[0], j.is[1], ss, ns0, ns1);
+    if (L < 0)
+      return fail(c, MQPIV_ERR_UNSUPPORTED,
+                  "the split-precision matrix-core form takes FFTCC, ZNCC and NSQECC with interSize <= 32, search "
+                  "size <= 64 and margin <= 23 (got interSize %d x %d, search size %d)",
+                  j.is[0], j.is[1], ss);
+    *fp64_out = 2;  // the matrix-core kernel (launch_piv2d)
+    *L_out = L;
     return MQPIV_OK;
   }
   if (j.precision_mode == MQPIV_FP64) {
@@ -729,8 +733,6 @@ static int plan_batch(mqpiv_ctx* c, const mqpiv_request* r, int n, BatchPlan& P)
     int rc = build_job(c, &r[i].params, r[i].dims, r[i].in_dtype, r[i].img1, r[i].img2, r[i].mask, r[i].skip_mask,
                        r[i].out_dtype, r[i].VF, r[i].SN, &j);
     if (rc) return fail(c, rc, "request %d: %s", i, c->err.c_str());
-    if (j.precision_mode == MQPIV_SPLIT_MFMA)
-      return fail(c, MQPIV_ERR_UNSUPPORTED, "request %d: the split-precision matrix-core form runs single calls", i);
     j.peaks = r[i].peaks;
     if (j.n_ia == 0) continue;
     if (j.ndims == 3) {

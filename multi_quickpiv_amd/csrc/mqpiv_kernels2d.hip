@@ -51,6 +51,12 @@ extern template size_t lds_bytes_2d<128, double, Lpl2<double, 128>::v>(int, int,
 extern template hipError_t launch_2d_l<128, double, Lpl2<double, 128>::v>(int, const Job&, const Job*, const int64_t*, int,
                                                     int64_t, int, int, int, char*, hipStream_t);
 
+// The split-precision matrix-core form (precision 2 below; mqpiv_k2d_wave.hip).
+int wave2d_len(int alg, int is0, int is1, int ss, int ns0, int ns1);
+size_t wave2d_lds(int L, int ns0, int ns1);
+hipError_t launch_wave2d(int L, int alg, const Job& j0, const Job* d_jobs, const int64_t* d_prefix, int n_jobs,
+                         int64_t n_ia_total, int ns0, int ns1, hipStream_t s);
+
 // Transform length for a search size: the smallest supported length >= ss
 // (up to 128; FP64 at 128 keeps its transform buffer in global memory), or -1.
 int fft_len_2d(int fp64, int ss) {
@@ -66,7 +72,10 @@ int fft_len_2d(int fp64, int ss) {
   return -1;
 }
 
+// fp64: 0 FP32 radix transforms, 1 FP64 radix transforms, 2 the split-precision
+// matrix-core form.
 size_t lds_2d(int fp64, int L, int alg, int ns0, int ns1, int ss0) {
+  if (fp64 == 2) return wave2d_lds(L, ns0, ns1);
   if (fp64) {
     switch (L) {
       case 16: return lds_bytes_2d<16, double, Lpl2<double, 16>::v>(alg, ns0, ns1, ss0);
@@ -93,13 +102,14 @@ size_t lds_2d(int fp64, int L, int alg, int ns0, int ns1, int ss0) {
 // n_ia IAs at (fp64, L) needs, or 0 for lengths whose buffer is in LDS.  The
 // flags must be zero when a launch starts; every launch leaves them zero.
 size_t gz_scratch_2d(int fp64, int L, int64_t n_ia) {
-  if (!(fp64 && gz_len<128, double>() && L == 128) || n_ia <= 0) return 0;
+  if (!(fp64 == 1 && gz_len<128, double>() && L == 128) || n_ia <= 0) return 0;
   return kGzFlagBytes + (size_t)gz_slots_2d(n_ia) * gz_stride_2d<128, double>();
 }
 
 hipError_t launch_piv2d(int fp64, int L, int alg, const Job& j0, const Job* d_jobs, const int64_t* d_prefix,
                         int n_jobs, int64_t n_ia_total, int ns0, int ns1, int ss0, void* gz, hipStream_t s) {
   if (n_ia_total <= 0) return hipSuccess;
+  if (fp64 == 2) return launch_wave2d(L, alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, s);
   if (fp64) {
     switch (L) {
       case 16: return launch_2d_l<16, double, Lpl2<double, 16>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, (char*)gz, s);

@@ -247,28 +247,6 @@ struct OpMaxNaN {
   }
 };
 
-}  // namespace mqpiv
-#include "mqpiv_mfma2d.hpp"
-namespace mqpiv {
-
-// The matrix-core form runs FP32 lengths 64 and 96 when one IA's block has a
-// whole number of waves per 16-row block and of product entries per lane.
-template <int L, typename T, int LPL>
-constexpr bool mf_ok() {
-  if constexpr ((L == 64 || L == 96) && std::is_same<T, float>::value) {
-    using C = Cfg2<L, T, LPL>;
-    return C::IPB == 1 && (C::TPI / 64) % (L / 16) == 0 && (L * MfCfg<L>::K1) % C::TPI == 0;
-  } else {
-    return false;
-  }
-}
-template <bool MF, int L>
-constexpr size_t mf_bytes() {
-  if constexpr (MF) return MfCfg<L>::region;
-  else return 0;
-}
-
-
 // (value, linear index) argmax with the reference's tie-break: the first element
 // in column-major order whose value is strictly above everything before it,
 // starting from 0 (common_ops.jl:112-121).  i = INT32_MAX means "none".
@@ -530,11 +508,8 @@ __device__ __forceinline__ double sat_box(const double* __restrict__ S, int64_t 
 #ifndef MQPIV_OCC_CAP
 #define MQPIV_OCC_CAP 8
 #endif
-template <int L, int ALG, typename T, int LPL, bool MF = false>
+template <int L, int ALG, typename T, int LPL>
 constexpr int min_waves_2d() {
-  // the matrix-core form holds many operand fragments: 4 waves per SIMD (128 VGPRs)
-  // at length 64; at 96 one 12-wave IA fills a CU either way, so 3 (168 VGPRs)
-  if constexpr (MF) return L == 96 ? 3 : 4;
   // global transform buffer: one IA's block per CU, whose waves all fit the
   // SIMDs together (16 waves: 4 per SIMD, 128 VGPRs)
   if constexpr (gz_len<L, T>()) return Cfg2<L, T, LPL>::NT / 256 > 0 ? Cfg2<L, T, LPL>::NT / 256 : 1;
@@ -550,13 +525,10 @@ constexpr int min_waves_2d() {
 // One IA per block-slot.  SH: search regions shifted per IA (Job::shift,
 // multipass); a separate instantiation, so that kernels without shifts carry
 // none of its registers.
-// MF: the split-precision matrix-core correlation (mqpiv_mfma2d.hpp) in place of
-// the register FFTs; FP32 lengths 64 and 96 (4 or 8 lanes a line, 4 or 12
-// waves an IA, windows up to L x L), no shifts.
 // gz / gz_slots: the global parts of the transform buffers of a gz_len launch
 // (gz_slots of them after the slot flags), else unused.
-template <int L, int ALG, typename T, int LPL, bool SH, bool MF = false>
-__global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL, MF>())) void piv2d_kernel(Job j0, const Job* __restrict__ jobs,
+template <int L, int ALG, typename T, int LPL, bool SH>
+__global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL>())) void piv2d_kernel(Job j0, const Job* __restrict__ jobs,
                                                                       const int64_t* __restrict__ prefix,
                                                                       int n_jobs, int64_t n_ia_total, int ns0_cap,
                                                                       int ns1_cap, char* gz, int gz_slots) {
@@ -564,7 +536,7 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
   using C = Cfg2<L, T, LPL>;
   constexpr int H = C::H;
   constexpr bool GZ = gz_len<L, T>();
-  static_assert(!GZ || (C::IPB == 1 && !MF), "a global transform buffer holds one IA");
+  static_assert(!GZ || C::IPB == 1, "a global transform buffer holds one IA");
   extern __shared__ __align__(16) unsigned char smem[];
 
   for (;;) {  // runs once (a loop keeps the register allocation of this body as measured)
@@ -597,10 +569,8 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
 
   // LDS layout is uniform across the launch (jobs of a batch may differ in
   // their margins): the host passes the largest ns0, ns1 of the launch
-  static_assert(!MF || (mf_ok<L, T, LPL>() && !SH && ALG != ALG_MASK_NSQECC),
-                "matrix-core form: FP32 lengths 64 and 96, unshifted, unmasked");
   constexpr int LDc = L + 1;
-  const size_t zbytes = GZ ? 0 : (MF ? mf_bytes<MF, L>() : region_bytes_2d(L, sizeof(T), ns0_cap, ns1_cap, ss0));
+  const size_t zbytes = GZ ? 0 : region_bytes_2d(L, sizeof(T), ns0_cap, ns1_cap, ss0);
   const size_t c1bytes = (ALG == ALG_MASK_NSQECC) ? sim_bytes(ns0_cap, ns1_cap, sizeof(T)) : 0;
   const size_t per_ia = zbytes + c1bytes;
   unsigned char* base = smem + (size_t)sub * per_ia;
@@ -836,8 +806,7 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
   }
 
   // ---- correlation(s)
-  if constexpr (MF) correlate_2d_mfma<L, LPL, C::TPI, Group<C::TPI>>(x, Z, t, ns0, ns1, red);
-  else correlate_2d<L, LPL, T>(x, zb, t, ns0, ns1);
+  correlate_2d<L, LPL, T>(x, zb, t, ns0, ns1);
   if constexpr (ALG == ALG_MASK_NSQECC) {
     for (int i = t; i < ns0 * ns1; i += C::TPI) C1[i] = S[i];
     // second correlation: mask (x) G^2 (mask_nsqecc.jl:155)
@@ -1271,11 +1240,11 @@ size_t lds_bytes_2d(int alg, int ns0, int ns1, int ss0) {
   return (z + c1) * C::IPB + (size_t)C::IPB * kRedBytes + zw;
 }
 
-template <int L, int ALG, typename T, int LPL, bool SH, bool MF = false>
+template <int L, int ALG, typename T, int LPL, bool SH>
 hipError_t launch_2d_k(const Job& j0, const Job* d_jobs, const int64_t* d_prefix, int n_jobs, int64_t n_ia_total,
                        int ns0, int ns1, size_t lds, char* gz, hipStream_t s) {
   using C = Cfg2<L, T, LPL>;
-  const void* fn = (const void*)piv2d_kernel<L, ALG, T, LPL, SH, MF>;
+  const void* fn = (const void*)piv2d_kernel<L, ALG, T, LPL, SH>;
   allow_full_lds(fn);
   const int64_t nb = (n_ia_total + C::IPB - 1) / C::IPB;
   int slots = 0;
@@ -1283,7 +1252,7 @@ hipError_t launch_2d_k(const Job& j0, const Job* d_jobs, const int64_t* d_prefix
     if (!gz) return hipErrorInvalidValue;  // the caller provides gz_scratch_2d bytes
     slots = gz_slots_2d(n_ia_total);
   }
-  hipLaunchKernelGGL((piv2d_kernel<L, ALG, T, LPL, SH, MF>), dim3((unsigned)nb), dim3(C::NT), lds, s, j0, d_jobs,
+  hipLaunchKernelGGL((piv2d_kernel<L, ALG, T, LPL, SH>), dim3((unsigned)nb), dim3(C::NT), lds, s, j0, d_jobs,
                      d_prefix, n_jobs, n_ia_total, ns0, ns1, gz, slots);
   return hipGetLastError();
 }
@@ -1293,16 +1262,6 @@ hipError_t launch_2d_k(const Job& j0, const Job* d_jobs, const int64_t* d_prefix
 template <int L, int ALG, typename T, int LPL>
 hipError_t launch_2d_t(const Job& j0, const Job* d_jobs, const int64_t* d_prefix, int n_jobs,
                        int64_t n_ia_total, int ns0, int ns1, int ss0, char* gz, hipStream_t s) {
-  if (j0.precision_mode == kPrecSplitMfma) {
-    // single unshifted jobs only (the host checks); window up to 64 x 64
-    if constexpr (mf_ok<L, T, LPL>() && ALG != ALG_MASK_NSQECC) {
-      if (d_jobs || j0.shift || ns0 > L || ns1 > L) return hipErrorInvalidValue;
-      return launch_2d_k<L, ALG, T, LPL, false, true>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1,
-                                                      mf_bytes<true, L>() + kRedBytes, gz, s);
-    } else {
-      return hipErrorInvalidValue;
-    }
-  }
   size_t lds = lds_bytes_2d<L, T, LPL>(ALG, ns0, ns1, ss0);
   if (lds > 160 * 1024) return hipErrorInvalidValue;
   if (j0.shift && !d_jobs)

@@ -273,7 +273,6 @@ static int submit(mqpiv_server* sv, const mqpiv_request* req, bool on_device, ui
   if (req->params.corr_alg == MQPIV_MASK_NSQECC && !req->mask) return MQPIV_ERR_NULL;
   if (req->params.compute_sn && !req->SN) return MQPIV_ERR_NULL;
   if (req->skip_mask) return MQPIV_ERR_UNSUPPORTED;
-  if (req->params.precision_mode == MQPIV_SPLIT_MFMA) return MQPIV_ERR_UNSUPPORTED;  // single calls only
   if ((req->in_dtype != MQPIV_F32 && req->in_dtype != MQPIV_F64) ||
       (req->out_dtype != MQPIV_F32 && req->out_dtype != MQPIV_F64))
     return MQPIV_ERR_BAD_PARAMS;

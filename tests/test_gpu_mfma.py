@@ -2,7 +2,7 @@
 Float64 oracle (bar: tests/parity.py), and the calls it refuses.
 
 The form computes the same fully-overlapping correlation as the FFT kernels
-(fftcc.jl:101-126) with the 64- or 96-point DFT as dense products of hi + lo FP16
+(fftcc.jl:101-126) as chained products with DFT matrices on hi + lo FP16
 operands accumulated in FP32, so its error is of the order of an FP32
 transform's."""
 from __future__ import annotations
@@ -78,9 +78,9 @@ def test_mfma_refuses_what_it_does_not_run(what):
         with pytest.raises(RuntimeError, match="matrix-core"):
             PIV(a, a, np.ones_like(a), p, mode="mfma")
     elif what == "search_size":
-        a = np.zeros((128, 128), np.float32)
-        p = setPIVParameters(corr_alg="nsqecc", interSize=64, searchMargin=20, step=32)
-        with pytest.raises(RuntimeError, match="up to 96"):
+        a = np.zeros((256, 256), np.float32)
+        p = setPIVParameters(corr_alg="nsqecc", interSize=64, searchMargin=40, step=32)
+        with pytest.raises(RuntimeError, match="matrix-core"):
             PIV(a, a, p, mode="mfma")
     else:
         a = np.zeros((128, 128), np.float32)
