@@ -22,16 +22,20 @@
 // as its B operand (stages 2, 5) or, transposed, as its A operand (stage 4).
 // Two D tiles of consecutive rows give a 32-deep K fragment whose k order is
 // a fixed permutation of the rows (wv_perm); the DFT fragments are stored in
-// that order, so no tile is ever moved between lanes or through LDS.  Complex
-// products use three real products (3M: Wr Xr, Wi Xi, (Wr + Wi)(Xr + Xi)).
+// that order, so no tile is ever moved between lanes or through LDS.  A complex
+// product takes four real products accumulated straight into its real and
+// imaginary tiles (Re = Wr Xr + (-Wi) Xi, Im = Wi Xr + Wr Xi).
 //
 // Split operands: v = hi + lo with hi = fp16(v), lo = fp16(v - hi); a product is
 // a_hi b_hi + a_hi b_lo + a_lo b_hi in one FP32 accumulator (a_lo b_lo, about
-// 2^-22 relative, is dropped).  Each stage's operands are scaled by a power of
-// two so that their largest magnitude lies in [2^12, 2^13) (a wave maximum per
-// IA), and the DFT fragments by 2^13, which keeps the lo parts of all but
-// negligible values in FP16's normal range.  The powers are undone exactly at
-// the end.
+// 2^-22 relative, is dropped).  The windows and P are scaled by a power of two
+// so that their largest magnitude lies in [2^4, 2^5) (a wave maximum per IA)
+// and the DFT fragments by 4: every product's output (at most L 2^5 4 per
+// real product pair) then lands in FP16's range as it is, with no rescaling
+// between the stages, while the lo parts of all values above 2^-3 of the
+// operands stay in FP16's normal range (smaller ones keep an absolute error
+// of at most 2^-25, below 2^-27 of the operand's maximum).  The powers are
+// undone exactly at the end.
 //
 // Fragment conventions of v_mfma_f32_16x16x32_f16 (lane l, g = l/16, c = l%16):
 // A (16x32): A[c][8g + e]; B (32x16): B[8g + e][c]; D (16x16): D[4g + v][c].
@@ -43,6 +47,17 @@
 #include "mqpiv_kernels2d.hpp"
 
 namespace mqpiv {
+
+// Scheduling barriers between the products' fragment groups (1: on), which
+// keep the compiler from hoisting every fragment load of a stage at once.
+#ifndef MQPIV_WV_SB
+#define MQPIV_WV_SB 1
+#endif
+__device__ __forceinline__ void wv_sb() {
+#if MQPIV_WV_SB
+  __builtin_amdgcn_sched_barrier(0);
+#endif
+}
 
 typedef _Float16 wh8 __attribute__((ext_vector_type(8)));
 typedef float wf4 __attribute__((ext_vector_type(4)));
@@ -63,26 +78,29 @@ struct WvCfg {
   static constexpr int FT = FC / 16, FK = FC / 32;  // IA tiles along j, 32-steps along i
   static_assert(FT % 2 == 0, "IA tiles in pairs");
   static constexpr int nW1 = KS * NT * 2;
-  static constexpr int nW2 = NT * KS * 6;
-  static constexpr int nW4 = KS * NTS * 6;
+  static constexpr int nW2 = NT * KS * 4;
+  static constexpr int nW4 = KS * NTS * 4;
   static constexpr int nW5 = NTS * KS * 2;
   static constexpr int o1 = 0, o2 = nW1, o4 = o2 + nW2, o5 = o4 + nW4, nF = o5 + nW5;
   static constexpr size_t table_bytes = (size_t)nF * 1024;  // a fragment: 64 lanes x 8 halfs
-  static constexpr int WAVES = 8;                           // waves (IAs in flight) per block
-  // fragment indices; p = 0 hi, 1 lo; q = 0 real, 1 imaginary, 2 their sum
+  static constexpr int WAVES = 12;                          // waves (IAs in flight) per block
+  static constexpr int OCC = WAVES / 4;                     // waves per SIMD
+  static constexpr int LOG2L = L <= 32 ? 5 : (L <= 64 ? 6 : 7);  // ceil(log2 L)
+  // fragment indices; p = 0 hi, 1 lo; q = 0 real, 1 imaginary (the negated
+  // imaginary part is formed from it after the load: FP16 sign flips)
   static constexpr int f1(int ks, int tn, int p) { return o1 + (ks * NT + tn) * 2 + p; }
-  static constexpr int f2(int tm, int ks, int q, int p) { return o2 + ((tm * KS + ks) * 3 + q) * 2 + p; }
-  static constexpr int f4(int ks, int tns, int q, int p) { return o4 + ((ks * NTS + tns) * 3 + q) * 2 + p; }
+  static constexpr int f2(int tm, int ks, int q, int p) { return o2 + ((tm * KS + ks) * 2 + q) * 2 + p; }
+  static constexpr int f4(int ks, int tns, int q, int p) { return o4 + ((ks * NTS + tns) * 2 + q) * 2 + p; }
   static constexpr int f5(int tms, int ks, int p) { return o5 + (tms * KS + ks) * 2 + p; }
 };
 
-// The DFT fragments, scaled by 2^13, as hi/lo FP16 pairs in lane order.
+// The DFT fragments, scaled by 4, as hi/lo FP16 pairs in lane order.
 template <int L, int NTS>
 struct alignas(16) WvTables {
   using C = WvCfg<L, NTS>;
   _Float16 f[C::nF][64][8];
   constexpr void put(int fr, int lane, int e, double v) {
-    v *= 8192.0;
+    v *= 4.0;
     const _Float16 hi = (_Float16)v;
     f[fr][lane][e] = hi;
     f[fr + 1][lane][e] = (_Float16)(v - (double)hi);
@@ -109,7 +127,6 @@ struct alignas(16) WvTables {
             const int a = (q * j) % L;
             put(C::f2(tm, ks, 0, 0), lane, e, tw.c[a]);
             put(C::f2(tm, ks, 1, 0), lane, e, -tw.s[a]);
-            put(C::f2(tm, ks, 2, 0), lane, e, tw.c[a] - tw.s[a]);
           }
         // stage 4 B: k slot q = 32 ks + perm(g, e), column s1 = 16 tns + c; conj W = exp(+2 pi i q s1 / L)
         for (int ks = 0; ks < C::KS; ++ks)
@@ -118,7 +135,6 @@ struct alignas(16) WvTables {
             const int a = (q * s1) % L;
             put(C::f4(ks, tns, 0, 0), lane, e, tw.c[a]);
             put(C::f4(ks, tns, 1, 0), lane, e, tw.s[a]);
-            put(C::f4(ks, tns, 2, 0), lane, e, tw.c[a] + tw.s[a]);
           }
         // stage 5 A: row s0 = 16 tms + c; k slot = (virtual tile 2 ks + e/4, row 4g + e%4):
         // virtual tiles [Re Y tiles 0..NH-1, Im Y tiles 0..NH-1]
@@ -160,17 +176,26 @@ __device__ __forceinline__ float wv_wave_max(float v) {
   Group<64>::steps([&](auto mv) { v = fmaxf(v, mv(0.0f, v)); });
   return bcast63(v);
 }
-// power of two k with m 2^k in [2^12, 2^13) (0 for m = 0), clamped to stay finite
+// power of two k with m 2^k in [2^4, 2^5) (0 for m = 0), clamped to stay finite
 __device__ __forceinline__ int wv_pow2(float m) {
   if (!(m > 0.0f)) return 0;
   int e = 0;
   (void)frexpf(m, &e);
-  const int k = 13 - e;
+  const int k = 5 - e;
   return k < -100 ? -100 : (k > 100 ? 100 : k);
 }
 // log out of line: its constants would otherwise be hoisted out of the IA loop
 __device__ __noinline__ double wv_log(double x) { return log(x); }
 __device__ __forceinline__ wh8 wv_frag(const uint4* T, int f, int lane) { return __builtin_bit_cast(wh8, T[f * 64 + lane]); }
+// -x for a fragment of FP16 values (sign bits flipped)
+__device__ __forceinline__ wh8 wv_neg(wh8 x) {
+  uint4 u = __builtin_bit_cast(uint4, x);
+  u.x ^= 0x80008000u;
+  u.y ^= 0x80008000u;
+  u.z ^= 0x80008000u;
+  u.w ^= 0x80008000u;
+  return __builtin_bit_cast(wh8, u);
+}
 
 // Per-wave LDS window bytes for ns0 x ns1 similarity values.
 __host__ __device__ inline size_t wv_win_bytes(int ns0, int ns1) { return ((size_t)ns0 * ns1 * 4 + 15) & ~(size_t)15; }
@@ -179,7 +204,7 @@ __host__ __device__ inline size_t wv_win_bytes(int ns0, int ns1) { return ((size
 // b runs on XCD b % 8) takes a contiguous range of IAs, so overlapping search
 // windows meet in its L2.
 template <int L, int NTS, int ALG>
-__global__ __launch_bounds__((64 * WvCfg<L, NTS>::WAVES), 2) void piv2d_wave_kernel(
+__global__ __launch_bounds__((64 * WvCfg<L, NTS>::WAVES), (WvCfg<L, NTS>::OCC)) void piv2d_wave_kernel(
     Job j0, const Job* __restrict__ jobs, const int64_t* __restrict__ prefix, int n_jobs, int64_t n_ia_total,
     int ns0_cap, int ns1_cap) {
   using C = WvCfg<L, NTS>;
@@ -241,9 +266,9 @@ __global__ __launch_bounds__((64 * WvCfg<L, NTS>::WAVES), 2) void piv2d_wave_ker
     // ---- the windows, one after the other: lane (g, c) holds, for j-tile tm
     // and i-step ks, the 8 values x(i = 32 ks + 8g + e, j = 16 tm + c).
     // Positions outside the IA / the clipped search region read an in-window
-    // pixel and are zeroed (piv_ops.jl:99-127).  The IA runs its first two
-    // products before the search window is loaded, so that one window's tiles
-    // are held at a time.
+    // pixel and are zeroed (piv_ops.jl:99-127).  The search window runs its
+    // first two products before the IA is loaded, so that one window's tiles
+    // are held at a time (the larger first: fewer registers at the peak).
     const bool want_max = J.has_threshold != 0;
     const int64_t g_org = g0.sr + d0 * g1.sr, f_org = g0.tlf + d0 * g1.tlf;
     double sumF2 = 0.0, muG = 0.0;
@@ -261,7 +286,7 @@ __global__ __launch_bounds__((64 * WvCfg<L, NTS>::WAVES), 2) void piv2d_wave_ker
       for (int ks = 0; ks < NK; ++ks)
 #pragma unroll
         for (int tn = 0; tn < NT; ++tn) {
-          __builtin_amdgcn_sched_barrier(0);
+          wv_sb();
           const wh8 bh = wv_frag(T, C::f1(ks, tn, 0), lane), bl = wv_frag(T, C::f1(ks, tn, 1), lane);
 #pragma unroll
           for (int tm = 0; tm < MT; ++tm) d1[tm][tn] = wv_mm(xa[tm][ks].h, bh, d1[tm][tn]);
@@ -271,68 +296,179 @@ __global__ __launch_bounds__((64 * WvCfg<L, NTS>::WAVES), 2) void piv2d_wave_ker
           for (int tm = 0; tm < MT; ++tm) d1[tm][tn] = wv_mm(xa[tm][ks].l, bh, d1[tm][tn]);
         }
     };
-    // stage 2: X^ = W X (A = table fragments, B = the stage-1 tiles), 3M complex
-    // products; consume(tm, th, Re, Im) takes each finished tile
+    // stage 2: X^ = W X (A = table fragments, B = the stage-1 tiles), complex
+    // products as four real ones; consume(tm, th, Re, Im) takes each finished tile
     auto stage2 = [&](auto& d1, auto nkc, auto&& consume) {
       constexpr int NK = decltype(nkc)::value;
-      float am = 0.f;
-#pragma unroll
-      for (int a = 0; a < 2 * NK; ++a)
-#pragma unroll
-        for (int bb = 0; bb < NT; ++bb) am = fmaxf(am, wv_amax4(d1[a][bb]));
-      const int k1 = wv_pow2(wv_wave_max(am));
-      const float s1 = ldexpf(1.0f, k1);
-      WvHl xr[NK][NH], xi[NK][NH], xs[NK][NH];
+      WvHl xr[NK][NH], xi[NK][NH];
 #pragma unroll
       for (int ks = 0; ks < NK; ++ks)
 #pragma unroll
         for (int th = 0; th < NH; ++th) {
-          const wf8 r = wv_cat(d1[2 * ks][th], d1[2 * ks + 1][th]) * s1;
-          const wf8 im = wv_cat(d1[2 * ks][th + NH], d1[2 * ks + 1][th + NH]) * s1;
-          xr[ks][th] = wv_split(r);
-          xi[ks][th] = wv_split(im);
-          xs[ks][th] = wv_split(r + im);
+          xr[ks][th] = wv_split(wv_cat(d1[2 * ks][th], d1[2 * ks + 1][th]));
+          xi[ks][th] = wv_split(wv_cat(d1[2 * ks][th + NH], d1[2 * ks + 1][th + NH]));
         }
 #pragma unroll
       for (int tm = 0; tm < NT; ++tm) {
         const wf4 z4 = {0.f, 0.f, 0.f, 0.f};
-        wf4 t1[NH], t2[NH], t3[NH];
+        wf4 re[NH], im[NH];
 #pragma unroll
-        for (int th = 0; th < NH; ++th) t1[th] = t2[th] = t3[th] = z4;
+        for (int th = 0; th < NH; ++th) re[th] = im[th] = z4;
 #pragma unroll
         for (int ks = 0; ks < NK; ++ks) {
-          __builtin_amdgcn_sched_barrier(0);
+          wv_sb();
           const wh8 rh = wv_frag(T, C::f2(tm, ks, 0, 0), lane), rl = wv_frag(T, C::f2(tm, ks, 0, 1), lane);
           const wh8 ih = wv_frag(T, C::f2(tm, ks, 1, 0), lane), il = wv_frag(T, C::f2(tm, ks, 1, 1), lane);
-          const wh8 sh = wv_frag(T, C::f2(tm, ks, 2, 0), lane), sl = wv_frag(T, C::f2(tm, ks, 2, 1), lane);
+          const wh8 nh = wv_neg(ih), nl = wv_neg(il);
 #pragma unroll
           for (int th = 0; th < NH; ++th) {
-            t1[th] = wv_mm(rh, xr[ks][th].h, t1[th]);
-            t2[th] = wv_mm(ih, xi[ks][th].h, t2[th]);
-            t3[th] = wv_mm(sh, xs[ks][th].h, t3[th]);
+            re[th] = wv_mm(rh, xr[ks][th].h, re[th]);
+            im[th] = wv_mm(ih, xr[ks][th].h, im[th]);
           }
 #pragma unroll
           for (int th = 0; th < NH; ++th) {
-            t1[th] = wv_mm(rh, xr[ks][th].l, t1[th]);
-            t2[th] = wv_mm(ih, xi[ks][th].l, t2[th]);
-            t3[th] = wv_mm(sh, xs[ks][th].l, t3[th]);
+            re[th] = wv_mm(nh, xi[ks][th].h, re[th]);
+            im[th] = wv_mm(rh, xi[ks][th].h, im[th]);
           }
 #pragma unroll
           for (int th = 0; th < NH; ++th) {
-            t1[th] = wv_mm(rl, xr[ks][th].h, t1[th]);
-            t2[th] = wv_mm(il, xi[ks][th].h, t2[th]);
-            t3[th] = wv_mm(sl, xs[ks][th].h, t3[th]);
+            re[th] = wv_mm(rh, xr[ks][th].l, re[th]);
+            im[th] = wv_mm(ih, xr[ks][th].l, im[th]);
+          }
+#pragma unroll
+          for (int th = 0; th < NH; ++th) {
+            re[th] = wv_mm(nh, xi[ks][th].l, re[th]);
+            im[th] = wv_mm(rh, xi[ks][th].l, im[th]);
+          }
+#pragma unroll
+          for (int th = 0; th < NH; ++th) {
+            re[th] = wv_mm(rl, xr[ks][th].h, re[th]);
+            im[th] = wv_mm(il, xr[ks][th].h, im[th]);
+          }
+#pragma unroll
+          for (int th = 0; th < NH; ++th) {
+            re[th] = wv_mm(nl, xi[ks][th].h, re[th]);
+            im[th] = wv_mm(rl, xi[ks][th].h, im[th]);
           }
         }
 #pragma unroll
-        for (int th = 0; th < NH; ++th) consume(tm, th, t1[th] - t2[th], t3[th] - t1[th] - t2[th]);
+        for (int th = 0; th < NH; ++th) consume(tm, th, re[th], im[th]);
       }
-      return k1;
     };
 
-    wf4 pr[NT][NH], pi[NT][NH];
-    int kF, k1f;
-    // -- the IA: F (zncc: F - mean, in the images' element type, Q12), sum(F^2),
+    wf4 pr[NT][NH], pi[NT][NH];  // G^, then P
+    int kF;
+    // -- the search window first: G (zncc: G - mean of the clipped search region)
+    int kG;
+    {
+      float gv[NT][KS][8];
+      uint32_t mi[KS] = {};
+      bool mj[NT];
+      {
+        uint32_t ri[KS][8], cj[NT];
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {
+            const int x = 32 * ks + 8 * g + e - g0.goff;
+            mi[ks] |= (x >= 0 && x < g0.gsz ? 1u : 0u) << e;
+            ri[ks][e] = (uint32_t)min(max(x, 0), g0.gsz - 1);
+          }
+#pragma unroll
+        for (int tm = 0; tm < NT; ++tm) {
+          const int x = 16 * tm + c - g1.goff;
+          mj[tm] = x >= 0 && x < g1.gsz;
+          cj[tm] = ud0 * (uint32_t)min(max(x, 0), g1.gsz - 1);
+        }
+        auto fetch = [&](auto tin) {
+          using TIN = decltype(tin);
+          const gptr<char> gb = as_global<char>(J.img2) + g_org * (int64_t)sizeof(TIN);
+#pragma unroll
+          for (int tm = 0; tm < NT; ++tm)
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+              for (int e = 0; e < 8; ++e)
+                gv[tm][ks][e] = (float)*(gptr<TIN>)(gb + (cj[tm] + ri[ks][e]) * (uint32_t)sizeof(TIN));
+        };
+        // FP32 windows whose rows start at the buffer's first row, on 16-byte
+        // boundaries, in whole groups of 4: each lane loads its 8 rows of a
+        // column as two 16-byte vectors (a group past the data reads the
+        // first group; the mask below zeroes it)
+        const bool vec = dt == DT_F32 && g0.goff == 0 && (g0.gsz & 3) == 0 && (g0.sr & 3) == 0 && (d0 & 3) == 0 &&
+                         (((uintptr_t)J.img2) & 15) == 0;
+        if (vec) {
+          const gptr<char> gb = as_global<char>(J.img2) + g_org * (int64_t)sizeof(float);
+#pragma unroll
+          for (int tm = 0; tm < NT; ++tm)
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+              for (int h = 0; h < 2; ++h) {
+                const int r = 32 * ks + 8 * g + 4 * h;
+                const uint32_t rr = r + 4 <= g0.gsz ? (uint32_t)r : 0u;
+                const wf4 v = *(gptr<wf4>)(gb + (cj[tm] + rr) * (uint32_t)sizeof(float));
+#pragma unroll
+                for (int q = 0; q < 4; ++q) gv[tm][ks][4 * h + q] = v[q];
+              }
+        } else if (dt == DT_F32) {
+          fetch(float{});
+        } else {
+          fetch(double{});
+        }
+      }
+      // an unclipped search region that fills the buffer has no position outside it
+      const bool full = g0.goff == 0 && g1.goff == 0 && g0.gsz == L && g1.gsz == L;
+      auto in = [&](int tm, int ks, int e) { return full || (mj[tm] && ((mi[ks] >> e) & 1u)); };
+      float meanG = 0.f;
+      if constexpr (ALG == ALG_ZNCC) {
+        double sg = 0.0;
+#pragma unroll
+        for (int tm = 0; tm < NT; ++tm)
+#pragma unroll
+          for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) sg += in(tm, ks, e) ? (double)gv[tm][ks][e] : 0.0;
+        sg = Group<64>::reduce(sg, OpAdd{}, nullptr, lane);
+        const int64_t nG = (int64_t)g0.gsz * g1.gsz;
+        const float m32 = (float)sg / (float)nG;
+        muG = dt == DT_F32 ? (double)m32 : sg / (double)nG;
+        meanG = dt == DT_F32 ? m32 : (float)muG;
+      }
+      float am = 0.f;
+#pragma unroll
+      for (int tm = 0; tm < NT; ++tm)
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {
+            float v = gv[tm][ks][e];
+            if constexpr (ALG == ALG_ZNCC) v = v - meanG;
+            v = in(tm, ks, e) ? v : 0.f;
+            gv[tm][ks][e] = v;
+            am = fmaxf(am, fabsf(v));
+          }
+      kG = wv_pow2(wv_wave_max(am));
+      const float sG = ldexpf(1.0f, kG);
+      WvHl ga[NT][KS];
+#pragma unroll
+      for (int tm = 0; tm < NT; ++tm)
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+          wf8 v;
+#pragma unroll
+          for (int e = 0; e < 8; ++e) v[e] = gv[tm][ks][e] * sG;
+          ga[tm][ks] = wv_split(v);
+        }
+      wf4 d1g[NT][NT];
+      stage1(ga, d1g, std::integral_constant<int, NT>{}, std::integral_constant<int, KS>{});
+      stage2(d1g, std::integral_constant<int, KS>{}, [&](int tm, int th, wf4 re, wf4 im) {
+        pr[tm][th] = re;
+        pi[tm][th] = im;
+      });
+    }
+
+    // -- then the IA: F (zncc: F - mean, in the images' element type, Q12), sum(F^2),
     // the maximum for the skip test (piv_ops.jl:134-157)
     {
       float fv[FT][FK][8];
@@ -365,10 +501,32 @@ __global__ __launch_bounds__((64 * WvCfg<L, NTS>::WAVES), 2) void piv2d_wave_ker
               for (int e = 0; e < 8; ++e)
                 fv[tm][ks][e] = (float)*(gptr<TIN>)(fb + (cj[tm] + ri[ks][e]) * (uint32_t)sizeof(TIN));
         };
-        if (dt == DT_F32) fetch(float{});
-        else fetch(double{});
+        // as the search window: 16-byte row groups of 4 where they line up
+        const bool vec = dt == DT_F32 && (is0 & 3) == 0 && (g0.tlf & 3) == 0 && (d0 & 3) == 0 &&
+                         (((uintptr_t)J.img1) & 15) == 0;
+        if (vec) {
+          const gptr<char> fb = as_global<char>(J.img1) + f_org * (int64_t)sizeof(float);
+#pragma unroll
+          for (int tm = 0; tm < FT; ++tm)
+#pragma unroll
+            for (int ks = 0; ks < FK; ++ks)
+#pragma unroll
+              for (int h = 0; h < 2; ++h) {
+                const int r = 32 * ks + 8 * g + 4 * h;
+                const uint32_t rr = r + 4 <= is0 ? (uint32_t)r : 0u;
+                const wf4 v = *(gptr<wf4>)(fb + (cj[tm] + rr) * (uint32_t)sizeof(float));
+#pragma unroll
+                for (int q = 0; q < 4; ++q) fv[tm][ks][4 * h + q] = v[q];
+              }
+        } else if (dt == DT_F32) {
+          fetch(float{});
+        } else {
+          fetch(double{});
+        }
       }
-      auto in = [&](int tm, int ks, int e) { return mj[tm] && ((mi[ks] >> e) & 1u); };
+      // an IA of the kernel's full size has no position outside it (uniform)
+      const bool full = is0 == C::FC && is1 == C::FC;
+      auto in = [&](int tm, int ks, int e) { return full || (mj[tm] && ((mi[ks] >> e) & 1u)); };
       double sf = 0.0, sff = 0.0, mx1 = -INFINITY;
 #pragma unroll
       for (int tm = 0; tm < FT; ++tm)
@@ -422,98 +580,13 @@ __global__ __launch_bounds__((64 * WvCfg<L, NTS>::WAVES), 2) void piv2d_wave_ker
         }
       wf4 d1f[FT][NT];
       stage1(fa, d1f, std::integral_constant<int, FT>{}, std::integral_constant<int, FK>{});
-      k1f = stage2(d1f, std::integral_constant<int, FT / 2>{}, [&](int tm, int th, wf4 re, wf4 im) {
-        pr[tm][th] = re;
-        pi[tm][th] = im;
-      });
-    }
-    // -- the search window: G (zncc: G - mean of the clipped search region)
-    int kG, k1g;
-    {
-      float gv[NT][KS][8];
-      uint32_t mi[KS] = {};
-      bool mj[NT];
-      {
-        uint32_t ri[KS][8], cj[NT];
-#pragma unroll
-        for (int ks = 0; ks < KS; ++ks)
-#pragma unroll
-          for (int e = 0; e < 8; ++e) {
-            const int x = 32 * ks + 8 * g + e - g0.goff;
-            mi[ks] |= (x >= 0 && x < g0.gsz ? 1u : 0u) << e;
-            ri[ks][e] = (uint32_t)min(max(x, 0), g0.gsz - 1);
-          }
-#pragma unroll
-        for (int tm = 0; tm < NT; ++tm) {
-          const int x = 16 * tm + c - g1.goff;
-          mj[tm] = x >= 0 && x < g1.gsz;
-          cj[tm] = ud0 * (uint32_t)min(max(x, 0), g1.gsz - 1);
-        }
-        auto fetch = [&](auto tin) {
-          using TIN = decltype(tin);
-          const gptr<char> gb = as_global<char>(J.img2) + g_org * (int64_t)sizeof(TIN);
-#pragma unroll
-          for (int tm = 0; tm < NT; ++tm)
-#pragma unroll
-            for (int ks = 0; ks < KS; ++ks)
-#pragma unroll
-              for (int e = 0; e < 8; ++e)
-                gv[tm][ks][e] = (float)*(gptr<TIN>)(gb + (cj[tm] + ri[ks][e]) * (uint32_t)sizeof(TIN));
-        };
-        if (dt == DT_F32) fetch(float{});
-        else fetch(double{});
-      }
-      auto in = [&](int tm, int ks, int e) { return mj[tm] && ((mi[ks] >> e) & 1u); };
-      float meanG = 0.f;
-      if constexpr (ALG == ALG_ZNCC) {
-        double sg = 0.0;
-#pragma unroll
-        for (int tm = 0; tm < NT; ++tm)
-#pragma unroll
-          for (int ks = 0; ks < KS; ++ks)
-#pragma unroll
-            for (int e = 0; e < 8; ++e) sg += in(tm, ks, e) ? (double)gv[tm][ks][e] : 0.0;
-        sg = Group<64>::reduce(sg, OpAdd{}, nullptr, lane);
-        const int64_t nG = (int64_t)g0.gsz * g1.gsz;
-        const float m32 = (float)sg / (float)nG;
-        muG = dt == DT_F32 ? (double)m32 : sg / (double)nG;
-        meanG = dt == DT_F32 ? m32 : (float)muG;
-      }
-      float am = 0.f;
-#pragma unroll
-      for (int tm = 0; tm < NT; ++tm)
-#pragma unroll
-        for (int ks = 0; ks < KS; ++ks)
-#pragma unroll
-          for (int e = 0; e < 8; ++e) {
-            float v = gv[tm][ks][e];
-            if constexpr (ALG == ALG_ZNCC) v = v - meanG;
-            v = in(tm, ks, e) ? v : 0.f;
-            gv[tm][ks][e] = v;
-            am = fmaxf(am, fabsf(v));
-          }
-      kG = wv_pow2(wv_wave_max(am));
-      const float sG = ldexpf(1.0f, kG);
-      WvHl ga[NT][KS];
-#pragma unroll
-      for (int tm = 0; tm < NT; ++tm)
-#pragma unroll
-        for (int ks = 0; ks < KS; ++ks) {
-          wf8 v;
-#pragma unroll
-          for (int e = 0; e < 8; ++e) v[e] = gv[tm][ks][e] * sG;
-          ga[tm][ks] = wv_split(v);
-        }
-      wf4 d1g[NT][NT];
-      stage1(ga, d1g, std::integral_constant<int, NT>{}, std::integral_constant<int, KS>{});
-      // P = conj(F^) G^ as each G^ tile completes
-      k1g = stage2(d1g, std::integral_constant<int, KS>{}, [&](int tm, int th, wf4 gr, wf4 gi) {
-        const wf4 fr = pr[tm][th], fi = pi[tm][th];
+      // P = conj(F^) G^ as each F^ tile completes
+      stage2(d1f, std::integral_constant<int, FT / 2>{}, [&](int tm, int th, wf4 fr, wf4 fi) {
+        const wf4 gr = pr[tm][th], gi = pi[tm][th];
         pr[tm][th] = fr * gr + fi * gi;
         pi[tm][th] = fr * gi - fi * gr;
       });
     }
-
     // ---- stage 4: Y = P^T conj W (A = P, transposed by the tile layout; B = table)
     float amP = 0.f;
 #pragma unroll
@@ -524,64 +597,67 @@ __global__ __launch_bounds__((64 * WvCfg<L, NTS>::WAVES), 2) void piv2d_wave_ker
     const float sP = ldexpf(1.0f, kP);
     wf4 yr[NH][NTS], yi[NH][NTS];
     {
-      WvHl ar[NH][KS], ai[NH][KS], as[NH][KS];
+      WvHl ar[NH][KS], ai[NH][KS];
 #pragma unroll
       for (int th = 0; th < NH; ++th)
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) {
-          const wf8 r = wv_cat(pr[2 * ks][th], pr[2 * ks + 1][th]) * sP;
-          const wf8 im = wv_cat(pi[2 * ks][th], pi[2 * ks + 1][th]) * sP;
-          ar[th][ks] = wv_split(r);
-          ai[th][ks] = wv_split(im);
-          as[th][ks] = wv_split(r + im);
+          ar[th][ks] = wv_split(wv_cat(pr[2 * ks][th], pr[2 * ks + 1][th]) * sP);
+          ai[th][ks] = wv_split(wv_cat(pi[2 * ks][th], pi[2 * ks + 1][th]) * sP);
         }
 #pragma unroll
       for (int tns = 0; tns < NTS; ++tns) {
         const wf4 z4 = {0.f, 0.f, 0.f, 0.f};
-        wf4 t1[NH], t2[NH], t3[NH];
+        wf4 re[NH], im[NH];
 #pragma unroll
-        for (int th = 0; th < NH; ++th) t1[th] = t2[th] = t3[th] = z4;
+        for (int th = 0; th < NH; ++th) re[th] = im[th] = z4;
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) {
-          __builtin_amdgcn_sched_barrier(0);
+          wv_sb();
           const wh8 rh = wv_frag(T, C::f4(ks, tns, 0, 0), lane), rl = wv_frag(T, C::f4(ks, tns, 0, 1), lane);
           const wh8 ih = wv_frag(T, C::f4(ks, tns, 1, 0), lane), il = wv_frag(T, C::f4(ks, tns, 1, 1), lane);
-          const wh8 sh = wv_frag(T, C::f4(ks, tns, 2, 0), lane), sl = wv_frag(T, C::f4(ks, tns, 2, 1), lane);
+          const wh8 nh = wv_neg(ih), nl = wv_neg(il);
+          // Re Y = Pr Cr + Pi (-Ci), Im Y = Pr Ci + Pi Cr
 #pragma unroll
           for (int th = 0; th < NH; ++th) {
-            t1[th] = wv_mm(ar[th][ks].h, rh, t1[th]);
-            t2[th] = wv_mm(ai[th][ks].h, ih, t2[th]);
-            t3[th] = wv_mm(as[th][ks].h, sh, t3[th]);
+            re[th] = wv_mm(ar[th][ks].h, rh, re[th]);
+            im[th] = wv_mm(ar[th][ks].h, ih, im[th]);
           }
 #pragma unroll
           for (int th = 0; th < NH; ++th) {
-            t1[th] = wv_mm(ar[th][ks].h, rl, t1[th]);
-            t2[th] = wv_mm(ai[th][ks].h, il, t2[th]);
-            t3[th] = wv_mm(as[th][ks].h, sl, t3[th]);
+            re[th] = wv_mm(ai[th][ks].h, nh, re[th]);
+            im[th] = wv_mm(ai[th][ks].h, rh, im[th]);
           }
 #pragma unroll
           for (int th = 0; th < NH; ++th) {
-            t1[th] = wv_mm(ar[th][ks].l, rh, t1[th]);
-            t2[th] = wv_mm(ai[th][ks].l, ih, t2[th]);
-            t3[th] = wv_mm(as[th][ks].l, sh, t3[th]);
+            re[th] = wv_mm(ar[th][ks].h, rl, re[th]);
+            im[th] = wv_mm(ar[th][ks].h, il, im[th]);
+          }
+#pragma unroll
+          for (int th = 0; th < NH; ++th) {
+            re[th] = wv_mm(ai[th][ks].h, nl, re[th]);
+            im[th] = wv_mm(ai[th][ks].h, rl, im[th]);
+          }
+#pragma unroll
+          for (int th = 0; th < NH; ++th) {
+            re[th] = wv_mm(ar[th][ks].l, rh, re[th]);
+            im[th] = wv_mm(ar[th][ks].l, ih, im[th]);
+          }
+#pragma unroll
+          for (int th = 0; th < NH; ++th) {
+            re[th] = wv_mm(ai[th][ks].l, nh, re[th]);
+            im[th] = wv_mm(ai[th][ks].l, rh, im[th]);
           }
         }
 #pragma unroll
         for (int th = 0; th < NH; ++th) {
-          yr[th][tns] = t1[th] - t2[th];
-          yi[th][tns] = t3[th] - t1[th] - t2[th];
+          yr[th][tns] = re[th];
+          yi[th][tns] = im[th];
         }
       }
     }
 
     // ---- stage 5: C = W5 Y (A = table, B = Y over the virtual tiles [Re, Im])
-    float amY = 0.f;
-#pragma unroll
-    for (int th = 0; th < NH; ++th)
-#pragma unroll
-      for (int tns = 0; tns < NTS; ++tns) amY = fmaxf(amY, fmaxf(wv_amax4(yr[th][tns]), wv_amax4(yi[th][tns])));
-    const int kY = wv_pow2(wv_wave_max(amY));
-    const float sY = ldexpf(1.0f, kY);
     wf4 cc[NTS][NTS];
     {
       auto vtile = [&](int vt, int tns) -> wf4 { return vt < NH ? yr[vt][tns] : yi[vt - NH][tns]; };
@@ -589,7 +665,7 @@ __global__ __launch_bounds__((64 * WvCfg<L, NTS>::WAVES), 2) void piv2d_wave_ker
 #pragma unroll
       for (int ks = 0; ks < KS; ++ks)
 #pragma unroll
-        for (int tns = 0; tns < NTS; ++tns) yo[ks][tns] = wv_split(wv_cat(vtile(2 * ks, tns), vtile(2 * ks + 1, tns)) * sY);
+        for (int tns = 0; tns < NTS; ++tns) yo[ks][tns] = wv_split(wv_cat(vtile(2 * ks, tns), vtile(2 * ks + 1, tns)));
       const wf4 z4 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int a = 0; a < NTS; ++a)
@@ -599,7 +675,7 @@ __global__ __launch_bounds__((64 * WvCfg<L, NTS>::WAVES), 2) void piv2d_wave_ker
       for (int tms = 0; tms < NTS; ++tms)
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) {
-          __builtin_amdgcn_sched_barrier(0);
+          wv_sb();
           const wh8 ah = wv_frag(T, C::f5(tms, ks, 0), lane), al = wv_frag(T, C::f5(tms, ks, 1), lane);
 #pragma unroll
           for (int tns = 0; tns < NTS; ++tns) cc[tms][tns] = wv_mm(ah, yo[ks][tns].h, cc[tms][tns]);
@@ -609,9 +685,9 @@ __global__ __launch_bounds__((64 * WvCfg<L, NTS>::WAVES), 2) void piv2d_wave_ker
           for (int tns = 0; tns < NTS; ++tns) cc[tms][tns] = wv_mm(al, yo[ks][tns].h, cc[tms][tns]);
         }
     }
-    // C = cc 2 / L^2 / 2^(kF + kG + k1f + k1g + kP + kY + 78) (the 2^13 of each of
-    // the six table products in the chain, and the five operand scales)
-    const double cscale = ldexp(2.0 / ((double)L * (double)L), -(kF + kG + k1f + k1g + kP + kY + 78));
+    // C = cc 2 / L^2 / (2^(kF + kG + kP) 4^6): the operand scales, and the 4 of
+    // each of the six table products in the chain
+    const double cscale = ldexp(2.0 / ((double)L * (double)L), -(kF + kG + kP + 12));
 #pragma unroll
     for (int tms = 0; tms < NTS; ++tms)
 #pragma unroll
@@ -652,7 +728,7 @@ __global__ __launch_bounds__((64 * WvCfg<L, NTS>::WAVES), 2) void piv2d_wave_ker
     };
     if constexpr (ALG == ALG_ZNCC || ALG == ALG_NSQECC) {
       const int64_t D0 = d0 + 1;
-      const double sqF = sqrt(sumF2);
+      const float sqFf = (float)sqrt(sumF2);
       const double mu = muG;
       const int64_t corner0 = (g0.sr + D0 * g1.sr) * (int64_t)sizeof(double);
       const gptr<char> sat2 = as_global<char>(J.sat2) + corner0;
@@ -705,16 +781,18 @@ __global__ __launch_bounds__((64 * WvCfg<L, NTS>::WAVES), 2) void piv2d_wave_ker
           }
           W = W > 0.0 ? W : 0.0;
           const double cv = (double)win[e] * cscale;
-          const double den = sqrt(W) * sqF;
-          double r;
+          // the cancelling sums in FP64; the ratio, stored in FP32, from FP32
+          // operands (hardware square root and reciprocal, within 2 ulp)
+          const float den = __builtin_amdgcn_sqrtf((float)W) * sqFf;
+          float r;
           if constexpr (ALG == ALG_ZNCC) {
-            r = cv / den;  // zncc_ops.jl:39-48
+            r = (float)cv * __builtin_amdgcn_rcpf(den);  // zncc_ops.jl:39-48
           } else {
-            const double num = (sumF2 - 2.0 * cv) + W;  // nsqecc_ops.jl:58-60
-            r = den / (num + den);                      // nsqecc_ops.jl:72-73
+            const double num = (sumF2 - 2.0 * cv) + W;           // nsqecc_ops.jl:58-60
+            r = den * __builtin_amdgcn_rcpf((float)num + den);  // nsqecc_ops.jl:72-73
           }
-          win[e] = (float)r;
-          visit(e, pw0[jj], pw1[jj], (float)r);
+          win[e] = r;
+          visit(e, pw0[jj], pw1[jj], r);
         }
       }
     } else {
