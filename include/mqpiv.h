@@ -64,11 +64,13 @@ extern "C" {
 #define MQPIV_PLAIN_FP32 0 /* FP32 transforms; FP64 window sums, sum(F^2) and denominators    */
 #define MQPIV_SPLIT_MFMA 1 /* 2-D DFTs as matrix-core products with hi + lo FP16 operands,
                               FP32 accumulation; FP64 sums and denominators as in mode 0.
-                              Single 2-D calls with search sizes up to 96; FFTCC, ZNCC
-                              and NSQECC; no multipass                                        */
-#define MQPIV_FP64 2       /* every step in FP64 (2-D search sizes up to 96)                     */
-#define MQPIV_AUTO 3       /* FP64 where FP32 transforms miss the parity bar (masked NSQECC, and
-                              ZNCC when SN is requested) and the size allows it; FP32 otherwise */
+                              FFTCC, ZNCC and NSQECC with interSize <= 32, search size <= 64
+                              and margin <= 23 (single calls, batches, graphs, the server);
+                              no multipass                                                    */
+#define MQPIV_FP64 2       /* every step in FP64 (2-D search sizes up to 128)                    */
+#define MQPIV_AUTO 3       /* FP64 where FP32-level transforms miss the parity bar (masked
+                              NSQECC, and ZNCC when SN is requested); elsewhere the split-
+                              precision form where it takes the job, FP32 transforms otherwise */
 
 /* The fields of PIVParameters (src/piv_parameters.jl:29-46) that the device
  * needs, already resolved: step = interSize - overlap when the reference's

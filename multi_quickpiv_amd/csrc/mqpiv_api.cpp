@@ -341,6 +341,16 @@ static int choose_2d(mqpiv_ctx* c, const Job& j, int* fp64_out, int* L_out) {
   } else if (j.precision_mode == MQPIV_AUTO) {
     bool want = j.alg == ALG_MASK_NSQECC || (j.alg == ALG_ZNCC && j.compute_sn);
     fp64 = want && fits(1);
+    // where FP32-level transforms meet the bar, the split-precision matrix-core
+    // form when it takes the job (it runs faster than the FP32 transforms)
+    if (!fp64 && !j.shift) {
+      const int Lw = wave2d_len(j.alg, j.is[0], j.is[1], ss, ns0, ns1);
+      if (Lw > 0) {
+        *fp64_out = 2;
+        *L_out = Lw;
+        return MQPIV_OK;
+      }
+    }
   }
   if (!fits(fp64))
     return fail(c, MQPIV_ERR_UNSUPPORTED, "search size %d exceeds the largest 2-D transform (128)", ss);
