@@ -64,9 +64,10 @@ extern "C" {
 #define MQPIV_PLAIN_FP32 0 /* FP32 transforms; FP64 window sums, sum(F^2) and denominators    */
 #define MQPIV_SPLIT_MFMA 1 /* 2-D DFTs as matrix-core products with hi + lo FP16 operands,
                               FP32 accumulation; FP64 sums and denominators as in mode 0.
-                              FFTCC, ZNCC and NSQECC with interSize <= 32, search size <= 64
-                              and margin <= 23 (single calls, batches, graphs, the server);
-                              no multipass                                                    */
+                              FFTCC, ZNCC and NSQECC with interSize <= 32 and search size
+                              <= 64, or interSize <= 64 and margin <= 16, windows up to
+                              48 x 48 (single calls, batches, graphs, the server); no
+                              multipass                                                       */
 #define MQPIV_FP64 2       /* every step in FP64 (2-D search sizes up to 128)                    */
 #define MQPIV_AUTO 3       /* FP64 where FP32-level transforms miss the parity bar (masked
                               NSQECC, and ZNCC when SN is requested); elsewhere the split-

@@ -327,9 +327,10 @@ static int choose_2d(mqpiv_ctx* c, const Job& j, int* fp64_out, int* L_out) {
     const int L = wave2d_len(j.alg, j.is[0], j.is[1], ss, ns0, ns1);
     if (L < 0)
       return fail(c, MQPIV_ERR_UNSUPPORTED,
-                  "the split-precision matrix-core form takes FFTCC, ZNCC and NSQECC with interSize <= 32, search "
-                  "size <= 64 and margin <= 23 (got interSize %d x %d, search size %d)",
-                  j.is[0], j.is[1], ss);
+                  "the split-precision matrix-core form takes FFTCC, ZNCC and NSQECC with interSize <= 32 and search "
+                  "size <= 64, or interSize <= 64 and searchMargin <= 16, windows up to 48 x 48 (got interSize %d x "
+                  "%d, searchMargin %d x %d)",
+                  j.is[0], j.is[1], j.sm[0], j.sm[1]);
     *fp64_out = 2;  // the matrix-core kernel (launch_piv2d)
     *L_out = L;
     return MQPIV_OK;

@@ -83,8 +83,7 @@ struct WvCfg {
   static constexpr int nW5 = NTS * KS * 2;
   static constexpr int o1 = 0, o2 = nW1, o4 = o2 + nW2, o5 = o4 + nW4, nF = o5 + nW5;
   static constexpr size_t table_bytes = (size_t)nF * 1024;  // a fragment: 64 lanes x 8 halfs
-  static constexpr int WAVES = 12;                          // waves (IAs in flight) per block
-  static constexpr int OCC = WAVES / 4;                     // waves per SIMD
+
   static constexpr int LOG2L = L <= 32 ? 5 : (L <= 64 ? 6 : 7);  // ceil(log2 L)
   // fragment indices; p = 0 hi, 1 lo; q = 0 real, 1 imaginary (the negated
   // imaginary part is formed from it after the load: FP16 sign flips)
@@ -200,21 +199,31 @@ __device__ __forceinline__ wh8 wv_neg(wh8 x) {
 // Per-wave LDS window bytes for ns0 x ns1 similarity values.
 __host__ __device__ inline size_t wv_win_bytes(int ns0, int ns1) { return ((size_t)ns0 * ns1 * 4 + 15) & ~(size_t)15; }
 
+// Waves per block (IAs in flight per CU) for IAs of NQ x NQ sub-blocks: one
+// block keeps three waves per SIMD within 168 VGPRs; more blocks also hold the
+// sum of their products (two waves per SIMD).
+template <int NQ>
+struct WvOcc {
+  static constexpr int WAVES = NQ == 1 ? 12 : 8;
+  static constexpr int OCC = WAVES / 4;
+};
+
 // One wave per IA, WAVES per block, a persistent grid: each XCD (hardware block
 // b runs on XCD b % 8) takes a contiguous range of IAs, so overlapping search
-// windows meet in its L2.
-template <int L, int NTS, int ALG>
-__global__ __launch_bounds__((64 * WvCfg<L, NTS>::WAVES), (WvCfg<L, NTS>::OCC)) void piv2d_wave_kernel(
+// windows meet in its L2.  NQ: at most NQ x NQ sub-blocks of FC x FC per IA.
+template <int L, int NTS, int ALG, int NQ>
+__global__ __launch_bounds__((64 * WvOcc<NQ>::WAVES), (WvOcc<NQ>::OCC)) void piv2d_wave_kernel(
     Job j0, const Job* __restrict__ jobs, const int64_t* __restrict__ prefix, int n_jobs, int64_t n_ia_total,
     int ns0_cap, int ns1_cap) {
   using C = WvCfg<L, NTS>;
   constexpr int NT = C::NT, KS = C::KS, NH = C::NH, FT = C::FT, FK = C::FK;
+  constexpr int WAVES = WvOcc<NQ>::WAVES;
   extern __shared__ __align__(16) unsigned char smem[];
   const uint4* T = reinterpret_cast<const uint4*>(smem);
   {
     const uint4* gtab = wv_gtab<L, NTS>();
     uint4* Tw = reinterpret_cast<uint4*>(smem);
-    for (int i = (int)threadIdx.x; i < C::nF * 64; i += 64 * C::WAVES) Tw[i] = gtab[i];
+    for (int i = (int)threadIdx.x; i < C::nF * 64; i += 64 * WAVES) Tw[i] = gtab[i];
   }
   __syncthreads();
   // the wave index, made wave-uniform for the compiler: every per-IA value below
@@ -227,13 +236,13 @@ __global__ __launch_bounds__((64 * WvCfg<L, NTS>::WAVES), (WvCfg<L, NTS>::OCC)) 
   int32_t first, stride, end;
   if ((nb & 7) == 0) {
     const int32_t x = b & 7, per = nb >> 3;
-    first = (int32_t)(((int64_t)x * nt) >> 3) + (b >> 3) * C::WAVES + w;
+    first = (int32_t)(((int64_t)x * nt) >> 3) + (b >> 3) * WAVES + w;
     end = (int32_t)(((int64_t)(x + 1) * nt) >> 3);
-    stride = per * C::WAVES;
+    stride = per * WAVES;
   } else {
-    first = b * C::WAVES + w;
+    first = b * WAVES + w;
     end = nt;
-    stride = nb * C::WAVES;
+    stride = nb * WAVES;
   }
 
 #pragma unroll 1
@@ -356,72 +365,167 @@ __global__ __launch_bounds__((64 * WvCfg<L, NTS>::WAVES), (WvCfg<L, NTS>::OCC)) 
       }
     };
 
-    wf4 pr[NT][NH], pi[NT][NH];  // G^, then P
-    int kF;
-    // -- the search window first: G (zncc: G - mean of the clipped search region)
-    int kG;
-    {
-      float gv[NT][KS][8];
-      uint32_t mi[KS] = {};
-      bool mj[NT];
-      {
-        uint32_t ri[KS][8], cj[NT];
+    // Window loads.  Sub-block (oa, ob) of the IA: rows [oa, oa + FC), columns
+    // [ob, ob + FC); its part of the search buffer: rows [oa, oa + L), columns
+    // [ob, ob + L).  The loaders leave the values in fv / gv and the positions
+    // inside the data in (mi, mj); full: every position is inside (uniform).
+    auto load_g = [&](int oa, int ob, float (&gv)[NT][KS][8], uint32_t (&mi)[KS], bool (&mj)[NT]) -> bool {
+      uint32_t ri[KS][8], cj[NT];
 #pragma unroll
-        for (int ks = 0; ks < KS; ++ks)
+      for (int ks = 0; ks < KS; ++ks) {
+        mi[ks] = 0;
 #pragma unroll
-          for (int e = 0; e < 8; ++e) {
-            const int x = 32 * ks + 8 * g + e - g0.goff;
-            mi[ks] |= (x >= 0 && x < g0.gsz ? 1u : 0u) << e;
-            ri[ks][e] = (uint32_t)min(max(x, 0), g0.gsz - 1);
-          }
-#pragma unroll
-        for (int tm = 0; tm < NT; ++tm) {
-          const int x = 16 * tm + c - g1.goff;
-          mj[tm] = x >= 0 && x < g1.gsz;
-          cj[tm] = ud0 * (uint32_t)min(max(x, 0), g1.gsz - 1);
-        }
-        auto fetch = [&](auto tin) {
-          using TIN = decltype(tin);
-          const gptr<char> gb = as_global<char>(J.img2) + g_org * (int64_t)sizeof(TIN);
-#pragma unroll
-          for (int tm = 0; tm < NT; ++tm)
-#pragma unroll
-            for (int ks = 0; ks < KS; ++ks)
-#pragma unroll
-              for (int e = 0; e < 8; ++e)
-                gv[tm][ks][e] = (float)*(gptr<TIN>)(gb + (cj[tm] + ri[ks][e]) * (uint32_t)sizeof(TIN));
-        };
-        // FP32 windows whose rows start at the buffer's first row, on 16-byte
-        // boundaries, in whole groups of 4: each lane loads its 8 rows of a
-        // column as two 16-byte vectors (a group past the data reads the
-        // first group; the mask below zeroes it)
-        const bool vec = dt == DT_F32 && g0.goff == 0 && (g0.gsz & 3) == 0 && (g0.sr & 3) == 0 && (d0 & 3) == 0 &&
-                         (((uintptr_t)J.img2) & 15) == 0;
-        if (vec) {
-          const gptr<char> gb = as_global<char>(J.img2) + g_org * (int64_t)sizeof(float);
-#pragma unroll
-          for (int tm = 0; tm < NT; ++tm)
-#pragma unroll
-            for (int ks = 0; ks < KS; ++ks)
-#pragma unroll
-              for (int h = 0; h < 2; ++h) {
-                const int r = 32 * ks + 8 * g + 4 * h;
-                const uint32_t rr = r + 4 <= g0.gsz ? (uint32_t)r : 0u;
-                const wf4 v = *(gptr<wf4>)(gb + (cj[tm] + rr) * (uint32_t)sizeof(float));
-#pragma unroll
-                for (int q = 0; q < 4; ++q) gv[tm][ks][4 * h + q] = v[q];
-              }
-        } else if (dt == DT_F32) {
-          fetch(float{});
-        } else {
-          fetch(double{});
+        for (int e = 0; e < 8; ++e) {
+          const int x = oa + 32 * ks + 8 * g + e - g0.goff;
+          mi[ks] |= (x >= 0 && x < g0.gsz ? 1u : 0u) << e;
+          ri[ks][e] = (uint32_t)min(max(x, 0), g0.gsz - 1);
         }
       }
-      // an unclipped search region that fills the buffer has no position outside it
-      const bool full = g0.goff == 0 && g1.goff == 0 && g0.gsz == L && g1.gsz == L;
-      auto in = [&](int tm, int ks, int e) { return full || (mj[tm] && ((mi[ks] >> e) & 1u)); };
-      float meanG = 0.f;
+#pragma unroll
+      for (int tm = 0; tm < NT; ++tm) {
+        const int x = ob + 16 * tm + c - g1.goff;
+        mj[tm] = x >= 0 && x < g1.gsz;
+        cj[tm] = ud0 * (uint32_t)min(max(x, 0), g1.gsz - 1);
+      }
+      auto fetch = [&](auto tin) {
+        using TIN = decltype(tin);
+        const gptr<char> gb = as_global<char>(J.img2) + g_org * (int64_t)sizeof(TIN);
+#pragma unroll
+        for (int tm = 0; tm < NT; ++tm)
+#pragma unroll
+          for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+              gv[tm][ks][e] = (float)*(gptr<TIN>)(gb + (cj[tm] + ri[ks][e]) * (uint32_t)sizeof(TIN));
+      };
+      // FP32 windows whose rows start at the buffer's first row, on 16-byte
+      // boundaries, in whole groups of 4: each lane loads its 8 rows of a
+      // column as two 16-byte vectors (a group past the data reads the first
+      // group; the mask zeroes it)
+      const bool vec = dt == DT_F32 && g0.goff == 0 && (g0.gsz & 3) == 0 && (g0.sr & 3) == 0 && (d0 & 3) == 0 &&
+                       (((uintptr_t)J.img2) & 15) == 0;
+      if (vec) {
+        const gptr<char> gb = as_global<char>(J.img2) + g_org * (int64_t)sizeof(float);
+#pragma unroll
+        for (int tm = 0; tm < NT; ++tm)
+#pragma unroll
+          for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+              const int r = oa + 32 * ks + 8 * g + 4 * h;
+              const uint32_t rr = r + 4 <= g0.gsz ? (uint32_t)r : 0u;
+              const wf4 v = *(gptr<wf4>)(gb + (cj[tm] + rr) * (uint32_t)sizeof(float));
+#pragma unroll
+              for (int q = 0; q < 4; ++q) gv[tm][ks][4 * h + q] = v[q];
+            }
+      } else if (dt == DT_F32) {
+        fetch(float{});
+      } else {
+        fetch(double{});
+      }
+      return g0.goff == 0 && g1.goff == 0 && oa + L <= g0.gsz && ob + L <= g1.gsz;
+    };
+    auto load_f = [&](int oa, int ob, float (&fv)[FT][FK][8], uint32_t (&mi)[FK], bool (&mj)[FT]) -> bool {
+      uint32_t ri[FK][8], cj[FT];
+#pragma unroll
+      for (int ks = 0; ks < FK; ++ks) {
+        mi[ks] = 0;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const int i = oa + 32 * ks + 8 * g + e;
+          mi[ks] |= (i < is0 ? 1u : 0u) << e;
+          ri[ks][e] = (uint32_t)min(i, is0 - 1);
+        }
+      }
+#pragma unroll
+      for (int tm = 0; tm < FT; ++tm) {
+        const int j = ob + 16 * tm + c;
+        mj[tm] = j < is1;
+        cj[tm] = ud0 * (uint32_t)min(j, is1 - 1);
+      }
+      auto fetch = [&](auto tin) {
+        using TIN = decltype(tin);
+        const gptr<char> fb = as_global<char>(J.img1) + f_org * (int64_t)sizeof(TIN);
+#pragma unroll
+        for (int tm = 0; tm < FT; ++tm)
+#pragma unroll
+          for (int ks = 0; ks < FK; ++ks)
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+              fv[tm][ks][e] = (float)*(gptr<TIN>)(fb + (cj[tm] + ri[ks][e]) * (uint32_t)sizeof(TIN));
+      };
+      // as the search window: 16-byte row groups of 4 where they line up
+      const bool vec = dt == DT_F32 && (is0 & 3) == 0 && (g0.tlf & 3) == 0 && (d0 & 3) == 0 &&
+                       (((uintptr_t)J.img1) & 15) == 0;
+      if (vec) {
+        const gptr<char> fb = as_global<char>(J.img1) + f_org * (int64_t)sizeof(float);
+#pragma unroll
+        for (int tm = 0; tm < FT; ++tm)
+#pragma unroll
+          for (int ks = 0; ks < FK; ++ks)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+              const int r = oa + 32 * ks + 8 * g + 4 * h;
+              const uint32_t rr = r + 4 <= is0 ? (uint32_t)r : 0u;
+              const wf4 v = *(gptr<wf4>)(fb + (cj[tm] + rr) * (uint32_t)sizeof(float));
+#pragma unroll
+              for (int q = 0; q < 4; ++q) fv[tm][ks][4 * h + q] = v[q];
+            }
+      } else if (dt == DT_F32) {
+        fetch(float{});
+      } else {
+        fetch(double{});
+      }
+      return oa + C::FC <= is0 && ob + C::FC <= is1;
+    };
+    // sum, sum of squares, maximum and minimum of the IA's values in one sub-block
+    auto f_stats = [&](const float (&fv)[FT][FK][8], const uint32_t (&mi)[FK], const bool (&mj)[FT], bool full,
+                       double& sf, double& sff, double& mx, double& mn) {
+#pragma unroll
+      for (int tm = 0; tm < FT; ++tm)
+#pragma unroll
+        for (int ks = 0; ks < FK; ++ks)
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {
+            const bool in = full || (mj[tm] && ((mi[ks] >> e) & 1u));
+            const double f = in ? (double)fv[tm][ks][e] : 0.0;
+            sf += f;
+            sff += f * f;
+            if (in) {
+              mx = fmax(mx, f);
+              mn = fmin(mn, f);
+            }
+          }
+    };
+    // the IA's statistics: ZNCC mean (zncc_ops.jl:10, in the images' element
+    // type, Q12) and sum((F - mean)^2) by expansion, NSQECC sum(F^2), and the
+    // maximum for the skip test (piv_ops.jl:134-157)
+    float meanF = 0.f;
+    auto f_finish = [&](double sf, double sff, double mx1) {
       if constexpr (ALG == ALG_ZNCC) {
+        const int64_t nF = (int64_t)is0 * is1;
+        const float m32 = (float)sf / (float)nF;
+        const double muF = dt == DT_F32 ? (double)m32 : sf / (double)nF;
+        meanF = dt == DT_F32 ? m32 : (float)muF;
+        const double v = (sff - 2.0 * muF * sf) + (double)nF * muF * muF;
+        sumF2 = v > 0.0 ? v : 0.0;
+      } else if constexpr (ALG == ALG_NSQECC) {
+        sumF2 = sff;
+      }
+      if (J.skip) skip = J.skip[ia] != 0;
+      else if (want_max && mx1 < J.threshold) skip = 1;
+    };
+
+    // G (zncc: G - mean of the clipped search region), scaled: stages 1 and 2;
+    // returns its scale
+    float meanG = 0.f;
+    auto g_phase = [&](int oa, int ob, auto&& consume) -> int {
+      float gv[NT][KS][8];
+      uint32_t mi[KS];
+      bool mj[NT];
+      const bool full = load_g(oa, ob, gv, mi, mj);
+      auto in = [&](int tm, int ks, int e) { return full || (mj[tm] && ((mi[ks] >> e) & 1u)); };
+      if constexpr (ALG == ALG_ZNCC && NQ == 1) {
         double sg = 0.0;
 #pragma unroll
         for (int tm = 0; tm < NT; ++tm)
@@ -448,7 +552,7 @@ __global__ __launch_bounds__((64 * WvCfg<L, NTS>::WAVES), (WvCfg<L, NTS>::OCC)) 
             gv[tm][ks][e] = v;
             am = fmaxf(am, fabsf(v));
           }
-      kG = wv_pow2(wv_wave_max(am));
+      const int kG = wv_pow2(wv_wave_max(am));
       const float sG = ldexpf(1.0f, kG);
       WvHl ga[NT][KS];
 #pragma unroll
@@ -462,97 +566,24 @@ __global__ __launch_bounds__((64 * WvCfg<L, NTS>::WAVES), (WvCfg<L, NTS>::OCC)) 
         }
       wf4 d1g[NT][NT];
       stage1(ga, d1g, std::integral_constant<int, NT>{}, std::integral_constant<int, KS>{});
-      stage2(d1g, std::integral_constant<int, KS>{}, [&](int tm, int th, wf4 re, wf4 im) {
-        pr[tm][th] = re;
-        pi[tm][th] = im;
-      });
-    }
-
-    // -- then the IA: F (zncc: F - mean, in the images' element type, Q12), sum(F^2),
-    // the maximum for the skip test (piv_ops.jl:134-157)
-    {
+      stage2(d1g, std::integral_constant<int, KS>{}, consume);
+      return kG;
+    };
+    // F (zncc: F - mean), scaled by 2^kfix (or, kfix = INT32_MIN, by its own
+    // maximum): stages 1 and 2; returns the scale.  With one sub-block the IA's
+    // statistics come from this load.
+    auto f_phase = [&](int oa, int ob, int kfix, auto&& consume) -> int {
       float fv[FT][FK][8];
-      uint32_t mi[FK] = {};
+      uint32_t mi[FK];
       bool mj[FT];
-      {
-        uint32_t ri[FK][8], cj[FT];
-#pragma unroll
-        for (int ks = 0; ks < FK; ++ks)
-#pragma unroll
-          for (int e = 0; e < 8; ++e) {
-            const int i = 32 * ks + 8 * g + e;
-            mi[ks] |= (i < is0 ? 1u : 0u) << e;
-            ri[ks][e] = (uint32_t)min(i, is0 - 1);
-          }
-#pragma unroll
-        for (int tm = 0; tm < FT; ++tm) {
-          const int j = 16 * tm + c;
-          mj[tm] = j < is1;
-          cj[tm] = ud0 * (uint32_t)min(j, is1 - 1);
-        }
-        auto fetch = [&](auto tin) {
-          using TIN = decltype(tin);
-          const gptr<char> fb = as_global<char>(J.img1) + f_org * (int64_t)sizeof(TIN);
-#pragma unroll
-          for (int tm = 0; tm < FT; ++tm)
-#pragma unroll
-            for (int ks = 0; ks < FK; ++ks)
-#pragma unroll
-              for (int e = 0; e < 8; ++e)
-                fv[tm][ks][e] = (float)*(gptr<TIN>)(fb + (cj[tm] + ri[ks][e]) * (uint32_t)sizeof(TIN));
-        };
-        // as the search window: 16-byte row groups of 4 where they line up
-        const bool vec = dt == DT_F32 && (is0 & 3) == 0 && (g0.tlf & 3) == 0 && (d0 & 3) == 0 &&
-                         (((uintptr_t)J.img1) & 15) == 0;
-        if (vec) {
-          const gptr<char> fb = as_global<char>(J.img1) + f_org * (int64_t)sizeof(float);
-#pragma unroll
-          for (int tm = 0; tm < FT; ++tm)
-#pragma unroll
-            for (int ks = 0; ks < FK; ++ks)
-#pragma unroll
-              for (int h = 0; h < 2; ++h) {
-                const int r = 32 * ks + 8 * g + 4 * h;
-                const uint32_t rr = r + 4 <= is0 ? (uint32_t)r : 0u;
-                const wf4 v = *(gptr<wf4>)(fb + (cj[tm] + rr) * (uint32_t)sizeof(float));
-#pragma unroll
-                for (int q = 0; q < 4; ++q) fv[tm][ks][4 * h + q] = v[q];
-              }
-        } else if (dt == DT_F32) {
-          fetch(float{});
-        } else {
-          fetch(double{});
-        }
-      }
-      // an IA of the kernel's full size has no position outside it (uniform)
-      const bool full = is0 == C::FC && is1 == C::FC;
+      const bool full = load_f(oa, ob, fv, mi, mj);
       auto in = [&](int tm, int ks, int e) { return full || (mj[tm] && ((mi[ks] >> e) & 1u)); };
-      double sf = 0.0, sff = 0.0, mx1 = -INFINITY;
-#pragma unroll
-      for (int tm = 0; tm < FT; ++tm)
-#pragma unroll
-        for (int ks = 0; ks < FK; ++ks)
-#pragma unroll
-          for (int e = 0; e < 8; ++e) {
-            const double f = in(tm, ks, e) ? (double)fv[tm][ks][e] : 0.0;
-            sf += f;
-            sff += f * f;
-            if (in(tm, ks, e)) mx1 = fmax(mx1, f);
-          }
-      Group<64>::reduce_n<3>(sf, sff, mx1, OpAdd{}, OpAdd{}, OpMax{}, nullptr, lane);
-      float meanF = 0.f;
-      if constexpr (ALG == ALG_ZNCC) {
-        const int64_t nF = (int64_t)is0 * is1;
-        const float m32 = (float)sf / (float)nF;
-        const double muF = dt == DT_F32 ? (double)m32 : sf / (double)nF;
-        meanF = dt == DT_F32 ? m32 : (float)muF;
-        const double v = (sff - 2.0 * muF * sf) + (double)nF * muF * muF;
-        sumF2 = v > 0.0 ? v : 0.0;
-      } else if constexpr (ALG == ALG_NSQECC) {
-        sumF2 = sff;
+      if constexpr (NQ == 1) {
+        double sf = 0.0, sff = 0.0, mx1 = -INFINITY, mn1 = INFINITY;
+        f_stats(fv, mi, mj, full, sf, sff, mx1, mn1);
+        Group<64>::reduce_n<3>(sf, sff, mx1, OpAdd{}, OpAdd{}, OpMax{}, nullptr, lane);
+        f_finish(sf, sff, mx1);
       }
-      if (J.skip) skip = J.skip[ia] != 0;
-      else if (want_max && mx1 < J.threshold) skip = 1;
       float am = 0.f;
 #pragma unroll
       for (int tm = 0; tm < FT; ++tm)
@@ -566,7 +597,7 @@ __global__ __launch_bounds__((64 * WvCfg<L, NTS>::WAVES), (WvCfg<L, NTS>::OCC)) 
             fv[tm][ks][e] = f;
             am = fmaxf(am, fabsf(f));
           }
-      kF = wv_pow2(wv_wave_max(am));
+      const int kF = kfix != INT32_MIN ? kfix : wv_pow2(wv_wave_max(am));
       const float sF = ldexpf(1.0f, kF);
       WvHl fa[FT][FK];
 #pragma unroll
@@ -580,13 +611,87 @@ __global__ __launch_bounds__((64 * WvCfg<L, NTS>::WAVES), (WvCfg<L, NTS>::OCC)) 
         }
       wf4 d1f[FT][NT];
       stage1(fa, d1f, std::integral_constant<int, FT>{}, std::integral_constant<int, FK>{});
+      stage2(d1f, std::integral_constant<int, FT / 2>{}, consume);
+      return kF;
+    };
+
+    wf4 pr[NT][NH], pi[NT][NH];  // P (one sub-block: G^ first)
+    int kFG;                     // P's scale: 2^kFG conj(F^) G^
+    if constexpr (NQ == 1) {
+      // the search window first (the larger: fewer registers at the peak)
+      const int kG = g_phase(0, 0, [&](int tm, int th, wf4 re, wf4 im) {
+        pr[tm][th] = re;
+        pi[tm][th] = im;
+      });
       // P = conj(F^) G^ as each F^ tile completes
-      stage2(d1f, std::integral_constant<int, FT / 2>{}, [&](int tm, int th, wf4 fr, wf4 fi) {
+      const int kF = f_phase(0, 0, INT32_MIN, [&](int tm, int th, wf4 fr, wf4 fi) {
         const wf4 gr = pr[tm][th], gi = pi[tm][th];
         pr[tm][th] = fr * gr + fi * gi;
         pi[tm][th] = fr * gi - fi * gr;
       });
+      kFG = kF + kG;
+    } else {
+      // sub-blocks of at most FC x FC: C(s) = sum over the blocks of their own
+      // correlation with their part of the search window, so the products are
+      // summed (P) and transformed back once.  The IA's statistics and the ZNCC
+      // means come first: one pass over the IA, and the search region's sum
+      // from the integral array of image 2.
+      const int nq0 = (is0 + C::FC - 1) / C::FC, nq1 = (is1 + C::FC - 1) / C::FC;
+      int kF;
+      {
+        double sf = 0.0, sff = 0.0, mx1 = -INFINITY, mn1 = INFINITY;
+        for (int qb = 0; qb < nq1; ++qb)
+          for (int qa = 0; qa < nq0; ++qa) {
+            float fv[FT][FK][8];
+            uint32_t mi[FK];
+            bool mj[FT];
+            const bool full = load_f(C::FC * qa, C::FC * qb, fv, mi, mj);
+            f_stats(fv, mi, mj, full, sf, sff, mx1, mn1);
+          }
+        Group<64>::reduce_n<3>(sf, sff, mx1, OpAdd{}, OpAdd{}, OpMax{}, nullptr, lane);
+        mn1 = -Group<64>::reduce(-mn1, OpMax{}, nullptr, lane);
+        f_finish(sf, sff, mx1);
+        // one scale for every block of the IA, from its extremes (after the
+        // mean, for ZNCC)
+        const double am = fmax(fabs(mx1 - (double)meanF), fabs(mn1 - (double)meanF));
+        kF = wv_pow2((float)am);
+      }
+      if constexpr (ALG == ALG_ZNCC) {
+        const int64_t D0 = d0 + 1;
+        const double* S1 = J.sat1;
+        const int64_t a0 = g0.sr, b0 = g0.sr + g0.gsz, a1 = g1.sr, b1 = g1.sr + g1.gsz;
+        const double sg = ((S1[b0 + D0 * b1] - S1[a0 + D0 * b1]) - S1[b0 + D0 * a1]) + S1[a0 + D0 * a1];
+        const int64_t nG = (int64_t)g0.gsz * g1.gsz;
+        const float m32 = (float)sg / (float)nG;
+        muG = dt == DT_F32 ? (double)m32 : sg / (double)nG;
+        meanG = dt == DT_F32 ? m32 : (float)muG;
+      }
+      const wf4 z4 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int tm = 0; tm < NT; ++tm)
+#pragma unroll
+        for (int th = 0; th < NH; ++th) pr[tm][th] = pi[tm][th] = z4;
+      int kG0 = 0;
+      for (int qb = 0; qb < nq1; ++qb)
+        for (int qa = 0; qa < nq0; ++qa) {
+          wf4 gq[NT][NH], hq[NT][NH];  // this block's G^
+          const int kG = g_phase(C::FC * qa, C::FC * qb, [&](int tm, int th, wf4 re, wf4 im) {
+            gq[tm][th] = re;
+            hq[tm][th] = im;
+          });
+          // P keeps the first block's scale; later blocks are brought to it (a
+          // power of two, exact)
+          if (qa == 0 && qb == 0) kG0 = kG;
+          const float rs = ldexpf(1.0f, kG0 - kG);
+          f_phase(C::FC * qa, C::FC * qb, kF, [&](int tm, int th, wf4 fr, wf4 fi) {
+            const wf4 gr = gq[tm][th], gi = hq[tm][th];
+            pr[tm][th] += (fr * gr + fi * gi) * rs;
+            pi[tm][th] += (fr * gi - fi * gr) * rs;
+          });
+        }
+      kFG = kF + kG0;
     }
+
     // ---- stage 4: Y = P^T conj W (A = P, transposed by the tile layout; B = table)
     float amP = 0.f;
 #pragma unroll
@@ -685,9 +790,9 @@ __global__ __launch_bounds__((64 * WvCfg<L, NTS>::WAVES), (WvCfg<L, NTS>::OCC)) 
           for (int tns = 0; tns < NTS; ++tns) cc[tms][tns] = wv_mm(al, yo[ks][tns].h, cc[tms][tns]);
         }
     }
-    // C = cc 2 / L^2 / (2^(kF + kG + kP) 4^6): the operand scales, and the 4 of
+    // C = cc 2 / L^2 / (2^(kFG + kP) 4^6): the operand scales, and the 4 of
     // each of the six table products in the chain
-    const double cscale = ldexp(2.0 / ((double)L * (double)L), -(kF + kG + kP + 12));
+    const double cscale = ldexp(2.0 / ((double)L * (double)L), -(kFG + kP + 12));
 #pragma unroll
     for (int tms = 0; tms < NTS; ++tms)
 #pragma unroll
@@ -736,9 +841,9 @@ __global__ __launch_bounds__((64 * WvCfg<L, NTS>::WAVES), (WvCfg<L, NTS>::OCC)) 
       const uint32_t rowb = (uint32_t)D0 * (uint32_t)sizeof(double);
       auto at = [](gptr<char> bp, uint32_t off) { return *(gptr<double>)(bp + off); };
       constexpr int NB = 4;
-      constexpr int NQ = (ALG == ALG_ZNCC) ? 2 : 1;
+      constexpr int NQC = (ALG == ALG_ZNCC) ? 2 : 1;
       for (int e0 = lane; e0 < nE; e0 += NB * 64) {
-        double q[NB][NQ][4];
+        double q[NB][NQC][4];
         double nbox[NB];
         int pw0[NB], pw1[NB];
 #pragma unroll
@@ -763,10 +868,10 @@ __global__ __launch_bounds__((64 * WvCfg<L, NTS>::WAVES), (WvCfg<L, NTS>::OCC)) 
           q[jj][0][2] = at(sat2, c10);
           q[jj][0][3] = at(sat2, c00);
           if constexpr (ALG == ALG_ZNCC) {
-            q[jj][NQ - 1][0] = at(sat1, c11);
-            q[jj][NQ - 1][1] = at(sat1, c01);
-            q[jj][NQ - 1][2] = at(sat1, c10);
-            q[jj][NQ - 1][3] = at(sat1, c00);
+            q[jj][NQC - 1][0] = at(sat1, c11);
+            q[jj][NQC - 1][1] = at(sat1, c01);
+            q[jj][NQC - 1][2] = at(sat1, c10);
+            q[jj][NQC - 1][3] = at(sat1, c00);
           }
           nbox[jj] = (double)((hi0w - lo0w) * (hi1w - lo1w));
         }
@@ -776,7 +881,7 @@ __global__ __launch_bounds__((64 * WvCfg<L, NTS>::WAVES), (WvCfg<L, NTS>::OCC)) 
           if (e >= nE) break;
           double W = ((q[jj][0][0] - q[jj][0][1]) - q[jj][0][2]) + q[jj][0][3];
           if constexpr (ALG == ALG_ZNCC) {
-            const double S1 = ((q[jj][NQ - 1][0] - q[jj][NQ - 1][1]) - q[jj][NQ - 1][2]) + q[jj][NQ - 1][3];
+            const double S1 = ((q[jj][NQC - 1][0] - q[jj][NQC - 1][1]) - q[jj][NQC - 1][2]) + q[jj][NQC - 1][3];
             W = (W - 2.0 * mu * S1) + nbox[jj] * mu * mu;
           }
           W = W > 0.0 ? W : 0.0;

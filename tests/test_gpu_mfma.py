@@ -21,9 +21,9 @@ CASES = [
     ((256, 256), 32, 16, 16),       # the 2048^2 workload's IA 32 geometry (length 64)
     ((203, 187), (24, 20), (10, 6), (9, 11)),
     ((96, 112), 16, 8, 8),          # search size 32, run at length 64
-    ((150, 130), (40, 30), (12, 17), (20, 15)),
-    ((320, 288), 64, 16, 16),       # the IA 64 geometry (length 96)
-    ((233, 251), (48, 56), (20, 12), (19, 23)),
+    ((150, 130), (40, 30), (12, 14), (20, 15)),  # 2 x 1 sub-blocks
+    ((320, 288), 64, 16, 16),       # the 2048^2 workload's IA 64 geometry (2 x 2 sub-blocks)
+    ((233, 251), (48, 56), (16, 12), (19, 23)),
 ]
 
 
@@ -79,9 +79,10 @@ def test_mfma_refuses_what_it_does_not_run(what):
             PIV(a, a, np.ones_like(a), p, mode="mfma")
     elif what == "search_size":
         a = np.zeros((256, 256), np.float32)
-        p = setPIVParameters(corr_alg="nsqecc", interSize=64, searchMargin=40, step=32)
-        with pytest.raises(RuntimeError, match="matrix-core"):
-            PIV(a, a, p, mode="mfma")
+        for isz, sm in ((64, 17), (32, 24), (96, 8)):
+            p = setPIVParameters(corr_alg="nsqecc", interSize=isz, searchMargin=sm, step=32)
+            with pytest.raises(RuntimeError, match="matrix-core"):
+                PIV(a, a, p, mode="mfma")
     else:
         a = np.zeros((128, 128), np.float32)
         p = setPIVParameters(corr_alg="nsqecc", interSize=16, searchMargin=4, step=16, multipass=2)
