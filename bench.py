@@ -83,12 +83,13 @@ def flop_dft_per_ia(cs: int, transforms: int = 3) -> float:
     return transforms * (4 * h * cs * cs + 8 * cs * cs * h) + (transforms / 3) * (6 * Pc + P)
 
 
-def uses_fp64(mode: str, alg: str, compute_sn: bool, ss: int) -> bool:
-    if mode == "fp64":
-        return True
-    if mode == "auto":
-        return (alg == "mask_nsqecc" or (alg == "zncc" and compute_sn)) and ss <= 96
-    return False
+def form_of(mode: str, alg: str, isz: int, compute_sn: bool) -> str:
+    """The transform form the library runs one of the headline runs in
+    ("f32", "f64" or "split-f16"; mqpiv_transform_form)."""
+    from multi_quickpiv_amd import setPIVParameters
+    from multi_quickpiv_amd.piv import transform_form
+    p = setPIVParameters(corr_alg=alg, interSize=isz, searchMargin=SM, step=STEP, computeSN=compute_sn)
+    return transform_form(p, (SIZE, SIZE), mode)[0]
 
 
 class _ResidentRequest:
@@ -454,7 +455,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--mode", default="auto", choices=["auto", "fp32", "fp64"])
+    ap.add_argument("--mode", default="auto", choices=["auto", "fp32", "fp64", "mfma"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calib-copy", action="store_true",
@@ -522,7 +523,7 @@ def main() -> int:
         r = DeviceRequest.from_frame(frame, p, mode=args.mode)
         n = int(np.prod(r.vf_size))
         jobs.append(dict(alg=alg, isz=isz, p=p, vf=r.vf_size, n=n, req=r, VF=r.VF, SN=r.SN,
-                         fp64=uses_fp64(args.mode, alg, True, isz + 2 * SM)))
+                         form=form_of(args.mode, alg, isz, True)))
     reqs = [j["req"] for j in jobs]
 
     def step():
@@ -615,12 +616,12 @@ def main() -> int:
     cs = jd["isz"] + 2 * SM  # reference transform size (good_size_real is the identity here)
     flop = flop_fft_per_ia(cs) * jd["n"]
     achieved = flop / (kern_ms[kd] * 1e-3) / 1e12
-    peak = PEAK_FP64_TFLOPS if jd["fp64"] else PEAK_FP32_TFLOPS
+    peak = PEAK_FP64_TFLOPS if jd["form"] == "f64" else PEAK_FP32_TFLOPS
     traffic = None
     tfile = ROOT / "profiles" / "piv2d_2048_zncc_nsqecc_is32_is64.md"
     if tfile.exists():
         for line in tfile.read_text().splitlines():
-            if line.startswith(f"traffic_bytes[{jd['alg']}{jd['isz']}_{'f64' if jd['fp64'] else 'f32'}]"):
+            if line.startswith(f"traffic_bytes[{jd['alg']}{jd['isz']}_{jd['form']}]"):
                 traffic = float(line.split("=")[1].split()[0])
 
     # parity spot check of the timed outputs against the resident data (the
@@ -631,7 +632,7 @@ def main() -> int:
     breakdown = []
     for k, j in enumerate(jobs):
         breakdown.append({"alg": j["alg"], "interSize": j["isz"], "n_ia": j["n"],
-                          "transforms": "f64" if j["fp64"] else "f32",
+                          "transforms": j["form"],
                           "kernel_ms": round(kern_ms[k], 4),
                           "ia_per_s": round(j["n"] / (kern_ms[k] * 1e-3), 1)})
 
@@ -653,8 +654,10 @@ def main() -> int:
     if not args.no_cpu:
         cpu = cpu_baseline_2d(a.cpu().numpy(), b.cpu().numpy(), jobs, nthreads)
 
-    dtype = "f32+f64" if any(j["fp64"] for j in jobs) and not all(j["fp64"] for j in jobs) else (
-        "f64" if all(j["fp64"] for j in jobs) else "f32")
+    # the arithmetic of the step: the forms its runs use ("split-f16": hi + lo FP16
+    # products accumulated in FP32; every form keeps FP64 window sums and
+    # denominators)
+    dtype = "+".join(sorted({j["form"] for j in jobs}))
     out = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -678,7 +681,7 @@ def main() -> int:
                        "p99": round(float(np.percentile(step_ms, 99)), 4),
                        "what": "one frame pair through all four runs"},
         "roofline": {"bound": "valu", "kernel": f"piv2d_kernel {jd['alg']} interSize {jd['isz']} "
-                                                 f"({'f64' if jd['fp64'] else 'f32'} transforms)",
+                                                 f"({jd['form']} transforms)",
                      "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
                      "frac": round(achieved / peak, 4), "traffic": traffic,
                      "flop_per_ia": round(flop_fft_per_ia(cs)), "ia_per_launch": jd["n"],

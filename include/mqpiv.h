@@ -99,6 +99,16 @@ const char* mqpiv_last_error(const mqpiv_ctx* ctx);
 /* nVF per dim and the IA count for an image of size dims[0..ndims-1]. */
 int mqpiv_vf_size(const mqpiv_params* p, const int64_t* dims, int64_t* vf_out3, int64_t* n_ia);
 
+/* The transform form a call with these parameters and dims runs: *form = 0
+ * (FP32 transforms), 1 (FP64 transforms) or 2 (the split-precision matrix-core
+ * form), *length = the transform length (2-D; for the split form 64, or 65 for
+ * IAs taken in 2 x 2 sub-blocks of length-64 products).  Host-only; no device
+ * work.  Same errors as the call itself would report for an unsupported job.
+ * No counterpart in the reference (its transforms are always FFTW at
+ * good_size_real, src/utils/fft_ops.jl:145-198); it lets a caller see which
+ * form the precision mode chose. */
+int mqpiv_transform_form(const mqpiv_params* p, const int64_t* dims, int32_t* form, int32_t* length);
+
 /* Allocation-free form of mqpiv_piv_device.  No counterpart in the reference: it
  * is for callers that cannot allocate device memory (code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc).  It calls no

@@ -43,6 +43,7 @@ EXPORTS = (
     "mqpiv_ctx_destroy",
     "mqpiv_last_error",
     "mqpiv_vf_size",
+    "mqpiv_transform_form",
     "mqpiv_piv",
     "mqpiv_piv_ex",
     "mqpiv_piv_device",
@@ -157,6 +158,9 @@ def load() -> C.CDLL:
         lib.mqpiv_vf_size.restype = C.c_int
         lib.mqpiv_vf_size.argtypes = [C.POINTER(MqpivParams), C.POINTER(C.c_int64),
                                       C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        lib.mqpiv_transform_form.restype = C.c_int
+        lib.mqpiv_transform_form.argtypes = [C.POINTER(MqpivParams), C.POINTER(C.c_int64),
+                                             C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
         lib.mqpiv_piv.restype = C.c_int
         lib.mqpiv_piv.argtypes = [vp, C.POINTER(MqpivParams), C.POINTER(C.c_int64), C.c_int32,
                                   vp, vp, vp, vp, C.c_int32, vp, vp]

@@ -360,6 +360,40 @@ static int choose_2d(mqpiv_ctx* c, const Job& j, int* fp64_out, int* L_out) {
   return MQPIV_OK;
 }
 
+extern "C" int mqpiv_transform_form(const mqpiv_params* p, const int64_t* dims, int32_t* form, int32_t* length) {
+  if (!form || !length) return MQPIV_ERR_NULL;
+  int rc = check_params(nullptr, p, dims);
+  if (rc) return rc;
+  Job j;
+  memset(&j, 0, sizeof(j));
+  j.ndims = p->ndims;
+  for (int d = 0; d < 3; ++d) {
+    j.dims[d] = d < p->ndims ? dims[d] : 1;
+    j.is[d] = d < p->ndims ? (int32_t)p->inter_size[d] : 1;
+    j.sm[d] = d < p->ndims ? (int32_t)p->search_margin[d] : 0;
+  }
+  j.alg = p->corr_alg;
+  j.compute_sn = p->compute_sn ? 1 : 0;
+  j.precision_mode = p->precision_mode;
+  if (p->ndims == 3) {
+    const int ss = std::max({j.is[0] + 2 * j.sm[0], j.is[1] + 2 * j.sm[1], j.is[2] + 2 * j.sm[2]});
+    int fp64 = 0;
+    if (j.precision_mode == MQPIV_FP64) fp64 = 1;
+    else if (j.precision_mode == MQPIV_AUTO) fp64 = (j.alg == ALG_ZNCC && j.compute_sn) ? 1 : 0;
+    const int L = fft_len_3d(fp64, ss);
+    if (L < 0 || j.alg == ALG_MASK_NSQECC) return MQPIV_ERR_UNSUPPORTED;
+    *form = fp64;
+    *length = L;
+    return MQPIV_OK;
+  }
+  int fp64 = 0, L = 0;
+  rc = choose_2d(nullptr, j, &fp64, &L);
+  if (rc) return rc;
+  *form = fp64;
+  *length = L;
+  return MQPIV_OK;
+}
+
 static int launch_job(mqpiv_ctx* c, Job& j, hipStream_t s, char* ws = nullptr) {
   if (j.n_ia == 0) return MQPIV_OK;
   if (j.ndims == 2) {

@@ -12,6 +12,8 @@ memory of a Julia array, so VF/SN come back in the reference's element order.
 """
 from __future__ import annotations
 
+import copy
+
 import ctypes as C
 from typing import Any, Optional, Tuple
 
@@ -59,6 +61,27 @@ def to_c_params(p: PIVParameters, corr_code: Optional[int] = None, mode: str = "
     cp.threshold = p.threshold
     cp.mask_threshold = p.mask_threshold
     return cp
+
+
+TRANSFORM_FORMS = {0: "f32", 1: "f64", 2: "split-f16"}
+
+
+def transform_form(pivparams: PIVParameters, shape: Tuple[int, ...], mode: str = "auto",
+                   masked: bool = False) -> Tuple[str, int]:
+    """The transform form and length a call with these parameters (and a mask,
+    if ``masked``) runs on an image of ``shape``: ("f32" | "f64" | "split-f16",
+    length).  Host-only (mqpiv_transform_form); raises as the call would for an
+    unsupported job."""
+    p = copy.copy(pivparams)
+    p.ndims = len(shape)
+    cp = to_c_params(p, mask_NSQECC.code if masked else None, mode, multipass_ok=True)
+    dims = (C.c_int64 * 3)(*(list(shape) + [1] * (3 - len(shape))))
+    form, length = C.c_int32(0), C.c_int32(0)
+    lib = _lib.load()
+    rc = lib.mqpiv_transform_form(C.byref(cp), dims, C.byref(form), C.byref(length))
+    if rc:
+        raise _lib.MqpivError(rc, "no transform form for these parameters")
+    return TRANSFORM_FORMS[form.value], length.value
 
 
 def _as_input(a: Any) -> np.ndarray:
