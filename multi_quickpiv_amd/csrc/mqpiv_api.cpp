@@ -254,13 +254,15 @@ static int build_job(mqpiv_ctx* c, const mqpiv_params* p, const int64_t* dims, i
   return MQPIV_OK;
 }
 
-// Integral arrays the normalisation reads: NSQECC needs sum(G^2), ZNCC also
-// sum(G) (3-D ZNCC also the means from sum(F) of image 1).  2-D ZNCC with FP64
-// transforms needs none: it forms its window sums in the kernel.  Integral
-// arrays a job needs: count, bytes of one (256-aligned) and of the prefix-scan
-// temporary.
+// Integral arrays the normalisation reads: 2-D NSQECC needs sum(G^2), 2-D ZNCC
+// also sum(G); 2-D ZNCC with FP64 transforms needs none (it forms its window
+// sums in the kernel).  3-D forms its window sums from the slabs (K1), so 3-D
+// ZNCC needs only the means' sums of image 2 and image 1, and 3-D NSQECC none.
+// Integral arrays a job needs: count, bytes of one (256-aligned) and of the
+// prefix-scan temporary.
 static int sat_count(int alg, int ndims, int fp64 = 0) {
-  if (alg == ALG_ZNCC) return ndims == 3 ? 3 : (fp64 == 1 ? 0 : 2);
+  if (ndims == 3) return alg == ALG_ZNCC ? 2 : 0;
+  if (alg == ALG_ZNCC) return fp64 == 1 ? 0 : 2;
   return alg == ALG_NSQECC ? 1 : 0;
 }
 
@@ -296,14 +298,19 @@ static int prepare_sats(mqpiv_ctx* c, Job& j, hipStream_t s, char* ws = nullptr,
   }
   char* b = ws ? ws : (char*)c->d_sat;
   double* t = (double*)(b + nsat * one);
+  if (j.ndims == 3) {  // ZNCC: sums of image 2 and of image 1 (the means)
+    hipError_t e = build_sat(j.img2, j.in_dtype, 1, j.dims, 3, (double*)b, t, s);
+    if (e == hipSuccess) e = build_sat(j.img1, j.in_dtype, 1, j.dims, 3, (double*)(b + one), t, s);
+    if (e != hipSuccess) return fail(c, MQPIV_ERR_DEVICE, "integral array build failed: %s", hipGetErrorString(e));
+    j.sat1 = (const double*)b;
+    j.sat1f = (const double*)(b + one);
+    return MQPIV_OK;
+  }
   hipError_t e = nsat >= 2 ? build_sat_pair(j.img2, j.in_dtype, j.dims, j.ndims, (double*)b, (double*)(b + one), t, s)
                            : build_sat(j.img2, j.in_dtype, 2, j.dims, j.ndims, (double*)b, t, s);
-  if (e == hipSuccess && nsat >= 3)
-    e = build_sat(j.img1, j.in_dtype, 1, j.dims, j.ndims, (double*)(b + 2 * one), t, s);
   if (e != hipSuccess) return fail(c, MQPIV_ERR_DEVICE, "integral array build failed: %s", hipGetErrorString(e));
   j.sat2 = (const double*)b;
   j.sat1 = nsat >= 2 ? (const double*)(b + one) : nullptr;
-  j.sat1f = nsat >= 3 ? (const double*)(b + 2 * one) : nullptr;
   return MQPIV_OK;
 }
 

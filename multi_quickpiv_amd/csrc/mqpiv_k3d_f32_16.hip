@@ -3,7 +3,7 @@
 #include "mqpiv_kernels3d.hpp"
 
 namespace mqpiv {
-template size_t scratch3_per_ia<16, float, 1>(int, int, int);
+template size_t scratch3_per_ia<16, float, 1>(int, int, int, int);
 template hipError_t launch3d_chunk<16, float, 1>(const Job&, void*, int64_t, int64_t, int64_t, const double*,
                                                    hipStream_t);
 }  // namespace mqpiv

@@ -11,32 +11,32 @@
 namespace mqpiv {
 
 template <int L, typename T, int LPL>
-size_t scratch3_per_ia(int ns0, int ns1, int ns2);
+size_t scratch3_per_ia(int alg, int ns0, int ns1, int ns2);
 template <int L, typename T, int LPL>
 hipError_t launch3d_chunk(const Job& J, void* scratch, int64_t chunk_cap, int64_t ia_base, int64_t n,
                           const double* sat1_img1, hipStream_t s);
-extern template size_t scratch3_per_ia<16, float, 1>(int, int, int);
+extern template size_t scratch3_per_ia<16, float, 1>(int, int, int, int);
 extern template hipError_t launch3d_chunk<16, float, 1>(const Job&, void*, int64_t, int64_t, int64_t,
                                                           const double*, hipStream_t);
-extern template size_t scratch3_per_ia<32, float, 2>(int, int, int);
+extern template size_t scratch3_per_ia<32, float, 2>(int, int, int, int);
 extern template hipError_t launch3d_chunk<32, float, 2>(const Job&, void*, int64_t, int64_t, int64_t,
                                                           const double*, hipStream_t);
-extern template size_t scratch3_per_ia<48, float, 4>(int, int, int);
+extern template size_t scratch3_per_ia<48, float, 4>(int, int, int, int);
 extern template hipError_t launch3d_chunk<48, float, 4>(const Job&, void*, int64_t, int64_t, int64_t,
                                                           const double*, hipStream_t);
-extern template size_t scratch3_per_ia<64, float, 4>(int, int, int);
+extern template size_t scratch3_per_ia<64, float, 4>(int, int, int, int);
 extern template hipError_t launch3d_chunk<64, float, 4>(const Job&, void*, int64_t, int64_t, int64_t,
                                                           const double*, hipStream_t);
-extern template size_t scratch3_per_ia<16, double, 2>(int, int, int);
+extern template size_t scratch3_per_ia<16, double, 2>(int, int, int, int);
 extern template hipError_t launch3d_chunk<16, double, 2>(const Job&, void*, int64_t, int64_t, int64_t,
                                                           const double*, hipStream_t);
-extern template size_t scratch3_per_ia<32, double, 4>(int, int, int);
+extern template size_t scratch3_per_ia<32, double, 4>(int, int, int, int);
 extern template hipError_t launch3d_chunk<32, double, 4>(const Job&, void*, int64_t, int64_t, int64_t,
                                                           const double*, hipStream_t);
-extern template size_t scratch3_per_ia<48, double, 4>(int, int, int);
+extern template size_t scratch3_per_ia<48, double, 4>(int, int, int, int);
 extern template hipError_t launch3d_chunk<48, double, 4>(const Job&, void*, int64_t, int64_t, int64_t,
                                                           const double*, hipStream_t);
-extern template size_t scratch3_per_ia<64, double, 4>(int, int, int);
+extern template size_t scratch3_per_ia<64, double, 4>(int, int, int, int);
 extern template hipError_t launch3d_chunk<64, double, 4>(const Job&, void*, int64_t, int64_t, int64_t,
                                                           const double*, hipStream_t);
 
@@ -49,20 +49,20 @@ int fft_len_3d(int fp64, int ss) {
   return -1;
 }
 
-static size_t per_ia(int fp64, int L, int ns0, int ns1, int ns2) {
+static size_t per_ia(int fp64, int L, int alg, int ns0, int ns1, int ns2) {
   if (fp64) {
     switch (L) {
-      case 16: return scratch3_per_ia<16, double, 2>(ns0, ns1, ns2);
-      case 32: return scratch3_per_ia<32, double, 4>(ns0, ns1, ns2);
-      case 48: return scratch3_per_ia<48, double, 4>(ns0, ns1, ns2);
-      case 64: return scratch3_per_ia<64, double, 4>(ns0, ns1, ns2);
+      case 16: return scratch3_per_ia<16, double, 2>(alg, ns0, ns1, ns2);
+      case 32: return scratch3_per_ia<32, double, 4>(alg, ns0, ns1, ns2);
+      case 48: return scratch3_per_ia<48, double, 4>(alg, ns0, ns1, ns2);
+      case 64: return scratch3_per_ia<64, double, 4>(alg, ns0, ns1, ns2);
     }
   } else {
     switch (L) {
-      case 16: return scratch3_per_ia<16, float, 1>(ns0, ns1, ns2);
-      case 32: return scratch3_per_ia<32, float, 2>(ns0, ns1, ns2);
-      case 48: return scratch3_per_ia<48, float, 4>(ns0, ns1, ns2);
-      case 64: return scratch3_per_ia<64, float, 4>(ns0, ns1, ns2);
+      case 16: return scratch3_per_ia<16, float, 1>(alg, ns0, ns1, ns2);
+      case 32: return scratch3_per_ia<32, float, 2>(alg, ns0, ns1, ns2);
+      case 48: return scratch3_per_ia<48, float, 4>(alg, ns0, ns1, ns2);
+      case 64: return scratch3_per_ia<64, float, 4>(alg, ns0, ns1, ns2);
     }
   }
   return 0;
@@ -76,14 +76,14 @@ int64_t chunk_3d(int fp64, int L, const Job& j) {
   const int ns0 = 2 * j.sm[0] + 1, ns1 = 2 * j.sm[1] + 1, ns2 = 2 * j.sm[2] + 1;
   size_t budget = (size_t)2 << 30;
   if (const char* e = getenv("MQPIV_3D_CHUNK_BYTES")) budget = (size_t)atoll(e);
-  int64_t c = (int64_t)(budget / per_ia(fp64, L, ns0, ns1, ns2));
+  int64_t c = (int64_t)(budget / per_ia(fp64, L, j.alg, ns0, ns1, ns2));
   c = std::max<int64_t>(c, 16);
   return std::min<int64_t>(c, j.n_ia);
 }
 
 size_t scratch_3d_bytes(int fp64, int L, const Job& j) {
   const int ns0 = 2 * j.sm[0] + 1, ns1 = 2 * j.sm[1] + 1, ns2 = 2 * j.sm[2] + 1;
-  return (size_t)chunk_3d(fp64, L, j) * per_ia(fp64, L, ns0, ns1, ns2) + 1024;
+  return (size_t)chunk_3d(fp64, L, j) * per_ia(fp64, L, j.alg, ns0, ns1, ns2) + 1024;
 }
 
 hipError_t launch_piv3d(int fp64, int L, const Job& j, void* scratch, const double* sat1_img1, hipStream_t s) {

@@ -5,15 +5,17 @@
 // stay in the last-level cache:
 //   K1 slab_fwd  one block per (IA, z-slab i2): pack Z = A + iB for the slab,
 //                forward 2-D transform (dims 1, 2) in registers + LDS, store
-//                the slab; per-slab partial sum(F^2) and IA maxima.
+//                the slab; per-slab partial sum(F^2) and IA maxima; for ZNCC
+//                and NSQECC the slab's 2-D window sums of G^2 (every window
+//                position) from a summed-area table of the slab in LDS.
 //   K2 zline     one block per (IA, pair of planes k1 / -k1): transform along
 //                dim 3, P = conj(A^) B^ with the partner bin from the paired
 //                plane, inverse along dim 3 keeping s2 < ns2.
 //   K3 slab_inv  one block per (IA, s2): inverse 2-D transform down to the
 //                (ns0 x ns1) window of real correlation values.
-//   K4 finalize  one block per IA: FP64 window sums from the 3-D integral
-//                arrays, ZNCC/NSQECC normalisation, argmax, 3-D Gaussian
-//                subpixel, SN, VF/SN in the reference layout.
+//   K4 finalize  one block per IA: FP64 window sums along dim 3 from the
+//                slabs' window sums, ZNCC/NSQECC normalisation, argmax, 3-D
+//                Gaussian subpixel, SN, VF/SN in the reference layout.
 //
 // Reference behaviour restated (files relative to the reference repo root):
 //   coordinates/offsets  src/utils/piv_ops.jl:38-85 (incl. SR_TLF_off, :72)
@@ -47,12 +49,16 @@ struct Cfg3 {
 //   W    : L^3 complex T, [i2|k2|s2][k1][k0]
 //   S    : ns0*ns1*ns2 T  (correlation window, column-major)
 //   part : L doubles x 3 (sum F^2, max img1, max mask) per slab
+//   box  : L x ns0*ns1 doubles, [i2][s0 + ns0 s1]: slab i2's sum of G^2 over the
+//          window at (s0, s1) (ZNCC: of G'^2, G' = G - mean) -- ZNCC and NSQECC
 struct Scratch3 {
   void* w;
   void* s;
   double* part;
+  double* box;
   int64_t w_stride;  // elements of cpx<T> per IA
   int64_t s_stride;  // elements of T per IA
+  int64_t box_stride;  // doubles per IA
 };
 
 struct Geo3 {
@@ -174,6 +180,8 @@ __global__ __launch_bounds__((Cfg3<L, T, LPL>::TPL)) void slab_fwd_kernel(Job J,
   const bool want_max = J.has_threshold;
   double sumF2 = 0.0, mx1 = -INFINITY;
   cpx<T> x[M];
+  constexpr bool kBox = ALG == ALG_ZNCC || ALG == ALG_NSQECC;
+  double gq[kBox ? M : 1];  // G^2 (ZNCC: G'^2) of the lane's samples, 0 outside the data
   // Every lane issues all its loads before using any: positions outside the
   // IA / clipped search region read an in-volume voxel instead (the IA origin,
   // or the search region's first column in this lane's row and plane) and
@@ -212,9 +220,14 @@ __global__ __launch_bounds__((Cfg3<L, T, LPL>::TPL)) void slab_fwd_kernel(Job J,
         else
           g = (double)gv[k];
       }
+      if constexpr (kBox) gq[k] = g * g;
       x[k] = cpx<T>{(T)a, (T)g};
     }
   };
+  if constexpr (kBox) {
+#pragma unroll
+    for (int k = 0; k < M; ++k) gq[k] = 0.0;
+  }
   if (line < L) {
     if (dt == DT_F32) load(float{});
     else load(double{});
@@ -243,6 +256,61 @@ __global__ __launch_bounds__((Cfg3<L, T, LPL>::TPL)) void slab_fwd_kernel(Job J,
     cpx<T>* W = (cpx<T>*)sc.w + il * sc.w_stride + (int64_t)i2 * L * L + (int64_t)line * L;
 #pragma unroll
     for (int k = 0; k < M; ++k) W[cM + k] = x[k];
+  }
+  if constexpr (kBox) {
+    // The slab's window sums of G^2 (ZNCC: of G'^2, the rounded differences, as
+    // zncc_ops.jl:22-24 and integralArraySQ!, common_ops.jl:142-180): an
+    // inclusive summed-area table of the slab, P(a, b) = sum over rows <= a and
+    // columns <= b, built in the transform buffer once the transform has left
+    // it (L x L doubles fit in L x (L + 1) complex values of either type), then
+    // for every window position (s0, s1) the box [s0, s0 + is0) x [s1, s1 + is1)
+    // in the order of the 2-D window sums (common_ops.jl:203-245).  Positions
+    // outside the clipped search region hold 0, so these are the sums of the
+    // clipped windows.  Each prefix pass splits a line over PS adjacent lanes:
+    // every lane scans its segment in registers and adds the totals of the
+    // line's earlier segments.
+    static_assert((size_t)L * L * sizeof(double) <= (size_t)L * LD * sizeof(cpx<T>), "table fits the buffer");
+    static_assert(C::TPL % L == 0, "whole lanes per line");
+    constexpr int PS = C::TPL / L, RS = L / PS;
+    static_assert(RS * PS == L && (PS & (PS - 1)) == 0, "equal segments, a power of two of lanes");
+    double* P = reinterpret_cast<double*>(smem);
+    __syncthreads();  // every lane has read the buffer
+    if (line < L) {
+#pragma unroll
+      for (int k = 0; k < M; ++k) P[line + L * (LPL * k + q)] = gq[k];
+    }
+    __syncthreads();
+    const int seg = t % PS, ln = t / PS;
+    auto pass = [&](int line_stride, int elem_stride) {
+      double v[RS];
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < RS; ++k) {
+        acc += P[ln * line_stride + (seg * RS + k) * elem_stride];
+        v[k] = acc;
+      }
+      double inc = acc;
+#pragma unroll
+      for (int d = 1; d < PS; d <<= 1) {
+        const double o = __shfl_up(inc, d, PS);
+        if (seg >= d) inc += o;
+      }
+      const double off = inc - acc;
+#pragma unroll
+      for (int k = 0; k < RS; ++k) P[ln * line_stride + (seg * RS + k) * elem_stride] = v[k] + off;
+      __syncthreads();
+    };
+    pass(L, 1);  // down each column (dim 1)
+    pass(1, L);  // along each row (dim 2)
+    const int ns0 = 2 * J.sm[0] + 1, ns1 = 2 * J.sm[1] + 1;
+    const int is0 = J.is[0], is1 = J.is[1];
+    auto at = [&](int a, int bb) { return (a < 0 || bb < 0) ? 0.0 : P[a + L * bb]; };
+    double* B = sc.box + il * sc.box_stride + (int64_t)i2 * ns0 * ns1;
+    for (int e = t; e < ns0 * ns1; e += C::TPL) {
+      const int s0 = e % ns0, s1 = e / ns0;
+      B[e] = ((at(s0 + is0 - 1, s1 + is1 - 1) - at(s0 - 1, s1 + is1 - 1)) - at(s0 + is0 - 1, s1 - 1)) +
+             at(s0 - 1, s1 - 1);
+    }
   }
 }
 
@@ -396,80 +464,33 @@ __global__ __launch_bounds__(256) void finalize3d_kernel(Job J, Scratch3 sc, int
   const bool skip = J.skip ? (J.skip[ia] != 0) : (J.has_threshold && mx1 < J.threshold);
 
   if constexpr (ALG == ALG_ZNCC || ALG == ALG_NSQECC) {
-    const int64_t D0 = J.dims[0] + 1, D01 = D0 * (J.dims[1] + 1);
-    double mu = 0.0;
-    if constexpr (ALG == ALG_ZNCC) {
-      Means3 m = means3(J, G, sat1_img1);
-      mu = (J.in_dtype == DT_F32) ? (double)m.g32 : m.g64;
-    }
+    // W(s) = sum over the window's planes s2 .. s2 + is2 - 1 of the slabs' window
+    // sums at (s0, s1), as the difference of two running sums along dim 3 (the
+    // prefix-sum form of the integral array's dim 3); one thread per (s0, s1)
     const double sqF = sqrt(sumF2);
-    // NB window positions per lane per round, all 8 (16 for ZNCC) corner loads
-    // issued before any use; corner indices are clamped into the search region
-    // and an empty window's sum is replaced by 0 afterwards.
-    constexpr int NB = 2;
-    const gptr<double> sat2 = as_global<double>(J.sat2);
-    const gptr<double> sat1 = as_global<double>(J.sat1);
-    BoxWalk wk(t, NT, ns0, ns1);
-    for (int e0 = t; e0 < nS; e0 += NB * NT) {
-      double q2[NB][8], q1[NB][8], nbox[NB];
-      bool empty[NB];
-#pragma unroll
-      for (int j = 0; j < NB; ++j) {
-        const int e = e0 + j * NT;
-        const bool in = e < nS;
-        const int s[3] = {in ? wk.a : 0, in ? wk.b : 0, in ? wk.c : 0};
-        wk.step();
-        const int isz[3] = {is0, is1, is2};
-        int64_t a[3], bb[3];
-        empty[j] = false;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          const Geo& g = G.g[d];
-          const int lo = s[d] > g.goff ? s[d] : g.goff;
-          int hi = (s[d] + isz[d]) < (g.goff + g.gsz) ? (s[d] + isz[d]) : (g.goff + g.gsz);
-          if (hi <= lo) {
-            empty[j] = true;
-            hi = lo;
-          }
-          a[d] = g.sr + (lo - g.goff);
-          bb[d] = g.sr + (hi - g.goff);
+    const int nP = ns0 * ns1;
+    const double* B = sc.box + il * sc.box_stride;
+    for (int e = t; e < nP; e += NT) {
+      double hi = 0.0, lo = 0.0;  // sums over planes < s2 + is2 and < s2
+      for (int k = 0; k < is2; ++k) hi += B[(int64_t)k * nP + e];
+      for (int s2 = 0; s2 < ns2; ++s2) {
+        if (s2 > 0) {
+          hi += B[(int64_t)(s2 + is2 - 1) * nP + e];
+          lo += B[(int64_t)(s2 - 1) * nP + e];
         }
-        // corner order of sat_box3: (b,b,b) (a,b,b) (b,a,b) (b,b,a) (a,a,b) (a,b,a) (b,a,a) (a,a,a)
-        const int64_t idx[8] = {bb[0] + D0 * bb[1] + D01 * bb[2], a[0] + D0 * bb[1] + D01 * bb[2],
-                                bb[0] + D0 * a[1] + D01 * bb[2],  bb[0] + D0 * bb[1] + D01 * a[2],
-                                a[0] + D0 * a[1] + D01 * bb[2],   a[0] + D0 * bb[1] + D01 * a[2],
-                                bb[0] + D0 * a[1] + D01 * a[2],   a[0] + D0 * a[1] + D01 * a[2]};
-#pragma unroll
-        for (int c = 0; c < 8; ++c) q2[j][c] = sat2[idx[c]];
-        if constexpr (ALG == ALG_ZNCC) {
-#pragma unroll
-          for (int c = 0; c < 8; ++c) q1[j][c] = sat1[idx[c]];
-        }
-        nbox[j] = (double)((bb[0] - a[0]) * (bb[1] - a[1]) * (bb[2] - a[2]));
-      }
-#pragma unroll
-      for (int j = 0; j < NB; ++j) {
-        const int e = e0 + j * NT;
-        if (e >= nS) break;
-        auto box = [&](const double (&v)[8]) {
-          return (((v[0] - v[1]) - v[2]) - v[3]) + ((v[4] + v[5]) + v[6]) - v[7];
-        };
-        double W = 0.0;
-        if (!empty[j]) {
-          W = box(q2[j]);
-          if constexpr (ALG == ALG_ZNCC) W = (W - 2.0 * mu * box(q1[j])) + nbox[j] * mu * mu;
-          W = W > 0.0 ? W : 0.0;
-        }
-        const double c = (double)S[e];
+        double W = hi - lo;
+        W = W > 0.0 ? W : 0.0;
+        const int64_t li = e + (int64_t)nP * s2;
+        const double c = (double)S[li];
         const double den = sqrt(W) * sqF;
         double r;
         if constexpr (ALG == ALG_ZNCC) {
-          r = c / den;
+          r = c / den;  // zncc_ops.jl:39-48
         } else {
-          const double num = (sumF2 - 2.0 * c) + W;
-          r = den / (num + den);
+          const double num = (sumF2 - 2.0 * c) + W;  // nsqecc_ops.jl:58-60
+          r = den / (num + den);                     // nsqecc_ops.jl:72-73
         }
-        S[e] = (T)r;
+        S[li] = (T)r;
       }
     }
     __syncthreads();
@@ -580,11 +601,12 @@ __global__ __launch_bounds__(256) void finalize3d_kernel(Job J, Scratch3 sc, int
 // ---------------------------------------------------------------------------
 // Host-side launch of one chunk of IAs [ia_base, ia_base + n).
 template <int L, typename T, int LPL>
-size_t scratch3_per_ia(int ns0, int ns1, int ns2) {
+size_t scratch3_per_ia(int alg, int ns0, int ns1, int ns2) {
   size_t w = (size_t)L * L * L * sizeof(cpx<T>);
   size_t s = ((size_t)ns0 * ns1 * ns2 * sizeof(T) + 255) & ~(size_t)255;
   size_t p = (size_t)3 * L * sizeof(double);
-  return w + s + p;
+  size_t bx = (alg == ALG_ZNCC || alg == ALG_NSQECC) ? (size_t)L * ns0 * ns1 * sizeof(double) : 0;
+  return w + s + p + bx;
 }
 
 template <int L, typename T, int LPL>
@@ -601,6 +623,10 @@ hipError_t launch3d_chunk(const Job& J, void* scratch, int64_t chunk_cap, int64_
   sc.s = b;
   b += (((size_t)chunk_cap * ns0 * ns1 * ns2 * sizeof(T)) + 255) & ~(size_t)255;
   sc.part = (double*)b;
+  b += (size_t)chunk_cap * 3 * L * sizeof(double);
+  const bool box = J.alg == ALG_ZNCC || J.alg == ALG_NSQECC;
+  sc.box = box ? (double*)b : nullptr;
+  sc.box_stride = (int64_t)L * ns0 * ns1;
   const size_t lds1 = (size_t)L * C::LD * sizeof(cpx<T>) + 16 * sizeof(double);
   const size_t lds2 = (size_t)2 * L * C::LD * sizeof(cpx<T>);
   const size_t lds3 = (size_t)L * C::LD * sizeof(cpx<T>);
