@@ -485,6 +485,11 @@ def main() -> int:
 
     if args.only_3d:
         torch.cuda.set_device(0)
+        if args.calib_copy:
+            src = torch.ones(64 << 20, dtype=torch.float32, device="cuda:0")  # 256 MiB
+            torch.empty_like(src).copy_(src)
+            torch.cuda.synchronize()
+            del src
         print(json.dumps({"volume_3d": volume_bench(args.mode, args.steps, False, 1)}))
         return 0
 
