@@ -85,7 +85,7 @@ def flop_dft_per_ia(cs: int, transforms: int = 3) -> float:
 
 def form_of(mode: str, alg: str, isz: int, compute_sn: bool) -> str:
     """The transform form the library runs one of the headline runs in
-    ("f32", "f64" or "split-f16"; mqpiv_transform_form)."""
+    ("f32", "f64", "split-f16" or "split-f16+f64"; mqpiv_transform_form)."""
     from multi_quickpiv_amd import setPIVParameters
     from multi_quickpiv_amd.piv import transform_form
     p = setPIVParameters(corr_alg=alg, interSize=isz, searchMargin=SM, step=STEP, computeSN=compute_sn)
@@ -615,13 +615,18 @@ def main() -> int:
         res["mfma_over_fp32"] = round(res["fp32"]["kernel_ms"] / res["mfma"]["kernel_ms"], 3)
         forms_is32[f'{j["alg"]}{j["isz"]}'] = res
 
-    # roofline of the dominant kernel
+    # roofline of the dominant run (its launch: for "split-f16+f64" the
+    # split-precision kernel, then the FP64 kernel over the IAs it flagged).
+    # The split-precision form is bound by the matrix cores: its algorithmic
+    # flops are the dense-DFT products' (flop_dft_per_ia), against the dense
+    # FP16 peak; the radix forms by the vector unit (flop_fft_per_ia)
     kd = int(np.argmax(kern_ms))
     jd = jobs[kd]
     cs = jd["isz"] + 2 * SM  # reference transform size (good_size_real is the identity here)
-    flop = flop_fft_per_ia(cs) * jd["n"]
-    achieved = flop / (kern_ms[kd] * 1e-3) / 1e12
-    peak = PEAK_FP64_TFLOPS if jd["form"] == "f64" else PEAK_FP32_TFLOPS
+    split = jd["form"].startswith("split-f16")
+    flop_ia = flop_dft_per_ia(cs) if split else flop_fft_per_ia(cs)
+    achieved = flop_ia * jd["n"] / (kern_ms[kd] * 1e-3) / 1e12
+    peak = PEAK_FP16_MFMA_TFLOPS if split else (PEAK_FP64_TFLOPS if jd["form"] == "f64" else PEAK_FP32_TFLOPS)
     traffic = None
     tfile = ROOT / "profiles" / "piv2d_2048_zncc_nsqecc_is32_is64.md"
     if tfile.exists():
@@ -685,11 +690,12 @@ def main() -> int:
         "latency_ms": {"p50": round(float(np.percentile(step_ms, 50)), 4),
                        "p99": round(float(np.percentile(step_ms, 99)), 4),
                        "what": "one frame pair through all four runs"},
-        "roofline": {"bound": "valu", "kernel": f"piv2d_kernel {jd['alg']} interSize {jd['isz']} "
-                                                 f"({jd['form']} transforms)",
+        "roofline": {"bound": "mfma" if split else "valu",
+                     "kernel": (f"piv2d_wave_kernel {jd['alg']} interSize {jd['isz']}" if split else
+                                f"piv2d_kernel {jd['alg']} interSize {jd['isz']}") + f" ({jd['form']} transforms)",
                      "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
                      "frac": round(achieved / peak, 4), "traffic": traffic,
-                     "flop_per_ia": round(flop_fft_per_ia(cs)), "ia_per_launch": jd["n"],
+                     "flop_per_ia": round(flop_ia), "ia_per_launch": jd["n"],
                      "kernel_ms": round(kern_ms[kd], 4)},
         "breakdown": breakdown,
         "per_call": {"value": round(n_step * args.steps / wall_calls, 1),

@@ -71,7 +71,11 @@ extern "C" {
 #define MQPIV_FP64 2       /* every step in FP64 (2-D search sizes up to 128)                    */
 #define MQPIV_AUTO 3       /* FP64 where FP32-level transforms miss the parity bar (masked
                               NSQECC, and ZNCC when SN is requested); elsewhere the split-
-                              precision form where it takes the job, FP32 transforms otherwise */
+                              precision form where it takes the job, FP32 transforms otherwise.
+                              ZNCC with SN runs the split-precision form where it takes the
+                              job, and the FP64 transforms recompute the IAs whose SN, peak
+                              or subpixel fit it cannot settle to the FP64 results' bar
+                              (form 3 of mqpiv_transform_form)                                */
 
 /* The fields of PIVParameters (src/piv_parameters.jl:29-46) that the device
  * needs, already resolved: step = interSize - overlap when the reference's
@@ -91,7 +95,10 @@ typedef struct mqpiv_params {
 typedef struct mqpiv_ctx mqpiv_ctx;
 
 /* Create a context on HIP device `device`.  hbm_budget_bytes = 0 means no limit
- * beyond the device's memory.  Returns MQPIV_OK and sets *out. */
+ * beyond the device's memory.  Returns MQPIV_OK and sets *out.  A context's
+ * device buffers (integral arrays, job tables, refine lists, transform
+ * buffers) serve every call made through it: calls through one context must
+ * not overlap on the device (one stream per context, as the server's slots). */
 int mqpiv_ctx_create(int device, uint64_t hbm_budget_bytes, mqpiv_ctx** out);
 int mqpiv_ctx_destroy(mqpiv_ctx* ctx);
 const char* mqpiv_last_error(const mqpiv_ctx* ctx);
@@ -100,9 +107,10 @@ const char* mqpiv_last_error(const mqpiv_ctx* ctx);
 int mqpiv_vf_size(const mqpiv_params* p, const int64_t* dims, int64_t* vf_out3, int64_t* n_ia);
 
 /* The transform form a call with these parameters and dims runs: *form = 0
- * (FP32 transforms), 1 (FP64 transforms) or 2 (the split-precision matrix-core
- * form), *length = the transform length (2-D; for the split form 64, or 65 for
- * IAs taken in 2 x 2 sub-blocks of length-64 products).  Host-only; no device
+ * (FP32 transforms), 1 (FP64 transforms), 2 (the split-precision matrix-core
+ * form) or 3 (that form, then FP64 transforms over the IAs it flags; ZNCC with
+ * SN in MQPIV_AUTO), *length = the transform length (2-D; for the split form
+ * 64, or 65 for IAs taken in 2 x 2 sub-blocks of length-64 products).  Host-only; no device
  * work.  Same errors as the call itself would report for an unsupported job.
  * No counterpart in the reference (its transforms are always FFTW at
  * good_size_real, src/utils/fft_ops.jl:145-198); it lets a caller see which

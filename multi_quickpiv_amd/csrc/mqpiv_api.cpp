@@ -29,7 +29,7 @@ namespace mqpiv {
 int fft_len_2d(int fp64, int ss);
 size_t lds_2d(int fp64, int L, int alg, int ns0, int ns1, int ss0);
 hipError_t launch_piv2d(int fp64, int L, int alg, const Job& j0, const Job* d_jobs, const int64_t* d_prefix,
-                        int n_jobs, int64_t n_ia_total, int ns0, int ns1, int ss0, void* gz, hipStream_t s);
+                        int n_jobs, int64_t n_ia_total, int ns0, int ns1, int ss0, void* gz, hipStream_t s, int Lr);
 size_t gz_scratch_2d(int fp64, int L, int64_t n_ia);
 int wave2d_len(int alg, int is0, int is1, int ss, int ns0, int ns1);
 size_t sat_bytes(const int64_t* dims, int nd);
@@ -314,6 +314,11 @@ static int prepare_sats(mqpiv_ctx* c, Job& j, hipStream_t s, char* ws = nullptr,
   return MQPIV_OK;
 }
 
+// FP64 transform length that recomputes the flagged IAs of a precision-3 job.
+static int refine_len(const Job& j) {
+  return fft_len_2d(1, std::max(j.is[0] + 2 * j.sm[0], j.is[1] + 2 * j.sm[1]));
+}
+
 // Precision and transform length of a 2-D job (see MQPIV_AUTO in mqpiv.h).
 static int choose_2d(mqpiv_ctx* c, const Job& j, int* fp64_out, int* L_out) {
   const int ss0 = j.is[0] + 2 * j.sm[0], ss1 = j.is[1] + 2 * j.sm[1];
@@ -350,11 +355,13 @@ static int choose_2d(mqpiv_ctx* c, const Job& j, int* fp64_out, int* L_out) {
     bool want = j.alg == ALG_MASK_NSQECC || (j.alg == ALG_ZNCC && j.compute_sn);
     fp64 = want && fits(1);
     // where FP32-level transforms meet the bar, the split-precision matrix-core
-    // form when it takes the job (it runs faster than the FP32 transforms)
-    if (!fp64 && !j.shift) {
+    // form when it takes the job (it runs faster than the FP32 transforms); for
+    // ZNCC with SN, that form with the IAs it flags recomputed by the FP64
+    // transforms (precision 3)
+    if (!j.shift && (!fp64 || j.alg == ALG_ZNCC)) {
       const int Lw = wave2d_len(j.alg, j.is[0], j.is[1], ss, ns0, ns1);
       if (Lw > 0) {
-        *fp64_out = 2;
+        *fp64_out = fp64 ? 3 : 2;
         *L_out = Lw;
         return MQPIV_OK;
       }
@@ -401,25 +408,38 @@ extern "C" int mqpiv_transform_form(const mqpiv_params* p, const int64_t* dims, 
   return MQPIV_OK;
 }
 
-static int launch_job(mqpiv_ctx* c, Job& j, hipStream_t s, char* ws = nullptr) {
+// ws: workspace for the integral arrays (else the context's buffer).  form /
+// gz (2-D, the allocation-free entry point): the precision and length to run
+// and the buffer of gz_scratch_2d bytes for them, instead of choose_2d's and the
+// context's.
+static int launch_job(mqpiv_ctx* c, Job& j, hipStream_t s, char* ws = nullptr, const int* form = nullptr,
+                      char* gz = nullptr) {
   if (j.n_ia == 0) return MQPIV_OK;
   if (j.ndims == 2) {
     const int ss0 = j.is[0] + 2 * j.sm[0];
     const int ns0 = 2 * j.sm[0] + 1, ns1 = 2 * j.sm[1] + 1;
     int fp64 = 0, L = 0;
-    int rc2 = choose_2d(c, j, &fp64, &L);
+    int rc2 = MQPIV_OK;
+    if (form) {
+      fp64 = form[0];
+      L = form[1];
+    } else {
+      rc2 = choose_2d(c, j, &fp64, &L);
+    }
     if (rc2) return rc2;
     rc2 = prepare_sats(c, j, s, ws, fp64);
     if (rc2) return rc2;
     const size_t gzb = gz_scratch_2d(fp64, L, j.n_ia);
-    if (gzb) {
+    if (gzb && !gz) {
       rc2 = ensure_gz(c, gzb, s);
       if (rc2) return rc2;
+      gz = (char*)c->d_gz;
     }
     hipError_t e;
     {
       KernelTimer kt(c, s);
-      e = launch_piv2d(fp64, L, j.alg, j, nullptr, nullptr, 1, j.n_ia, ns0, ns1, ss0, c->d_gz, s);
+      e = launch_piv2d(fp64, L, j.alg, j, nullptr, nullptr, 1, j.n_ia, ns0, ns1, ss0, gz, s,
+                       fp64 == 3 ? refine_len(j) : 0);
     }
     if (e != hipSuccess) return fail(c, MQPIV_ERR_DEVICE, "2-D launch failed: %s", hipGetErrorString(e));
     return MQPIV_OK;
@@ -474,7 +494,13 @@ extern "C" int mqpiv_piv_device_noalloc(mqpiv_ctx* c, const mqpiv_params* p, con
   int fp64 = 0, L = 0;
   rc = choose_2d(c, j, &fp64, &L);
   if (rc) return rc;
-  if (gz_scratch_2d(fp64, L, j.n_ia))
+  // precision 3 keeps its refine list after the integral arrays; where the two
+  // do not fit, the FP64 transforms run every IA (they need no integral arrays)
+  if (fp64 == 3 && job_ws_bytes(j, 3) + gz_scratch_2d(3, L, j.n_ia) > MQPIV_NOALLOC_SCRATCH_BYTES) {
+    fp64 = 1;
+    L = refine_len(j);
+  }
+  if (fp64 != 3 && gz_scratch_2d(fp64, L, j.n_ia))
     return fail(c, MQPIV_ERR_UNSUPPORTED,
                 "the allocation-free entry point does not run FP64 transforms at search size 128 (global transform buffers)");
   void* ptr = nullptr;
@@ -490,7 +516,8 @@ extern "C" int mqpiv_piv_device_noalloc(mqpiv_ctx* c, const mqpiv_params* p, con
     return fail(c, MQPIV_ERR_OOM, "the pair needs %zu bytes of integral arrays; the static scratch holds %llu", need,
                 (unsigned long long)MQPIV_NOALLOC_SCRATCH_BYTES);
   hipStream_t s = stream ? (hipStream_t)stream : c->stream;
-  return launch_job(c, j, s, need ? (char*)ptr : nullptr);
+  const int form[2] = {fp64, L};
+  return launch_job(c, j, s, need ? (char*)ptr : nullptr, form, fp64 == 3 ? (char*)ptr + need : nullptr);
 }
 
 extern "C" int mqpiv_piv(mqpiv_ctx* c, const mqpiv_params* p, const int64_t* dims, int32_t in_dtype,
@@ -753,6 +780,7 @@ struct BatchGroup {
   std::vector<int> members;  // indices into BatchPlan::jobs2d
   int64_t n_ia = 0;
   int ns0cap = 1, ns1cap = 1, ss0cap = 1;
+  int Lr = 0;            // precision 3: the FP64 transform length of every member
   size_t table_off = 0;  // byte offset of [Job x m][int64 prefix x (m+1)]
 };
 
@@ -775,7 +803,7 @@ size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 static int plan_batch(mqpiv_ctx* c, const mqpiv_request* r, int n, BatchPlan& P) {
   if (n < 0 || (n > 0 && !r)) return fail(c, MQPIV_ERR_NULL, "requests is NULL");
-  std::map<std::tuple<int, int, int>, int> key2g;
+  std::map<std::tuple<int, int, int, int>, int> key2g;
   // jobs that build integral arrays, by second image (pointer, type, size):
   // a later job of the same image reads the arrays of one that builds at
   // least as many (power 2, then power 1) instead of building its own
@@ -816,7 +844,10 @@ static int plan_batch(mqpiv_ctx* c, const mqpiv_request* r, int n, BatchPlan& P)
     } else {
       P.sat_off.push_back(SIZE_MAX);
     }
-    auto key = std::make_tuple(fp64, L, (int)j.alg);
+    // precision 3 groups by its FP64 length too: a recomputed IA then gets the
+    // values of the same call made alone
+    const int Lr = fp64 == 3 ? refine_len(j) : 0;
+    auto key = std::make_tuple(fp64, L, (int)j.alg, Lr);
     auto it = key2g.find(key);
     int g;
     if (it == key2g.end()) {
@@ -838,9 +869,11 @@ static int plan_batch(mqpiv_ctx* c, const mqpiv_request* r, int n, BatchPlan& P)
     bg.ns0cap = std::max(bg.ns0cap, 2 * j.sm[0] + 1);
     bg.ns1cap = std::max(bg.ns1cap, 2 * j.sm[1] + 1);
     bg.ss0cap = std::max(bg.ss0cap, j.is[0] + 2 * j.sm[0]);
+    bg.Lr = Lr;
   }
   for (auto& g : P.groups) {
-    if (lds_2d(g.fp64, g.L, g.alg, g.ns0cap, g.ns1cap, g.ss0cap) > 160 * 1024)
+    if (lds_2d(g.fp64, g.L, g.alg, g.ns0cap, g.ns1cap, g.ss0cap) > 160 * 1024 ||
+        (g.fp64 == 3 && lds_2d(1, g.Lr, g.alg, g.ns0cap, g.ns1cap, g.ss0cap) > 160 * 1024))
       return fail(c, MQPIV_ERR_UNSUPPORTED, "batch group (L=%d) does not fit in LDS", g.L);
     g.table_off = P.table_bytes;
     P.gz_bytes = std::max(P.gz_bytes, gz_scratch_2d(g.fp64, g.L, g.n_ia));
@@ -901,7 +934,7 @@ static int enqueue_2d(mqpiv_ctx* c, BatchPlan& P, char* sat_base, char* d_table,
       // every workgroup would load before it can address its windows
       const bool one = g.members.size() == 1;
       e = launch_piv2d(g.fp64, g.L, g.alg, P.jobs2d[g.members[0]], one ? nullptr : dj, one ? nullptr : dp,
-                       (int)g.members.size(), g.n_ia, g.ns0cap, g.ns1cap, g.ss0cap, gz, s);
+                       (int)g.members.size(), g.n_ia, g.ns0cap, g.ns1cap, g.ss0cap, gz, s, g.Lr);
     }
     if (e != hipSuccess) return fail(c, MQPIV_ERR_DEVICE, "batch launch failed: %s", hipGetErrorString(e));
   }

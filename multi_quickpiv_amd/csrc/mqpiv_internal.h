@@ -75,6 +75,13 @@ constexpr int kGzMaxSlots = 1024;
 constexpr size_t kGzFlagBytes = kGzMaxSlots * sizeof(int32_t);
 constexpr uint32_t kGzProbeLimit = 1u << 24;  // probes before a block gives up (NaN outputs)
 
+// Refine list of a split-precision ZNCC launch: the IAs whose results the FP64
+// transforms recompute.  int32 count at [0], the global IA indices of the
+// launch (the order in which the flagging kernel appended them) from
+// [kRefineHead]; at most one entry per IA.
+constexpr int kRefineHead = 64;
+inline size_t refine_bytes(int64_t n_ia) { return ((size_t)(kRefineHead + n_ia) * sizeof(int32_t) + 255) & ~(size_t)255; }
+
 // Compute units of the current device (cached per device).
 inline int device_cus() {
   static std::mutex mu;

@@ -9,7 +9,7 @@ namespace mqpiv {
 namespace {
 template <int L, int NTS, int ALG, int NQ>
 hipError_t launch_wave_k(const Job& j0, const Job* d_jobs, const int64_t* d_prefix, int n_jobs, int64_t n_ia_total,
-                         int ns0, int ns1, hipStream_t s) {
+                         int ns0, int ns1, int32_t* refine, hipStream_t s) {
   using C = WvCfg<L, NTS>;
   constexpr int WAVES = WvOcc<NQ>::WAVES;
   const void* fn = (const void*)piv2d_wave_kernel<L, NTS, ALG, NQ>;
@@ -21,19 +21,22 @@ hipError_t launch_wave_k(const Job& j0, const Job* d_jobs, const int64_t* d_pref
   int64_t nb = device_cus();
   if (need < nb) nb = need;
   hipLaunchKernelGGL((piv2d_wave_kernel<L, NTS, ALG, NQ>), dim3((unsigned)nb), dim3(64 * WAVES), lds, s, j0, d_jobs,
-                     d_prefix, n_jobs, n_ia_total, ns0, ns1);
+                     d_prefix, n_jobs, n_ia_total, ns0, ns1, refine);
   return hipGetLastError();
 }
 
 template <int L, int NTS, int NQ>
 hipError_t launch_wave_l(int alg, const Job& j0, const Job* d_jobs, const int64_t* d_prefix, int n_jobs,
-                         int64_t n_ia_total, int ns0, int ns1, hipStream_t s) {
+                         int64_t n_ia_total, int ns0, int ns1, int32_t* refine, hipStream_t s) {
+  // only ZNCC flags IAs for the FP64 transforms
+  if (refine && alg != ALG_ZNCC) return hipErrorInvalidValue;
   switch (alg) {
     case ALG_FFTCC:
-      return launch_wave_k<L, NTS, ALG_FFTCC, NQ>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, s);
-    case ALG_ZNCC: return launch_wave_k<L, NTS, ALG_ZNCC, NQ>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, s);
+      return launch_wave_k<L, NTS, ALG_FFTCC, NQ>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, nullptr, s);
+    case ALG_ZNCC:
+      return launch_wave_k<L, NTS, ALG_ZNCC, NQ>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, refine, s);
     case ALG_NSQECC:
-      return launch_wave_k<L, NTS, ALG_NSQECC, NQ>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, s);
+      return launch_wave_k<L, NTS, ALG_NSQECC, NQ>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, nullptr, s);
   }
   return hipErrorInvalidValue;
 }
@@ -72,12 +75,23 @@ size_t wave2d_lds(int L, int ns0, int ns1) {
   return (size_t)-1;
 }
 
+// refine: a refine list (count cleared by refine_reset) that ZNCC appends the
+// IAs whose FP32-level results may miss the FP64 results' bar to, or nullptr.
 hipError_t launch_wave2d(int L, int alg, const Job& j0, const Job* d_jobs, const int64_t* d_prefix, int n_jobs,
-                         int64_t n_ia_total, int ns0, int ns1, hipStream_t s) {
+                         int64_t n_ia_total, int ns0, int ns1, int32_t* refine, hipStream_t s) {
   if (n_ia_total <= 0) return hipSuccess;
-  if (L == 64) return launch_wave_l<64, 3, 1>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, s);
-  if (L == 65) return launch_wave_l<64, 3, 2>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, s);
+  if (L == 64) return launch_wave_l<64, 3, 1>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, refine, s);
+  if (L == 65) return launch_wave_l<64, 3, 2>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, refine, s);
   return hipErrorInvalidValue;
+}
+
+// Clears a refine list's count with a kernel (a launch that may be captured
+// into a graph records no memset node).
+__global__ void refine_reset_kernel(int32_t* refine) { refine[0] = 0; }
+
+hipError_t refine_reset(int32_t* refine, hipStream_t s) {
+  hipLaunchKernelGGL(refine_reset_kernel, dim3(1), dim3(1), 0, s, refine);
+  return hipGetLastError();
 }
 
 }  // namespace mqpiv

@@ -2,6 +2,8 @@
 // transform length; the kernel instances live in mqpiv_k2d_<type>_<L>.hip (one
 // translation unit each so they compile in parallel).
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 #include <stdint.h>
 
 #include "mqpiv_internal.h"
@@ -51,11 +53,25 @@ extern template size_t lds_bytes_2d<128, double, Lpl2<double, 128>::v>(int, int,
 extern template hipError_t launch_2d_l<128, double, Lpl2<double, 128>::v>(int, const Job&, const Job*, const int64_t*, int,
                                                     int64_t, int, int, int, char*, hipStream_t);
 
-// The split-precision matrix-core form (precision 2 below; mqpiv_k2d_wave.hip).
+template <int L, typename T, int LPL>
+hipError_t launch_2d_refine(const Job& j0, const Job* d_jobs, const int64_t* d_prefix, int n_jobs,
+                            int64_t n_ia_total, int ns0, int ns1, int ss0, const int32_t* list, hipStream_t s);
+#define MQPIV_REFINE_EXTERN(L)                                                                                    \
+  extern template hipError_t launch_2d_refine<L, double, Lpl2<double, L>::v>(                                     \
+      const Job&, const Job*, const int64_t*, int, int64_t, int, int, int, const int32_t*, hipStream_t);
+MQPIV_REFINE_EXTERN(16)
+MQPIV_REFINE_EXTERN(32)
+MQPIV_REFINE_EXTERN(48)
+MQPIV_REFINE_EXTERN(64)
+MQPIV_REFINE_EXTERN(96)
+#undef MQPIV_REFINE_EXTERN
+
+// The split-precision matrix-core form (precisions 2 and 3 below; mqpiv_k2d_wave.hip).
 int wave2d_len(int alg, int is0, int is1, int ss, int ns0, int ns1);
 size_t wave2d_lds(int L, int ns0, int ns1);
 hipError_t launch_wave2d(int L, int alg, const Job& j0, const Job* d_jobs, const int64_t* d_prefix, int n_jobs,
-                         int64_t n_ia_total, int ns0, int ns1, hipStream_t s);
+                         int64_t n_ia_total, int ns0, int ns1, int32_t* refine, hipStream_t s);
+hipError_t refine_reset(int32_t* refine, hipStream_t s);
 
 // Transform length for a search size: the smallest supported length >= ss
 // (up to 128; FP64 at 128 keeps its transform buffer in global memory), or -1.
@@ -73,9 +89,10 @@ int fft_len_2d(int fp64, int ss) {
 }
 
 // fp64: 0 FP32 radix transforms, 1 FP64 radix transforms, 2 the split-precision
-// matrix-core form.
+// matrix-core form, 3 that form with FP64 transforms recomputing the IAs it
+// flags (ZNCC; the FP64 kernel's LDS is checked by the caller).
 size_t lds_2d(int fp64, int L, int alg, int ns0, int ns1, int ss0) {
-  if (fp64 == 2) return wave2d_lds(L, ns0, ns1);
+  if (fp64 == 2 || fp64 == 3) return wave2d_lds(L, ns0, ns1);
   if (fp64) {
     switch (L) {
       case 16: return lds_bytes_2d<16, double, Lpl2<double, 16>::v>(alg, ns0, ns1, ss0);
@@ -101,15 +118,33 @@ size_t lds_2d(int fp64, int L, int alg, int ns0, int ns1, int ss0) {
 // Bytes of global transform buffers (slot flags, then the buffers) a launch of
 // n_ia IAs at (fp64, L) needs, or 0 for lengths whose buffer is in LDS.  The
 // flags must be zero when a launch starts; every launch leaves them zero.
+// Precision 3 needs its refine list there instead.
 size_t gz_scratch_2d(int fp64, int L, int64_t n_ia) {
+  if (fp64 == 3 && n_ia > 0) return std::max(refine_bytes(n_ia), kGzFlagBytes);  // the allocation clears the flags' bytes
   if (!(fp64 == 1 && gz_len<128, double>() && L == 128) || n_ia <= 0) return 0;
   return kGzFlagBytes + (size_t)gz_slots_2d(n_ia) * gz_stride_2d<128, double>();
 }
 
+// Lr: precision 3's FP64 transform length (>= the launch's largest search size).
 hipError_t launch_piv2d(int fp64, int L, int alg, const Job& j0, const Job* d_jobs, const int64_t* d_prefix,
-                        int n_jobs, int64_t n_ia_total, int ns0, int ns1, int ss0, void* gz, hipStream_t s) {
+                        int n_jobs, int64_t n_ia_total, int ns0, int ns1, int ss0, void* gz, hipStream_t s, int Lr) {
   if (n_ia_total <= 0) return hipSuccess;
-  if (fp64 == 2) return launch_wave2d(L, alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, s);
+  if (fp64 == 2) return launch_wave2d(L, alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, nullptr, s);
+  if (fp64 == 3) {
+    if (alg != ALG_ZNCC || !gz) return hipErrorInvalidValue;
+    int32_t* list = (int32_t*)gz;
+    hipError_t e = refine_reset(list, s);
+    if (e == hipSuccess) e = launch_wave2d(L, alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, list, s);
+    if (e != hipSuccess) return e;
+    switch (Lr) {
+      case 16: return launch_2d_refine<16, double, Lpl2<double, 16>::v>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, list, s);
+      case 32: return launch_2d_refine<32, double, Lpl2<double, 32>::v>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, list, s);
+      case 48: return launch_2d_refine<48, double, Lpl2<double, 48>::v>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, list, s);
+      case 64: return launch_2d_refine<64, double, Lpl2<double, 64>::v>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, list, s);
+      case 96: return launch_2d_refine<96, double, Lpl2<double, 96>::v>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, list, s);
+    }
+    return hipErrorInvalidValue;
+  }
   if (fp64) {
     switch (L) {
       case 16: return launch_2d_l<16, double, Lpl2<double, 16>::v>(alg, j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, (char*)gz, s);

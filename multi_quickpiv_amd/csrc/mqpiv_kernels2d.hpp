@@ -527,20 +527,32 @@ constexpr int min_waves_2d() {
 // none of its registers.
 // gz / gz_slots: the global parts of the transform buffers of a gz_len launch
 // (gz_slots of them after the slot flags), else unused.
-template <int L, int ALG, typename T, int LPL, bool SH>
+// LIST: the IAs are the entries of a device list (refine_list layout: count at
+// list[0], global IA indices from list[kRefineHead]), written by an earlier
+// kernel on the stream; the grid is persistent and every block steps through
+// the list by the grid size until the count is reached.
+template <int L, int ALG, typename T, int LPL, bool SH, bool LIST>
 __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL>())) void piv2d_kernel(Job j0, const Job* __restrict__ jobs,
                                                                       const int64_t* __restrict__ prefix,
                                                                       int n_jobs, int64_t n_ia_total, int ns0_cap,
-                                                                      int ns1_cap, char* gz, int gz_slots) {
+                                                                      int ns1_cap, char* gz, int gz_slots,
+                                                                      const int32_t* __restrict__ list) {
   const int32_t vb = xcd_remap32((int32_t)blockIdx.x, (int32_t)gridDim.x);
   using C = Cfg2<L, T, LPL>;
   constexpr int H = C::H;
   constexpr bool GZ = gz_len<L, T>();
   static_assert(!GZ || C::IPB == 1, "a global transform buffer holds one IA");
+  static_assert(!(LIST && (GZ || SH)), "list launches use LDS buffers and unshifted regions");
   extern __shared__ __align__(16) unsigned char smem[];
+  int32_t lpos = vb;  // LIST: this block's current list position
 
-  for (;;) {  // runs once (a loop keeps the register allocation of this body as measured)
-  const int tid = (int)threadIdx.x;
+  // without LIST the body runs once (a loop keeps the register allocation of
+  // this body as measured)
+  for (;;) {
+  int tid = (int)threadIdx.x;
+  // LIST: the thread index is re-read every iteration, so that values derived
+  // from it are not hoisted out of the loop and held across the body
+  if constexpr (LIST) __asm__ volatile("" : "+v"(tid));
   // one IA per block: every IA quantity below is uniform over the block, which
   // lets the compiler keep it in scalar registers
   const int t = C::IPB == 1 ? tid : tid % C::TPI;
@@ -549,10 +561,11 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
   const int h = t % LPL;  // lane within the line's group
 
   // IA indices in 32-bit arithmetic (the host keeps a launch below 2^31 IAs)
-  const int32_t nt = (int32_t)n_ia_total;
-  const int32_t ia_g = vb * C::IPB + sub;
+  const int32_t nt = LIST ? list[0] : (int32_t)n_ia_total;
+  if (LIST && lpos * C::IPB >= nt) break;  // uniform over the block
+  const int32_t ia_g = (LIST ? lpos : vb) * C::IPB + sub;
   const bool valid = ia_g < nt;
-  const int32_t ia_c = valid ? ia_g : nt - 1;
+  const int32_t ia_c = LIST ? list[kRefineHead + (valid ? ia_g : nt - 1)] : (valid ? ia_g : nt - 1);
   Job J = j0;
   int32_t ia32 = ia_c;
   if (jobs) {  // batched launch: locate the owning job
@@ -1222,7 +1235,9 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
       atomicExch(reinterpret_cast<int32_t*>(gz) + gz_slot, 0);
     }
   }
-  break;
+  if constexpr (!LIST) break;
+  lpos += (int32_t)gridDim.x;
+  __syncthreads();  // the next IA rewrites the LDS buffers
   }
 }
 
@@ -1244,7 +1259,7 @@ template <int L, int ALG, typename T, int LPL, bool SH>
 hipError_t launch_2d_k(const Job& j0, const Job* d_jobs, const int64_t* d_prefix, int n_jobs, int64_t n_ia_total,
                        int ns0, int ns1, size_t lds, char* gz, hipStream_t s) {
   using C = Cfg2<L, T, LPL>;
-  const void* fn = (const void*)piv2d_kernel<L, ALG, T, LPL, SH>;
+  const void* fn = (const void*)piv2d_kernel<L, ALG, T, LPL, SH, false>;
   allow_full_lds(fn);
   const int64_t nb = (n_ia_total + C::IPB - 1) / C::IPB;
   int slots = 0;
@@ -1252,8 +1267,32 @@ hipError_t launch_2d_k(const Job& j0, const Job* d_jobs, const int64_t* d_prefix
     if (!gz) return hipErrorInvalidValue;  // the caller provides gz_scratch_2d bytes
     slots = gz_slots_2d(n_ia_total);
   }
-  hipLaunchKernelGGL((piv2d_kernel<L, ALG, T, LPL, SH>), dim3((unsigned)nb), dim3(C::NT), lds, s, j0, d_jobs,
-                     d_prefix, n_jobs, n_ia_total, ns0, ns1, gz, slots);
+  hipLaunchKernelGGL((piv2d_kernel<L, ALG, T, LPL, SH, false>), dim3((unsigned)nb), dim3(C::NT), lds, s, j0, d_jobs,
+                     d_prefix, n_jobs, n_ia_total, ns0, ns1, gz, slots, (const int32_t*)nullptr);
+  return hipGetLastError();
+}
+
+// ZNCC over the IAs of a refine list (refine_list layout) that an earlier
+// kernel on the stream wrote: a persistent grid of as many blocks as the
+// transform buffers let the CUs hold, each stepping through the list.
+template <int L, typename T, int LPL>
+hipError_t launch_2d_refine(const Job& j0, const Job* d_jobs, const int64_t* d_prefix, int n_jobs,
+                            int64_t n_ia_total, int ns0, int ns1, int ss0, const int32_t* list, hipStream_t s) {
+  static_assert(!gz_len<L, T>(), "LDS transform buffers only");
+  using C = Cfg2<L, T, LPL>;
+  const void* fn = (const void*)piv2d_kernel<L, ALG_ZNCC, T, LPL, false, true>;
+  allow_full_lds(fn);
+  const size_t lds = lds_bytes_2d<L, T, LPL>(ALG_ZNCC, ns0, ns1, ss0);
+  if (lds > 160 * 1024 || !list) return hipErrorInvalidValue;
+  const int64_t by_lds = (int64_t)((160 * 1024) / (lds > 0 ? lds : 1));
+  const int64_t by_waves = (int64_t)min_waves_2d<L, ALG_ZNCC, T, LPL>() * 4 * 64 / C::NT;
+  int64_t per_cu = by_lds < by_waves ? by_lds : by_waves;
+  if (per_cu < 1) per_cu = 1;
+  int64_t nb = (n_ia_total + C::IPB - 1) / C::IPB;
+  if (nb > per_cu * device_cus()) nb = per_cu * device_cus();
+  if (nb < 1) return hipSuccess;
+  hipLaunchKernelGGL((piv2d_kernel<L, ALG_ZNCC, T, LPL, false, true>), dim3((unsigned)nb), dim3(C::NT), lds, s, j0,
+                     d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, (char*)nullptr, 0, list);
   return hipGetLastError();
 }
 

@@ -59,6 +59,25 @@ __device__ __forceinline__ void wv_sb() {
 #endif
 }
 
+// When ZNCC runs with a refine list, an IA goes to the FP64 transforms when its
+// FP32-level values could move an output past the FP64 results' bar (integer
+// peak identical, SN within 1e-4 relative, displacement within 0.01 px):
+// * SN = max / avg, avg the mean over the SN window.  avg's error stayed below
+//   3.6e-7 x mean|r| over ~12,000 IAs of four kinds of particle image (dense,
+//   sparse, strong background and noise, a 100x illumination ramp; interSize
+//   32 and 64), so |avg| >= kRefineSnK x mean|r| keeps SN's relative error
+//   below 1.8e-5;
+// * the peak: the two largest values of the peak window (or the largest and 0)
+//   at least kRefinePeak apart, ten times the largest value error expected;
+// * the Gaussian fit: |D| >= kRefineFit (1 + 4|q|) per dimension (D the fit's
+//   denominator, q its offset), which bounds q's error by 1e-3 px for a log
+//   error of 2e-5;
+// * every value of the peak window finite, and every window energy there
+//   keeping 24 bits of its non-negative terms.
+constexpr double kRefineSnK = 0.02;
+constexpr float kRefinePeak = 1e-4f;
+constexpr double kRefineFit = 0.02;
+
 typedef _Float16 wh8 __attribute__((ext_vector_type(8)));
 typedef float wf4 __attribute__((ext_vector_type(4)));
 typedef float wf8 __attribute__((ext_vector_type(8)));
@@ -214,7 +233,7 @@ struct WvOcc {
 template <int L, int NTS, int ALG, int NQ>
 __global__ __launch_bounds__((64 * WvOcc<NQ>::WAVES), (WvOcc<NQ>::OCC)) void piv2d_wave_kernel(
     Job j0, const Job* __restrict__ jobs, const int64_t* __restrict__ prefix, int n_jobs, int64_t n_ia_total,
-    int ns0_cap, int ns1_cap) {
+    int ns0_cap, int ns1_cap, int32_t* __restrict__ refine) {
   using C = WvCfg<L, NTS>;
   constexpr int NT = C::NT, KS = C::KS, NH = C::NH, FT = C::FT, FK = C::FK;
   constexpr int WAVES = WvOcc<NQ>::WAVES;
@@ -813,6 +832,14 @@ __global__ __launch_bounds__((64 * WvOcc<NQ>::WAVES), (WvOcc<NQ>::OCC)) void piv
     const int nE = ns0 * ns1;
     Best<float> bst{0.f, INT32_MAX};
     double mx = -INFINITY, smv = 0.0;
+    // ZNCC with a refine list: what decides whether this IA's FP32-level values
+    // meet the FP64 results' bar -- the two largest values of the peak window,
+    // the sum of |values| over the SN window, and whether a value there is not
+    // finite or a window energy lost its digits to cancellation
+    const bool rf = ALG == ALG_ZNCC && refine != nullptr;
+    float t1 = -INFINITY, t2 = -INFINITY;
+    double sab = 0.0;
+    bool rbad = false;
     const int st0 = 64 % ns0, st1 = 64 / ns0;
     int w0 = lane % ns0, w1 = lane / ns0;
     auto step = [&]() {
@@ -824,11 +851,22 @@ __global__ __launch_bounds__((64 * WvOcc<NQ>::WAVES), (WvOcc<NQ>::OCC)) void piv
       }
     };
     auto visit = [&](int e, int p0, int p1, float v) {
-      if (p0 >= lo0 && p0 <= hi0 && p1 >= lo1 && p1 <= hi1 && v > bst.v) bst = Best<float>{v, e};
+      const bool in_pk = p0 >= lo0 && p0 <= hi0 && p1 >= lo1 && p1 <= hi1;
+      if (in_pk && v > bst.v) bst = Best<float>{v, e};
+      if constexpr (ALG == ALG_ZNCC) {
+        if (rf && in_pk) {
+          rbad |= !(__builtin_fabsf(v) <= 3.4028235e38f);
+          t2 = fmaxf(t2, fminf(t1, v));
+          t1 = fmaxf(t1, v);
+        }
+      }
       if (want_sn && p0 >= lo0 && p0 <= shi0 && p1 >= lo1 && p1 <= shi1) {
         const double dv = (double)v;
         mx = (dv > mx || dv != dv) ? dv : mx;
         smv += dv;
+        if constexpr (ALG == ALG_ZNCC) {
+          if (rf) sab += __builtin_fabs(dv);
+        }
       }
     };
     if constexpr (ALG == ALG_ZNCC || ALG == ALG_NSQECC) {
@@ -882,7 +920,11 @@ __global__ __launch_bounds__((64 * WvOcc<NQ>::WAVES), (WvOcc<NQ>::OCC)) void piv
           double W = ((q[jj][0][0] - q[jj][0][1]) - q[jj][0][2]) + q[jj][0][3];
           if constexpr (ALG == ALG_ZNCC) {
             const double S1 = ((q[jj][NQC - 1][0] - q[jj][NQC - 1][1]) - q[jj][NQC - 1][2]) + q[jj][NQC - 1][3];
-            W = (W - 2.0 * mu * S1) + nbox[jj] * mu * mu;
+            const double G2 = W, M2 = nbox[jj] * mu * mu;
+            W = (W - 2.0 * mu * S1) + M2;
+            // the energy keeps at least 24 bits of its non-negative terms
+            if (rf && pw0[jj] >= lo0 && pw0[jj] <= hi0 && pw1[jj] >= lo1 && pw1[jj] <= hi1)
+              rbad |= !(W * 0x1p24 >= G2 + M2);
           }
           W = W > 0.0 ? W : 0.0;
           const double cv = (double)win[e] * cscale;
@@ -919,12 +961,24 @@ __global__ __launch_bounds__((64 * WvOcc<NQ>::WAVES), (WvOcc<NQ>::OCC)) void piv
     bst.i = bcast63(bst.i);
     mx = bcast63(mx);
     smv = bcast63(smv);
-    double sn = 0.0;
-    if (want_sn) {
-      const int u0 = shi0 - lo0 + 1, u1 = shi1 - lo1 + 1;
-      const int nsn = (u0 > 0 && u1 > 0) ? u0 * u1 : 0;
-      sn = nsn > 0 ? mx / ((smv - mx) / (double)(nsn - 1)) : (double)NAN;
+    if constexpr (ALG == ALG_ZNCC) {
+      if (rf) {
+        Group<64>::steps([&](auto mv) {
+          const float o1 = mv(-INFINITY, t1), o2 = mv(-INFINITY, t2);
+          t2 = fmaxf(fminf(t1, o1), fmaxf(t2, o2));
+          t1 = fmaxf(t1, o1);
+          sab += mv(0.0, sab);
+        });
+        t1 = bcast63(t1);
+        t2 = bcast63(t2);
+        sab = bcast63(sab);
+        rbad = __ballot(rbad) != 0;
+      }
     }
+    double sn = 0.0;
+    const int u0 = shi0 - lo0 + 1, u1 = shi1 - lo1 + 1;
+    const int nsn = (u0 > 0 && u1 > 0) ? u0 * u1 : 0;
+    if (want_sn) sn = nsn > 0 ? mx / ((smv - mx) / (double)(nsn - 1)) : (double)NAN;
 
     // ---- Gaussian subpixel + write (piv_ops.jl:249-326, 236-243): lanes 0-8 hold
     // the 3x3 neighbourhood, as in the fused kernel
@@ -961,10 +1015,28 @@ __global__ __launch_bounds__((64 * WvOcc<NQ>::WAVES), (WvOcc<NQ>::OCC)) void piv
         r1 = (double)(pk1 + 1) - (double)(sm1 + 1);
         if (interior) {
           const double lmx = lfit[0], up = lfit[1], dn = lfit[2], lf = lfit[3], rt = lfit[4];
-          const double q0 = (up - dn) / (2 * up - 4 * lmx + 2 * dn);
-          const double q1 = (lf - rt) / (2 * lf - 4 * lmx + 2 * rt);
+          const double D0 = 2 * up - 4 * lmx + 2 * dn, D1 = 2 * lf - 4 * lmx + 2 * rt;
+          const double q0 = (up - dn) / D0;
+          const double q1 = (lf - rt) / D1;
           r0 = ((double)(pk0 + 1) + q0) - (double)(sm0 + 1);
           r1 = ((double)(pk1 + 1) + q1) - (double)(sm1 + 1);
+          if constexpr (ALG == ALG_ZNCC) {
+            // an error e in the logs moves q by at most e (2 + 8 |q|) / |D|
+            if (rf)
+              rbad |= !(__builtin_fabs(D0) >= kRefineFit * (1.0 + 4.0 * __builtin_fabs(q0))) ||
+                      !(__builtin_fabs(D1) >= kRefineFit * (1.0 + 4.0 * __builtin_fabs(q1)));
+          }
+        }
+      }
+      if constexpr (ALG == ALG_ZNCC) {
+        if (rf && !skip) {
+          bool f = rbad;
+          f |= !(__builtin_fabsf(t1) >= kRefinePeak) || !((double)t1 - (double)t2 >= kRefinePeak);
+          if (want_sn && nsn > 1) f |= !(__builtin_fabs((smv - mx) / (double)(nsn - 1)) >= kRefineSnK * sab / nsn);
+          if (f) {
+            const int32_t pos = atomicAdd(refine, 1);
+            refine[kRefineHead + pos] = iag;
+          }
         }
       }
       if (J.peaks) {

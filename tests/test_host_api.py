@@ -159,18 +159,22 @@ def test_vf_size_and_param_errors_without_gpu():
 
 
 def test_transform_form_follows_the_precision_mode():
-    """mqpiv_transform_form (host only): AUTO keeps FP64 for ZNCC with SN and
-    masked NSQECC, the split-precision form where it takes the job (interSize
-    <= 32 with search size <= 64, or <= 64 with margin <= 16), FP32 transforms
-    beyond; explicit modes are honoured or refused."""
+    """mqpiv_transform_form (host only): AUTO keeps FP64 for masked NSQECC, runs
+    ZNCC with SN in the split-precision form with FP64 recomputation of the IAs
+    it flags where that form takes the job (FP64 beyond), the split-precision
+    form elsewhere where it takes the job (interSize <= 32 with search size
+    <= 64, or <= 64 with margin <= 16), FP32 transforms beyond; explicit modes
+    are honoured or refused."""
     from multi_quickpiv_amd.piv import transform_form
 
     def form(alg, isz, sm, sn=True, mode="auto", masked=False, shape=(2048, 2048)):
         p = setPIVParameters(corr_alg=alg, interSize=isz, searchMargin=sm, step=16, computeSN=sn)
         return transform_form(p, shape, mode, masked=masked)
 
-    assert form("zncc", 32, 16) == ("f64", 64)
-    assert form("zncc", 64, 16) == ("f64", 96)
+    assert form("zncc", 32, 16) == ("split-f16+f64", 64)
+    assert form("zncc", 64, 16) == ("split-f16+f64", 65)
+    assert form("zncc", 64, 32) == ("f64", 128)  # margin 32: beyond the split form
+    assert form("zncc", 32, 16, mode="mfma") == ("split-f16", 64)
     assert form("nsqecc", 32, 16) == ("split-f16", 64)
     assert form("nsqecc", 64, 16) == ("split-f16", 65)  # 2 x 2 sub-blocks
     assert form("zncc", 32, 16, sn=False) == ("split-f16", 64)
