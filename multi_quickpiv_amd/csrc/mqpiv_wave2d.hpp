@@ -74,6 +74,11 @@ __device__ __forceinline__ void wv_sb() {
 //   error of 2e-5;
 // * every value of the peak window finite, and every window energy there
 //   keeping 24 bits of its non-negative terms.
+#ifndef MQPIV_WV_BAND
+#define MQPIV_WV_BAND 16
+#endif
+constexpr int kWvBand = MQPIV_WV_BAND;  // vector-field rows per band of the visiting order
+
 constexpr double kRefineSnK = 0.02;
 constexpr float kRefinePeak = 1e-4f;
 constexpr double kRefineFit = 0.02;
@@ -278,13 +283,24 @@ __global__ __launch_bounds__((64 * WvOcc<NQ>::WAVES), (WvOcc<NQ>::OCC)) void piv
       J = jobs[jb];
       ia32 = iag - (int32_t)prefix[jb];
     }
-    const int64_t ia = ia32;
     const int sm0 = J.sm[0], sm1 = J.sm[1];
     const int is0 = J.is[0], is1 = J.is[1];
     const int ns0 = 2 * sm0 + 1, ns1 = 2 * sm1 + 1;
     const uint32_t vn0 = (uint32_t)J.vf_n[0];
-    const int32_t v1 = (int32_t)((uint32_t)ia32 / vn0);
-    const int32_t v0 = ia32 - v1 * (int32_t)vn0;
+    // visiting order: bands of kWvBand vector-field rows, each walked column by
+    // column, so that the IAs in flight on an XCD cover a compact patch of the
+    // image whose overlapping search regions its L2 holds
+    int32_t v0, v1;
+    {
+      const uint32_t bw = (uint32_t)kWvBand * (uint32_t)J.vf_n[1];
+      const uint32_t band = (uint32_t)ia32 / bw;
+      const uint32_t r = (uint32_t)ia32 - band * bw;
+      const uint32_t b0 = band * (uint32_t)kWvBand;
+      const uint32_t h = vn0 - b0 < (uint32_t)kWvBand ? vn0 - b0 : (uint32_t)kWvBand;
+      v1 = (int32_t)(r / h);
+      v0 = (int32_t)(b0 + (r - (uint32_t)v1 * h));
+    }
+    const int64_t ia = (int64_t)v0 + (int64_t)vn0 * v1;  // the IA's index in the vector field
     const Geo g0 = geo_of(v0, is0, sm0, J.st[0], (int32_t)J.dims[0]);
     const Geo g1 = geo_of(v1, is1, sm1, J.st[1], (int32_t)J.dims[1]);
     const int dt = J.in_dtype;
@@ -1035,7 +1051,7 @@ __global__ __launch_bounds__((64 * WvOcc<NQ>::WAVES), (WvOcc<NQ>::OCC)) void piv
           if (want_sn && nsn > 1) f |= !(__builtin_fabs((smv - mx) / (double)(nsn - 1)) >= kRefineSnK * sab / nsn);
           if (f) {
             const int32_t pos = atomicAdd(refine, 1);
-            refine[kRefineHead + pos] = iag;
+            refine[kRefineHead + pos] = iag - ia32 + (int32_t)ia;
           }
         }
       }
