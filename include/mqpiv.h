@@ -239,9 +239,13 @@ int mqpiv_profile_end(mqpiv_ctx* ctx, double* kernel_ms, int32_t cap, int32_t* n
  * runs one PIV call at a time on the caller's thread).  A worker thread admits
  * submitted pairs, oldest first, into one of n_slots execution slots as soon
  * as a slot is free and the pair's device footprint fits in the
- * hbm_budget_bytes pool (up to max_batch pairs per slot).  A batch holds pairs
- * of one dimensionality; a 3-D volume runs alone and one slot at most runs a
- * volume, so 2-D pairs may complete before a volume submitted ahead of them.
+ * hbm_budget_bytes pool (up to max_batch pairs per slot).  The device runs the
+ * pairs in submission order: a large pair (a 3-D volume, or a 2-D pair of more
+ * than 32,768 IAs) runs alone, after the batches admitted before it and before
+ * those admitted after it; batches of consecutive small pairs of one
+ * dimensionality may run beside each other.  The order is kept on the device
+ * (a slot's stream waits for the slots it follows), so a batch starts as soon
+ * as the ones ahead of it end.
  * The footprint is the pair's FP64 integral arrays plus, for host requests, its
  * staged inputs and outputs, each rounded to 256 B; every slot also keeps its
  * own context (3-D transform scratch, at most MQPIV_3D_CHUNK_BYTES, 2 GiB by

@@ -99,11 +99,19 @@ static void noalloc_setup(int device) {
   else g_scratch_err[device] = std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e);
 }
 
+// Smallest sizes the context's grow-only refine/transform buffer and job tables
+// take (64 KiB of tables: about 300 jobs).
+static constexpr size_t kMinGrowBytes = (size_t)1 << 20;
+static constexpr size_t kMinTableBytes = (size_t)64 << 10;
+
 // The context's global transform buffers (gz_scratch_2d bytes): a new
 // allocation gets its slot flags cleared (launches leave them clear).
 static int ensure_gz(mqpiv_ctx* c, size_t need, hipStream_t s) {
   const size_t cap0 = c->gz_cap;
-  int rc = ensure(c, &c->d_gz, &c->gz_cap, need, s);
+  // at least 1 MiB (a refine list of 262,080 IAs), so that a context that
+  // serves requests of growing size rarely reallocates (each regrow waits for
+  // the stream)
+  int rc = ensure(c, &c->d_gz, &c->gz_cap, std::max(need, kMinGrowBytes), s);
   if (rc) return rc;
   if (c->gz_cap != cap0) HIP_OR_FAIL(c, hipMemsetAsync(c->d_gz, 0, kGzFlagBytes, s));
   return MQPIV_OK;
@@ -954,7 +962,7 @@ int batch_device_ws(mqpiv_ctx* c, const mqpiv_request* reqs, int32_t n, void* st
       rc = ensure(c, &c->d_bsat, &c->bsat_cap, P.sat_total + P.sat_tmp + 256, s);
       if (rc) return rc;
     }
-    rc = ensure(c, &c->d_table, &c->table_cap, P.table_bytes + 256, s);
+    rc = ensure(c, &c->d_table, &c->table_cap, std::max(P.table_bytes + 256, kMinTableBytes), s);
     if (rc) return rc;
     if (P.gz_bytes) {
       rc = ensure_gz(c, P.gz_bytes, s);
@@ -963,11 +971,12 @@ int batch_device_ws(mqpiv_ctx* c, const mqpiv_request* reqs, int32_t n, void* st
     // pinned staging for the job tables; the previous batch's copy must be done
     if (c->table_done) hipEventSynchronize(c->table_done);
     if (P.table_bytes > c->h_table_cap) {
+      const size_t want = std::max(P.table_bytes, kMinTableBytes);
       if (c->h_table) hipHostFree(c->h_table);
       c->h_table = nullptr;
       c->h_table_cap = 0;
-      HIP_OR_FAIL(c, hipHostMalloc(&c->h_table, P.table_bytes, hipHostMallocDefault));
-      c->h_table_cap = P.table_bytes;
+      HIP_OR_FAIL(c, hipHostMalloc(&c->h_table, want, hipHostMallocDefault));
+      c->h_table_cap = want;
     }
     if (!c->table_done) HIP_OR_FAIL(c, hipEventCreateWithFlags(&c->table_done, hipEventDisableTiming));
     fill_tables(P, (char*)c->d_bsat, (char*)c->h_table, ws);

@@ -45,6 +45,9 @@ struct Slot {
   hipEvent_t done = nullptr;
   std::vector<Req*> reqs;
   bool busy = false;
+  // completion events of the batches this one must follow on the device (taken
+  // at admission from slots still running then)
+  std::vector<hipEvent_t> after;
 };
 
 }  // namespace
@@ -120,6 +123,9 @@ static int launch_slot(mqpiv_server* sv, Slot& sl) {
     }
     if (r.ws_bytes) ws[i] = sv->d_pool + r.d_off + r.io_bytes;
   }
+  // the inputs are staged; the kernels start once the batches ahead are done
+  for (hipEvent_t e : sl.after)
+    if (hipStreamWaitEvent(sl.stream, e, 0) != hipSuccess) return MQPIV_ERR_DEVICE;
   int rc = batch_device_ws(sl.ctx, dev.data(), (int32_t)n, sl.stream, ws.data());
   if (rc) return rc;
   for (size_t i = 0; i < n; ++i) {
@@ -143,6 +149,11 @@ static int launch_slot(mqpiv_server* sv, Slot& sl) {
 
 static void finish(mqpiv_server* sv, Req* r, int status, double t) { mqpiv_srv::finish(*sv, r, status, t); }
 
+// A request that fills the device by itself: a 3-D volume, or a 2-D pair of more
+// than kBigIA IAs (a 2048 x 2048 pair at interSize 16, 16,129 IAs, is not).
+static constexpr int64_t kBigIA = 32768;
+static bool is_big(const Req& r) { return r.req.params.ndims == 3 || r.n_ia > kBigIA; }
+
 static void worker_loop(mqpiv_server* sv) {
   hipSetDevice(sv->device);
   std::unique_lock<std::mutex> lk(sv->mu);
@@ -165,36 +176,41 @@ static void worker_loop(mqpiv_server* sv) {
       for (auto& sl : sv->slots) any |= sl.busy;
       if (!any) break;
     }
-    // admit pending requests into free slots (round-robin), oldest first, while
-    // memory lasts.  A batch holds requests of one dimensionality: a batch
-    // completes with its longest member, so a 2-D pair does not wait for a 3-D
-    // volume admitted beside it (the next free slot takes it instead).  At most
-    // one slot runs 3-D volumes, one volume at a time: two side by side would
-    // each take about twice as long, so a volume waits for the running one while
-    // the 2-D pairs behind it go ahead (in their own arrival order).
+    // admit pending requests into free slots (round-robin) in arrival order,
+    // while memory lasts.  The device runs them in that order as well: a large
+    // request (a 3-D volume, or a 2-D pair of more than kBigIA IAs, which fills
+    // the device by itself) runs alone -- work beside it only takes CUs from it,
+    // and the requests of a batch complete together, so anything sharing its
+    // launch or the device would delay it.  Small requests form batches of the
+    // consecutive small requests of one dimensionality from the front (their
+    // launches would leave most CUs idle alone), which may run beside other
+    // small batches.  The order is kept on the device, not by waiting here: a
+    // batch's stream waits for the completion events of the slots it must
+    // follow (every running slot for a large request, the running large
+    // request for a small batch), so it starts the moment they end.
     const size_t ns = sv->slots.size();
-    bool vol_busy = false;
-    for (auto& s2 : sv->slots) vol_busy |= s2.busy && !s2.reqs.empty() && s2.reqs.front()->req.params.ndims == 3;
     for (size_t k = 0; k < ns; ++k) {
       Slot& sl = sv->slots[(sv->next_slot + k) % ns];
       if (sl.busy || sv->pending.empty()) continue;
-      int want = 0;  // dimensionality of this batch, from its first admissible request
+      const Req& front = *sv->pending.front();
+      const bool big = is_big(front);
+      const int front_nd = front.req.params.ndims;
       for (auto it = sv->pending.begin(); it != sv->pending.end() && (int)sl.reqs.size() < sv->max_batch;) {
         Req* r = *it;
-        const int nd = r->req.params.ndims;
-        if ((vol_busy && nd == 3) || (want && nd != want)) {
-          ++it;
-          continue;
-        }
+        if (!big && (is_big(*r) || r->req.params.ndims != front_nd)) break;
         const size_t off = sv->pool.alloc(r->d_bytes);
         if (off == SIZE_MAX) break;  // wait for memory to free up
         r->d_off = off;
         it = sv->pending.erase(it);
         sl.reqs.push_back(r);
-        want = nd;
-        if (nd == 3) break;  // a volume runs alone
+        if (big) break;  // a large request runs alone
       }
+      sl.after.clear();
+      for (auto& s2 : sv->slots)
+        if (s2.busy && (big || (s2.reqs.size() == 1 && is_big(*s2.reqs.front())))) sl.after.push_back(s2.done);
       if (sl.reqs.empty()) continue;
+      const double t_admit = now_s();
+      for (Req* r : sl.reqs) r->t_admit = t_admit;
       // copies and launches run without the lock so that submitters are not held up
       lk.unlock();
       const int rc = launch_slot(sv, sl);
@@ -216,7 +232,7 @@ static void worker_loop(mqpiv_server* sv) {
     }
     if (progress) continue;
     bool any_busy = false;
-    for (auto& sl : sv->slots) any_busy |= sl.busy;
+    for (auto& s2 : sv->slots) any_busy |= s2.busy;
     if (any_busy) {
       lk.unlock();
       std::this_thread::sleep_for(std::chrono::microseconds(20));

@@ -4,6 +4,7 @@
 #pragma once
 #include <stddef.h>
 #include <stdint.h>
+#include <stdio.h>
 
 #include <iterator>
 #include <map>
@@ -71,7 +72,7 @@ struct Req {
   size_t npx = 0, esz = 0, osz = 0;
   size_t io_bytes = 0, ws_bytes = 0;  // staged inputs/outputs (host requests) + integral arrays
   size_t d_off = SIZE_MAX, d_bytes = 0;
-  double t_submit = 0, t_done = 0;
+  double t_submit = 0, t_admit = 0, t_done = 0;  // t_admit: taken into a slot's batch
   int status = -1;  // -1 pending/running; >= 0 final status
 };
 
@@ -93,6 +94,11 @@ inline void finish(Core& sv, Req* r, int status, double t) {
   if (r->d_off != SIZE_MAX) sv.pool.release(r->d_off, r->d_bytes);
   r->d_off = SIZE_MAX;
   if (status == MQPIV_OK) {
+#ifdef MQPIV_SERVER_TRACE  // per-request latency lines on stderr (a measurement build)
+    fprintf(stderr, "mqpiv-server-trace id=%llu ndims=%d n_ia=%lld submit=%.6f admit_ms=%.3f latency_ms=%.3f\n",
+            (unsigned long long)r->id, (int)r->req.params.ndims, (long long)r->n_ia, r->t_submit,
+            (r->t_admit - r->t_submit) * 1e3, (t - r->t_submit) * 1e3);
+#endif
     sv.lat.push_back(t - r->t_submit);
     sv.ia_done += r->n_ia;
     sv.pairs_done += 1;
