@@ -41,7 +41,7 @@ hipError_t build_sat_pair(const void* img, int dt, const int64_t* dims, int nd, 
 int fft_len_3d(int fp64, int ss);
 size_t scratch_3d_bytes(int fp64, int L, const Job& j);
 hipError_t launch_piv3d(int fp64, int L, const Job& j, void* scratch, const double* sat1_img1, hipStream_t s);
-hipError_t launch_skip_flags_2d(const Job& j, const void* mask_vf, uint8_t* flags, hipStream_t s);
+hipError_t launch_skip_flags(const Job& j, const void* mask_vf, uint8_t* flags, hipStream_t s);
 hipError_t launch_coarse_to_shift(const double* vfc, const int32_t* nc, const int32_t* isc, const int32_t* stc,
                                   const int32_t* nf, const int32_t* isf, const int32_t* stf, int nd,
                                   int64_t n_fine, int32_t* out, hipStream_t s);
@@ -322,6 +322,13 @@ static int prepare_sats(mqpiv_ctx* c, Job& j, hipStream_t s, char* ws = nullptr,
   return MQPIV_OK;
 }
 
+// AUTO's 3-D precision: FP64 transforms where FP32 ones miss the parity bar
+// (ZNCC with SN; masked NSQECC, whose num = sum(F^2) + C2 - 2 C1 cancels near the
+// match), FP32 otherwise.
+static int auto_fp64_3d(const Job& j) {
+  return ((j.alg == ALG_ZNCC && j.compute_sn) || j.alg == ALG_MASK_NSQECC) ? 1 : 0;
+}
+
 // FP64 transform length that recomputes the flagged IAs of a precision-3 job.
 static int refine_len(const Job& j) {
   return fft_len_2d(1, std::max(j.is[0] + 2 * j.sm[0], j.is[1] + 2 * j.sm[1]));
@@ -401,9 +408,9 @@ extern "C" int mqpiv_transform_form(const mqpiv_params* p, const int64_t* dims, 
     const int ss = std::max({j.is[0] + 2 * j.sm[0], j.is[1] + 2 * j.sm[1], j.is[2] + 2 * j.sm[2]});
     int fp64 = 0;
     if (j.precision_mode == MQPIV_FP64) fp64 = 1;
-    else if (j.precision_mode == MQPIV_AUTO) fp64 = (j.alg == ALG_ZNCC && j.compute_sn) ? 1 : 0;
+    else if (j.precision_mode == MQPIV_AUTO) fp64 = auto_fp64_3d(j);
     const int L = fft_len_3d(fp64, ss);
-    if (L < 0 || j.alg == ALG_MASK_NSQECC) return MQPIV_ERR_UNSUPPORTED;
+    if (L < 0) return MQPIV_ERR_UNSUPPORTED;
     *form = fp64;
     *length = L;
     return MQPIV_OK;
@@ -453,13 +460,12 @@ static int launch_job(mqpiv_ctx* c, Job& j, hipStream_t s, char* ws = nullptr, c
     return MQPIV_OK;
   }
   // 3-D
-  if (j.alg == ALG_MASK_NSQECC) return fail(c, MQPIV_ERR_UNSUPPORTED, "masked NSQECC is built for 2-D only");
   int src = prepare_sats(c, j, s, ws);
   if (src) return src;
   const int ss = std::max({j.is[0] + 2 * j.sm[0], j.is[1] + 2 * j.sm[1], j.is[2] + 2 * j.sm[2]});
   int fp64 = 0;
   if (j.precision_mode == MQPIV_FP64) fp64 = 1;
-  else if (j.precision_mode == MQPIV_AUTO) fp64 = (j.alg == ALG_ZNCC && j.compute_sn) ? 1 : 0;
+  else if (j.precision_mode == MQPIV_AUTO) fp64 = auto_fp64_3d(j);
   const int L = fft_len_3d(fp64, ss);
   if (L < 0) return fail(c, MQPIV_ERR_UNSUPPORTED, "3-D search size %d exceeds the largest transform (64)", ss);
   int rc = ensure(c, &c->d_scratch, &c->scratch_bytes, scratch_3d_bytes(fp64, L, j), s);
@@ -595,7 +601,7 @@ static int run_staged(mqpiv_ctx* c, const mqpiv_params* p, const int64_t* dims, 
   j.mask = b_mask ? dm : nullptr;
   j.skip = skip ? (const uint8_t*)dk : nullptr;
   if (mask_vf && (j.has_threshold || j.has_mask_threshold)) {
-    HIP_OR_FAIL(c, launch_skip_flags_2d(j, dmv, (uint8_t*)dk, s));
+    HIP_OR_FAIL(c, launch_skip_flags(j, dmv, (uint8_t*)dk, s));
     j.skip = (const uint8_t*)dk;
   }
   j.vf = dv;
@@ -620,12 +626,11 @@ extern "C" int mqpiv_piv_ex(mqpiv_ctx* c, const mqpiv_params* p, const int64_t* 
                     [&](Job& j, hipStream_t s) { return launch_job(c, j, s); });
 }
 
-// Checks of the double-mask form: the masked measure, 2-D, both masks given.
+// Checks of the double-mask form: the masked measure, both masks given.
 static int check_masks(mqpiv_ctx* c, const mqpiv_params* p, const void* mask_vf, const void* mask_ia) {
   if (!p) return fail(c, MQPIV_ERR_NULL, "params is NULL");
   if (p->corr_alg != MQPIV_MASK_NSQECC)
     return fail(c, MQPIV_ERR_BAD_PARAMS, "the two-mask form runs the masked NSQECC measure");
-  if (p->ndims != 2) return fail(c, MQPIV_ERR_UNSUPPORTED, "masked NSQECC is built for 2-D only");
   if (!mask_vf || !mask_ia) return fail(c, MQPIV_ERR_NULL, "a mask pointer is NULL");
   return MQPIV_OK;
 }
@@ -657,7 +662,7 @@ extern "C" int mqpiv_piv_masks_device(mqpiv_ctx* c, const mqpiv_params* p, const
   if (j.n_ia > 0 && (j.has_threshold || j.has_mask_threshold)) {
     rc = ensure(c, &c->d_flags, &c->flags_cap, (size_t)j.n_ia, s);
     if (rc) return rc;
-    HIP_OR_FAIL(c, launch_skip_flags_2d(j, d_mask_vf, (uint8_t*)c->d_flags, s));
+    HIP_OR_FAIL(c, launch_skip_flags(j, d_mask_vf, (uint8_t*)c->d_flags, s));
     j.skip = (const uint8_t*)c->d_flags;
   }
   return launch_job(c, j, s);

@@ -48,12 +48,14 @@ struct Cfg3 {
 // Per-chunk scratch layout (all per IA of the chunk):
 //   W    : L^3 complex T, [i2|k2|s2][k1][k0]
 //   S    : ns0*ns1*ns2 T  (correlation window, column-major)
+//   S2   : ns0*ns1*ns2 T  (masked NSQECC: the second correlation, mask x G^2)
 //   part : L doubles x 3 (sum F^2, max img1, max mask) per slab
 //   box  : L x ns0*ns1 doubles, [i2][s0 + ns0 s1]: slab i2's sum of G^2 over the
 //          window at (s0, s1) (ZNCC: of G'^2, G' = G - mean) -- ZNCC and NSQECC
 struct Scratch3 {
   void* w;
   void* s;
+  void* s2;  // masked NSQECC only
   double* part;
   double* box;
   int64_t w_stride;  // elements of cpx<T> per IA
@@ -147,10 +149,15 @@ __device__ __forceinline__ Means3 means3(const Job& J, const Geo3& G, const doub
 }
 
 // ---------------------------------------------------------------------------
-// K1: forward 2-D transform of slab i2 of the packed window.
-template <int L, int ALG, typename T, int LPL>
+// K1: forward 2-D transform of slab i2 of the packed window.  Masked NSQECC
+// (mask_nsqecc.jl:141-160) runs the pipeline twice: PASS 0 packs (F m) + i G
+// and writes the slab partials (sum of (F m)^2 / w, the maxima of F and m),
+// PASS 1 packs m + i G^2 (masked_nsqecc_ops.jl:19-42).
+template <int L, int ALG, typename T, int LPL, int PASS = 0>
 __global__ __launch_bounds__((Cfg3<L, T, LPL>::TPL)) void slab_fwd_kernel(Job J, Scratch3 sc, int64_t ia_base,
                                                                           int64_t n_ia, const double* sat1_img1) {
+  constexpr bool kMask = ALG == ALG_MASK_NSQECC;
+  static_assert(PASS == 0 || kMask, "a second pass is the masked measure's");
   using C = Cfg3<L, T, LPL>;
   constexpr int M = C::M, LD = C::LD;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -177,8 +184,8 @@ __global__ __launch_bounds__((Cfg3<L, T, LPL>::TPL)) void slab_fwd_kernel(Job J,
   const bool g_row = g_pl && k0 >= g0.goff && k0 < g0.goff + g0.gsz;
   const int64_t f_base = (g0.tlf + k0) + d0 * g1.tlf + d01 * (g2.tlf + i2);
   const int64_t g_base = (g0.sr + (k0 - g0.goff)) + d0 * (g1.sr - g1.goff) + d01 * (g2.sr + (i2 - g2.goff));
-  const bool want_max = J.has_threshold;
-  double sumF2 = 0.0, mx1 = -INFINITY;
+  const bool want_max = J.has_threshold || J.has_mask_threshold;
+  double sumF2 = 0.0, mx1 = -INFINITY, mx2 = -INFINITY;
   cpx<T> x[M];
   constexpr bool kBox = ALG == ALG_ZNCC || ALG == ALG_NSQECC;
   double gq[kBox ? M : 1];  // G^2 (ZNCC: G'^2) of the lane's samples, 0 outside the data
@@ -194,11 +201,13 @@ __global__ __launch_bounds__((Cfg3<L, T, LPL>::TPL)) void slab_fwd_kernel(Job J,
     using TIN = decltype(tin);
     const gptr<TIN> pf = as_global<TIN>(J.img1) + f_lane;
     const gptr<TIN> pg = as_global<TIN>(J.img2) + g_lane;
-    TIN fv[M], gv[M];
+    const gptr<TIN> pm = as_global<TIN>(kMask ? J.mask : J.img1) + f_lane;
+    TIN fv[M], gv[M], mv[kMask ? M : 1];
 #pragma unroll
     for (int k = 0; k < M; ++k) {
       const int k1 = LPL * k + q;
-      fv[k] = pf[f_at(k1) ? d0 * k1 : 0];
+      if (PASS == 0) fv[k] = pf[f_at(k1) ? d0 * k1 : 0];
+      if constexpr (kMask) mv[k] = pm[f_at(k1) ? d0 * k1 : 0];
       gv[k] = pg[d0 * (g_at(k1) ? k1 : g1.goff)];
     }
 #pragma unroll
@@ -206,19 +215,36 @@ __global__ __launch_bounds__((Cfg3<L, T, LPL>::TPL)) void slab_fwd_kernel(Job J,
       const int k1 = LPL * k + q;
       double a = 0.0, g = 0.0;
       if (f_at(k1)) {
-        const TIN f = fv[k];
-        if (want_max) mx1 = fmax(mx1, (double)f);
-        if constexpr (ALG == ALG_ZNCC)
-          a = std::is_same<TIN, float>::value ? (double)((float)f - mu.f32) : (double)f - mu.f64;
-        else
-          a = (double)f;
-        if constexpr (ALG != ALG_FFTCC) sumF2 += a * a;
+        if constexpr (kMask) {
+          const double m = (double)mv[k];
+          if constexpr (PASS == 0) {
+            const TIN f = fv[k];
+            if (want_max) {
+              mx1 = fmax(mx1, (double)f);
+              mx2 = fmax(mx2, m);
+            }
+            a = (double)f * m;  // masked_nsqecc_ops.jl:19-25
+            const double wi = m > 0 ? m : 1.0;
+            sumF2 += a * a / wi;  // mask_nsqecc.jl:144-149
+          } else {
+            a = m;
+          }
+        } else {
+          const TIN f = fv[k];
+          if (want_max) mx1 = fmax(mx1, (double)f);
+          if constexpr (ALG == ALG_ZNCC)
+            a = std::is_same<TIN, float>::value ? (double)((float)f - mu.f32) : (double)f - mu.f64;
+          else
+            a = (double)f;
+          if constexpr (ALG != ALG_FFTCC) sumF2 += a * a;
+        }
       }
       if (g_at(k1)) {
         if constexpr (ALG == ALG_ZNCC)
           g = std::is_same<TIN, float>::value ? (double)((float)gv[k] - mu.g32) : (double)gv[k] - mu.g64;
         else
           g = (double)gv[k];
+        if constexpr (PASS == 1) g = g * g;  // masked_nsqecc_ops.jl:36-42
       }
       if constexpr (kBox) gq[k] = g * g;
       x[k] = cpx<T>{(T)a, (T)g};
@@ -232,12 +258,16 @@ __global__ __launch_bounds__((Cfg3<L, T, LPL>::TPL)) void slab_fwd_kernel(Job J,
     if (dt == DT_F32) load(float{});
     else load(double{});
   }
-  sumF2 = Group<C::TPL>::reduce(sumF2, OpAdd{}, red, t);
-  mx1 = Group<C::TPL>::reduce(mx1, OpMax{}, red, t);
-  if (t == 0) {
-    double* part = sc.part + il * (3 * L);
-    part[i2] = sumF2;
-    part[L + i2] = mx1;
+  if constexpr (PASS == 0) {
+    sumF2 = Group<C::TPL>::reduce(sumF2, OpAdd{}, red, t);
+    mx1 = Group<C::TPL>::reduce(mx1, OpMax{}, red, t);
+    if constexpr (kMask) mx2 = Group<C::TPL>::reduce(mx2, OpMax{}, red, t);
+    if (t == 0) {
+      double* part = sc.part + il * (3 * L);
+      part[i2] = sumF2;
+      part[L + i2] = mx1;
+      part[2 * L + i2] = mx2;
+    }
   }
 
   // forward along dim 2 (registers), then dim 1 through LDS
@@ -455,13 +485,32 @@ __global__ __launch_bounds__(256) void finalize3d_kernel(Job J, Scratch3 sc, int
   T* S = (T*)sc.s + il * sc.s_stride;
 
   // IA constants from the per-slab partials
-  double sumF2 = 0.0, mx1 = -INFINITY;
+  double sumF2 = 0.0, mx1 = -INFINITY, mx2 = -INFINITY;
   const double* part = sc.part + il * (3 * L);
   for (int i = 0; i < L; ++i) {
     sumF2 += part[i];
     mx1 = fmax(mx1, part[L + i]);
+    mx2 = fmax(mx2, part[2 * L + i]);
   }
-  const bool skip = J.skip ? (J.skip[ia] != 0) : (J.has_threshold && mx1 < J.threshold);
+  // (mx2, the mask maximum, is only gathered by the masked measure, which is
+  // the only one the host gives a mask threshold)
+  const bool skip = J.skip ? (J.skip[ia] != 0)
+                           : ((J.has_threshold && mx1 < J.threshold) ||
+                              (J.has_mask_threshold && mx2 < J.mask_threshold));
+
+  if constexpr (ALG == ALG_MASK_NSQECC) {
+    // mask_nsqecc.jl:155 and masked_nsqecc_ops.jl:123-143: C1 = (F m) x G,
+    // C2 = m x G^2, num = sum(F^2) + C2 - 2 C1, den = sqrt|C2| sqrt(sum(F^2))
+    const double sqF = sqrt(sumF2);
+    const T* S2 = (const T*)sc.s2 + il * sc.s_stride;
+    for (int e = t; e < nS; e += NT) {
+      const double c1 = (double)S[e], c2 = (double)S2[e];
+      const double num = (-2.0 * c1 + sumF2) + c2;
+      const double den = sqrt(fabs(c2)) * sqF;
+      S[e] = (T)(den / (num + den));
+    }
+    __syncthreads();
+  }
 
   if constexpr (ALG == ALG_ZNCC || ALG == ALG_NSQECC) {
     // W(s) = sum over the window's planes s2 .. s2 + is2 - 1 of the slabs' window
@@ -604,6 +653,7 @@ template <int L, typename T, int LPL>
 size_t scratch3_per_ia(int alg, int ns0, int ns1, int ns2) {
   size_t w = (size_t)L * L * L * sizeof(cpx<T>);
   size_t s = ((size_t)ns0 * ns1 * ns2 * sizeof(T) + 255) & ~(size_t)255;
+  if (alg == ALG_MASK_NSQECC) s *= 2;  // the second correlation's window
   size_t p = (size_t)3 * L * sizeof(double);
   size_t bx = (alg == ALG_ZNCC || alg == ALG_NSQECC) ? (size_t)L * ns0 * ns1 * sizeof(double) : 0;
   return w + s + p + bx;
@@ -622,6 +672,11 @@ hipError_t launch3d_chunk(const Job& J, void* scratch, int64_t chunk_cap, int64_
   b += (size_t)chunk_cap * L * L * L * sizeof(cpx<T>);
   sc.s = b;
   b += (((size_t)chunk_cap * ns0 * ns1 * ns2 * sizeof(T)) + 255) & ~(size_t)255;
+  sc.s2 = nullptr;
+  if (J.alg == ALG_MASK_NSQECC) {
+    sc.s2 = b;
+    b += (((size_t)chunk_cap * ns0 * ns1 * ns2 * sizeof(T)) + 255) & ~(size_t)255;
+  }
   sc.part = (double*)b;
   b += (size_t)chunk_cap * 3 * L * sizeof(double);
   const bool box = J.alg == ALG_ZNCC || J.alg == ALG_NSQECC;
@@ -633,6 +688,8 @@ hipError_t launch3d_chunk(const Job& J, void* scratch, int64_t chunk_cap, int64_
   allow_full_lds((const void*)slab_fwd_kernel<L, ALG_FFTCC, T, LPL>);
   allow_full_lds((const void*)slab_fwd_kernel<L, ALG_ZNCC, T, LPL>);
   allow_full_lds((const void*)slab_fwd_kernel<L, ALG_NSQECC, T, LPL>);
+  allow_full_lds((const void*)slab_fwd_kernel<L, ALG_MASK_NSQECC, T, LPL, 0>);
+  allow_full_lds((const void*)slab_fwd_kernel<L, ALG_MASK_NSQECC, T, LPL, 1>);
   allow_full_lds((const void*)zline_kernel<L, T, LPL>);
   allow_full_lds((const void*)slab_inv_kernel<L, T, LPL>);
   switch (J.alg) {
@@ -648,6 +705,21 @@ hipError_t launch3d_chunk(const Job& J, void* scratch, int64_t chunk_cap, int64_
       hipLaunchKernelGGL((slab_fwd_kernel<L, ALG_NSQECC, T, LPL>), dim3((unsigned)(n * L)), dim3(C::TPL), lds1, s,
                          J, sc, ia_base, n, sat1_img1);
       break;
+    case ALG_MASK_NSQECC: {
+      // the second correlation first, into S2 (K3 writes the window its
+      // Scratch3 names), then the first, into S
+      Scratch3 sc2 = sc;
+      sc2.s = sc.s2;
+      hipLaunchKernelGGL((slab_fwd_kernel<L, ALG_MASK_NSQECC, T, LPL, 1>), dim3((unsigned)(n * L)), dim3(C::TPL), lds1,
+                         s, J, sc2, ia_base, n, sat1_img1);
+      hipLaunchKernelGGL((zline_kernel<L, T, LPL>), dim3((unsigned)(n * (L / 2 + 1))), dim3(C::NT2), lds2, s, sc2, n,
+                         ns2);
+      hipLaunchKernelGGL((slab_inv_kernel<L, T, LPL>), dim3((unsigned)(n * ns2)), dim3(C::TPL), lds3, s, sc2, n, ns0,
+                         ns1, ns2);
+      hipLaunchKernelGGL((slab_fwd_kernel<L, ALG_MASK_NSQECC, T, LPL, 0>), dim3((unsigned)(n * L)), dim3(C::TPL), lds1,
+                         s, J, sc, ia_base, n, sat1_img1);
+      break;
+    }
     default:
       return hipErrorInvalidValue;
   }
@@ -667,6 +739,10 @@ hipError_t launch3d_chunk(const Job& J, void* scratch, int64_t chunk_cap, int64_
     case ALG_NSQECC:
       hipLaunchKernelGGL((finalize3d_kernel<ALG_NSQECC, T>), dim3((unsigned)n), dim3(256), 0, s, J, sc, ia_base, n,
                          L, sat1_img1);
+      break;
+    case ALG_MASK_NSQECC:
+      hipLaunchKernelGGL((finalize3d_kernel<ALG_MASK_NSQECC, T>), dim3((unsigned)n), dim3(256), 0, s, J, sc, ia_base,
+                         n, L, sat1_img1);
       break;
   }
   return hipGetLastError();

@@ -80,3 +80,69 @@ def test_3d_shear_recovers_displacement(oracle):
     assert abs(VF[0, 1, 1, 1] - (2.3 + 0.235)) < 0.15
     assert abs(VF[1, 1, 1, 1] + 1.1) < 0.15
     assert abs(VF[2, 1, 1, 1] - 0.7) < 0.15
+
+
+def _ellipsoid_mask(shape, n, seed):
+    """1 outside n seeded random ellipsoids (about 20-30% of the volume), 0 inside."""
+    r = np.random.default_rng(seed)
+    g = np.meshgrid(*[np.arange(s, dtype=np.float64) for s in shape], indexing="ij")
+    m = np.ones(shape, dtype=np.float32)
+    for _ in range(n):
+        c = [r.uniform(0, s) for s in shape]
+        ax = [r.uniform(0.1, 0.25) * s for s in shape]
+        inside = sum(((g[d] - c[d]) / ax[d]) ** 2 for d in range(3)) <= 1.0
+        m[inside] = 0.0
+    return m
+
+
+MASK_CASES = [
+    ((40, 36, 32), 8, 4, 8),
+    ((48, 44, 40), (12, 8, 8), (4, 3, 2), (6, 7, 8)),
+    ((64, 64, 48), 16, 8, 12),
+]
+
+
+@pytest.mark.parametrize("shape,isz,sm,st", MASK_CASES)
+@pytest.mark.parametrize("mode", ["auto", "fp64"])
+def test_3d_masked_nsqecc_parity(oracle, shape, isz, sm, st, mode):
+    """Masked NSQECC on volumes (mask_nsqecc.jl:126-160 handles N = 3): the two
+    correlations (F m) x G and m x G^2 through the 3-D pipeline, FP64 in AUTO."""
+    a, b = _vol(shape, seed=41)
+    m = _ellipsoid_mask(shape, 5, 42)
+    p = setPIVParameters(corr_alg="nsqecc", interSize=isz, searchMargin=sm, step=st, computeSN=True)
+    print(f"case 3d masked mode={mode} shape={shape} is={isz} sm={sm} st={st}", flush=True)
+    VF, SN, PK = PIV(a, b, m, p, return_peaks=True, mode=mode)
+    op = oracle.make_params("mask_nsqecc", 3, isz, sm, st, True, np.float32)
+    VFo, SNo, aux = oracle.piv(a, b, op, mask=m, mode=oracle.DIRECT, nthreads=8, want_aux=True)
+    rep = compare(VF, SN, PK, VFo, SNo, aux)
+    print(rep, flush=True)
+    assert rep.ok(), rep
+
+
+def test_3d_masked_thresholds_and_double_mask(oracle):
+    """3-D masked PIV with a threshold and a mask threshold (IAs skipped on image 1
+    or on the mask), single-mask and double-mask forms (piv_ops.jl:148-157,
+    multi_quickPIV.jl:155-213): the double-mask skip flags come from the device
+    kernel over mask_VF, the correlation uses mask_IA."""
+    shape = (40, 36, 32)
+    a, b = _vol(shape, seed=43)
+    m_ia = _ellipsoid_mask(shape, 4, 44)
+    m_vf = _ellipsoid_mask(shape, 6, 45)
+    thr = float(np.quantile(a, 0.995))
+    p = setPIVParameters(corr_alg="nsqecc", interSize=8, searchMargin=4, step=8, computeSN=True,
+                         threshold=thr, mask_threshold=0.5)
+    op = oracle.make_params("mask_nsqecc", 3, 8, 4, 8, True, np.float32, thr, 0.5)
+    VF, SN, PK = PIV(a, b, m_ia, p, return_peaks=True)
+    VFo, SNo, aux = oracle.piv(a, b, op, mask=m_ia, mode=oracle.DIRECT, nthreads=8, want_aux=True)
+    rep = compare(VF, SN, PK, VFo, SNo, aux)
+    print("single", rep, flush=True)
+    skipped = (VFo == 0).all(axis=0) & (SNo == 0)
+    assert skipped.any() and (~skipped).any()
+    assert rep.ok(), rep
+    VF2, SN2, PK2 = PIV(a, b, m_vf, m_ia, p, return_peaks=True)
+    flags = oracle.skip_flags(a, op, mask=m_vf)
+    VFo2, SNo2, aux2 = oracle.piv(a, b, op, mask=m_ia, skip_mask=flags, mode=oracle.DIRECT, nthreads=8,
+                                  want_aux=True)
+    rep2 = compare(VF2, SN2, PK2, VFo2, SNo2, aux2)
+    print("double", rep2, flush=True)
+    assert rep2.ok(), rep2

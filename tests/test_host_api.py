@@ -185,6 +185,7 @@ def test_transform_form_follows_the_precision_mode():
     assert form("nsqecc", 32, 16, mode="fp64") == ("f64", 64)
     assert form("nsqecc", 64, 16, mode="mfma") == ("split-f16", 65)
     assert form("nsqecc", 32, 16, shape=(64, 64, 64)) == ("f32", 64)  # 3-D: FP32 transforms
+    assert form("nsqecc", 16, 8, masked=True, shape=(64, 64, 64)) == ("f64", 32)  # 3-D masked: FP64
     with pytest.raises(RuntimeError):
         form("nsqecc", 64, 32, mode="mfma")
     with pytest.raises(RuntimeError):

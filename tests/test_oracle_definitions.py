@@ -131,6 +131,75 @@ def test_masked_nsqecc_matches_definition(oracle, mode, kind):
         assert _masked_err(a, b, m, lin, mode) < RTOL, lin
 
 
+def _windows_nd(a, b, m, p, lin):
+    """_windows for any ndims (3-D volumes)."""
+    nd = a.ndim
+    co = O.coords(p, a.shape, lin)
+    tlf, sr_tlf, sr_brb, off = co[0] - 1, co[2] - 1, co[3] - 1, co[4]
+    isz = tuple(p.is_[d] for d in range(nd))
+    ss = tuple(isz[d] + 2 * p.sm[d] for d in range(nd))
+    ia = tuple(slice(tlf[d], tlf[d] + isz[d]) for d in range(nd))
+    src = b[tuple(slice(sr_tlf[d], sr_brb[d] + 1) for d in range(nd))]
+    G = np.zeros(ss)
+    G[tuple(slice(off[d], off[d] + src.shape[d]) for d in range(nd))] = src
+    return a[ia], m[ia], G
+
+
+def _masked_def_nd(F, M, G, sm):
+    ns = tuple(2 * s + 1 for s in sm)
+    sF = np.sum(M * F * F)
+    S = np.empty(ns)
+    for s in np.ndindex(*ns):
+        Gw = G[tuple(slice(s[d], s[d] + F.shape[d]) for d in range(F.ndim))]
+        num = np.sum(M * (F - Gw) ** 2)
+        den = np.sqrt(np.sum(M * Gw * Gw)) * np.sqrt(sF)
+        S[s] = den / (den + num)
+    return S
+
+
+# 3-D: a 20 x 18 x 16 volume at interSize (6, 5, 4), margin (2, 2, 1), step
+# (5, 4, 4); IAs of the first (clipped) corner, the interior and the last corner
+_P3 = ("mask_nsqecc", 3, (6, 5, 4), (2, 2, 1), (5, 4, 4))
+
+
+@pytest.mark.parametrize("mode", [O.DIRECT, O.FFT])
+@pytest.mark.parametrize("kind", ["binary", "fractional"])
+def test_masked_nsqecc_3d_matches_definition(oracle, mode, kind):
+    r = np.random.default_rng(21)
+    shape = (20, 18, 16)
+    a = 0.1 + 0.02 * r.standard_normal(shape)
+    idx = tuple(r.integers(0, s, 60) for s in shape)
+    a[idx] += r.uniform(0.5, 1.0, 60)
+    b = np.roll(a, (1, -2, 1), axis=(0, 1, 2)) + 0.02 * r.standard_normal(shape)
+    m = (r.random(shape) > 0.3).astype(np.float64)
+    if kind == "fractional":
+        m = r.random(shape) * (r.random(shape) > 0.2)
+    p = O.make_params(*_P3)
+    n = int(np.prod(O.vf_size(p, shape)))
+    for lin in (1, n // 2, n):
+        F, M, G = _windows_nd(a, b, m, p, lin)
+        ref = _masked_def_nd(F, M, G, tuple(p.sm[d] for d in range(3)))
+        got, _ = O.ia_similarity(a, b, p, lin, mask=m, mode=mode)
+        assert got.shape == ref.shape
+        assert float(np.max(np.abs(got - ref) / np.abs(ref))) < RTOL, lin
+
+
+@pytest.mark.parametrize("plant", ["PLANT_MASK_C1_UNMASKED", "PLANT_MASK_C2_BOXSUM", "PLANT_MASK_NO_W"])
+def test_masked_3d_definition_check_catches_planted_mistake(oracle, plant):
+    r = np.random.default_rng(22)
+    shape = (20, 18, 16)
+    a = 0.1 + 0.02 * r.standard_normal(shape)
+    b = np.roll(a, (1, -2, 1), axis=(0, 1, 2)) + 0.02 * r.standard_normal(shape)
+    m = r.random(shape) * (r.random(shape) > 0.2)
+    p = O.make_params(*_P3)
+    lin = int(np.prod(O.vf_size(p, shape))) // 2
+    F, M, G = _windows_nd(a, b, m, p, lin)
+    ref = _masked_def_nd(F, M, G, tuple(p.sm[d] for d in range(3)))
+    with planted(getattr(O, plant)):
+        got, _ = O.ia_similarity(a, b, p, lin, mask=m, mode=O.DIRECT)
+    assert float(np.max(np.abs(got - ref) / np.abs(ref))) > 1e-3
+
+
 def test_masked_nsqecc_ignores_image1_where_mask_is_zero(oracle):
     a, b = _pair((48, 44), 14)
     m = _binary_mask(a.shape, 15)
