@@ -508,6 +508,9 @@ __device__ __forceinline__ double sat_box(const double* __restrict__ S, int64_t 
 #ifndef MQPIV_OCC_CAP
 #define MQPIV_OCC_CAP 8
 #endif
+#ifndef MQPIV_ZNCC96_OCC
+#define MQPIV_ZNCC96_OCC 5
+#endif
 template <int L, int ALG, typename T, int LPL>
 constexpr int min_waves_2d() {
   // global transform buffer: one IA's block per CU, whose waves all fit the
@@ -519,7 +522,10 @@ constexpr int min_waves_2d() {
   constexpr size_t per_block = (z + c1 + kRedBytes) * C::IPB;
   constexpr int blocks = (int)((160 * 1024) / per_block);
   constexpr int waves = blocks * (C::NT / 64) / 4;
-  return waves < 1 ? 1 : (waves > MQPIV_OCC_CAP ? MQPIV_OCC_CAP : waves);
+  // FP32 ZNCC at length 96 keeps its means and window-sum state beside 12
+  // values a lane: at the LDS-allowed 6 waves per SIMD (80 VGPRs) it spills
+  constexpr int cap = (L == 96 && ALG == ALG_ZNCC && std::is_same<T, float>::value) ? MQPIV_ZNCC96_OCC : MQPIV_OCC_CAP;
+  return waves < 1 ? 1 : (waves > cap ? cap : waves);
 }
 
 // One IA per block-slot.  SH: search regions shifted per IA (Job::shift,
