@@ -335,14 +335,16 @@ def volume_bench(mode: str, steps: int, cpu: bool, nthreads: int) -> dict:
     return out
 
 
-def batched_bench(mode: str, steps: int, seed: int = 4000) -> dict:
-    """Batched uneven masked NSQECC (one graph launch): 16 FP32 particle pairs
-    with sides drawn from the multiples of 256 in [512, 4096], uniform shifts
-    U[-6, 6]^2, a 20-ellipse mask (about 30% excluded) per pair, interSize
-    drawn from {16, 32, 64} with searchMargin = step = interSize / 2.  The
-    whole set runs as one captured HIP graph (integral-array-free masked
-    measure: one grouped kernel launch per precision / transform length);
-    the plain path runs the same pairs one call after another."""
+def batched_bench(mode: str, steps: int, seed: int = 4000, n_pairs: int = 16, side_range=(512, 4096),
+                  sizes=(16, 32, 64)) -> dict:
+    """Batched uneven masked NSQECC (one graph launch): n_pairs FP32 particle
+    pairs with sides drawn from the multiples of 256 (of 64 below 512) in
+    `side_range`, uniform shifts U[-6, 6]^2, a 20-ellipse mask (about 30% excluded)
+    per pair, interSize drawn from `sizes` with searchMargin = step =
+    interSize / 2.  The whole set runs as one captured HIP graph
+    (integral-array-free masked measure: one grouped kernel launch per
+    precision / transform length); the plain path runs the same pairs one call
+    after another."""
     import torch
 
     from multi_quickpiv_amd import _lib, setPIVParameters, synth
@@ -351,9 +353,10 @@ def batched_bench(mode: str, steps: int, seed: int = 4000) -> dict:
     rng = np.random.default_rng(seed)
     reqs = []
     sides = []
-    for i in range(16):
-        S = int(rng.choice(np.arange(512, 4097, 256)))
-        isz = int(rng.choice([16, 32, 64]))
+    grid = 256 if side_range[0] >= 512 else 64
+    for i in range(n_pairs):
+        S = int(rng.choice(np.arange(side_range[0], side_range[1] + 1, grid)))
+        isz = int(rng.choice(list(sizes)))
         d = tuple(rng.uniform(-6, 6, size=2))
         a, b = synth.particle_pair((S, S), synth.n_particles((S, S), 0.05), 1.2, synth.uniform(d), seed + i,
                                    device="cuda", dtype=torch.float32)
@@ -385,8 +388,9 @@ def batched_bench(mode: str, steps: int, seed: int = 4000) -> dict:
     torch.cuda.synchronize()
     t_plain = (time.perf_counter() - t0) / steps
     g.close()
-    return {"workload": "16 masked NSQECC FP32 pairs, sides 512-4096 (multiples of 256), interSize {16,32,64}, "
-                        "searchMargin = step = interSize/2, 20-ellipse masks (C4)",
+    return {"workload": f"{n_pairs} masked NSQECC FP32 pairs, sides {side_range[0]}-{side_range[1]} (multiples of {grid}), "
+                        f"interSize {{{','.join(map(str, sizes))}}}, searchMargin = step = interSize/2, 20-ellipse masks"
+                        + (" (C4)" if (n_pairs, tuple(side_range), tuple(sizes)) == (16, (512, 4096), (16, 32, 64)) else ""),
             "pairs": [{"side": S, "interSize": i} for S, i in sides], "ia_total": n_ia,
             "graph": {"ms_per_set": round(t_graph * 1e3, 3), "ia_per_s": round(n_ia / t_graph, 1)},
             "plain": {"ms_per_set": round(t_plain * 1e3, 3), "ia_per_s": round(n_ia / t_plain, 1)},
@@ -622,6 +626,20 @@ def main() -> int:
         res["mfma_over_fp32"] = round(res["fp32"]["kernel_ms"] / res["mfma"]["kernel_ms"], 3)
         forms_is32[f'{j["alg"]}{j["isz"]}'] = res
 
+    # the batched split-precision form against the plain per-frame FP32 path
+    # at interSize 32: the step's IA-32 runs (one batched call, kernel time)
+    # and the same runs as plain FP32 calls (one call each, wall time per call)
+    i32 = [k for k, j in enumerate(jobs) if j["isz"] == 32]
+    n32 = sum(jobs[k]["n"] for k in i32)
+    rate_b = n32 / (sum(kern_ms[k] for k in i32) * 1e-3)
+    rate_p = n32 / sum(jobs[k]["n"] / forms_is32[f'{jobs[k]["alg"]}{jobs[k]["isz"]}']["fp32"]["ia_per_s_per_call"]
+                       for k in i32)
+    north_star = {"what": "IA/s of the batched split-precision form (the step's interSize-32 runs, ZNCC+SN "
+                          "with its FP64 recomputation and NSQECC) over the plain per-frame FP32 path (mode "
+                          "fp32, one call per run) on the 2048^2 pair",
+                  "batched_split_ia_per_s": round(rate_b, 1), "plain_fp32_ia_per_s": round(rate_p, 1),
+                  "ratio": round(rate_b / rate_p, 3), "target": 5.0}
+
     # roofline of the dominant run (its launch: for "split-f16+f64" the
     # split-precision kernel, then the FP64 kernel over the IAs it flagged).
     # The split-precision form is bound by the matrix cores: its algorithmic
@@ -658,9 +676,11 @@ def main() -> int:
     if not args.no_3d:
         volume = volume_bench(args.mode, 3, not args.no_cpu, nthreads)
 
-    batched = None
+    batched = batched_small = None
     if not args.no_batched:
         batched = batched_bench(args.mode, 3)
+        # where batching pays: many small pairs, whose launches alone leave most CUs idle
+        batched_small = batched_bench(args.mode, 10, seed=4500, n_pairs=64, side_range=(128, 512), sizes=(16, 32))
 
     serving = None
     if not args.no_serve:
@@ -709,9 +729,11 @@ def main() -> int:
                      "ms_per_step": round(wall_calls * 1e3 / args.steps, 4),
                      "what": "the same four runs as four separate PIV calls per step"},
         "transform_forms": forms_is32,
+        "north_star": north_star,
         "outputs_finite": finite,
         "volume_3d": volume,
         "batched_masked": batched,
+        "batched_masked_small": batched_small,
         "serving": serving,
         "cpu_baseline": cpu,
     }
