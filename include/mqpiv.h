@@ -75,7 +75,8 @@ extern "C" {
                               ZNCC with SN runs the split-precision form where it takes the
                               job, and the FP64 transforms recompute the IAs whose SN, peak
                               or subpixel fit it cannot settle to the FP64 results' bar
-                              (form 3 of mqpiv_transform_form)                                */
+                              (form 3 of mqpiv_transform_form); 2-D masked NSQECC runs FP32
+                              transforms with the same recomputation (form 4)                */
 
 /* The fields of PIVParameters (src/piv_parameters.jl:29-46) that the device
  * needs, already resolved: step = interSize - overlap when the reference's
@@ -108,8 +109,10 @@ int mqpiv_vf_size(const mqpiv_params* p, const int64_t* dims, int64_t* vf_out3, 
 
 /* The transform form a call with these parameters and dims runs: *form = 0
  * (FP32 transforms), 1 (FP64 transforms), 2 (the split-precision matrix-core
- * form) or 3 (that form, then FP64 transforms over the IAs it flags; ZNCC with
- * SN in MQPIV_AUTO), *length = the transform length (2-D; for the split form
+ * form), 3 (that form, then FP64 transforms over the IAs it flags; ZNCC with
+ * SN in MQPIV_AUTO) or 4 (FP32 transforms, then FP64 transforms of the same
+ * length over the IAs they flag; 2-D masked NSQECC in MQPIV_AUTO), *length =
+ * the transform length (2-D; for the split form
  * 64, or 65 for IAs taken in 2 x 2 sub-blocks of length-64 products).  Host-only; no device
  * work.  Same errors as the call itself would report for an unsupported job.
  * No counterpart in the reference (its transforms are always FFTW at

@@ -369,6 +369,13 @@ static int choose_2d(mqpiv_ctx* c, const Job& j, int* fp64_out, int* L_out) {
   } else if (j.precision_mode == MQPIV_AUTO) {
     bool want = j.alg == ALG_MASK_NSQECC || (j.alg == ALG_ZNCC && j.compute_sn);
     fp64 = want && fits(1);
+    // masked NSQECC: FP32 transforms, the IAs they cannot settle recomputed by
+    // the FP64 transforms of the same length (precision 4)
+    if (fp64 && j.alg == ALG_MASK_NSQECC && !j.shift && fits(0) && fft_len_2d(0, ss) == fft_len_2d(1, ss)) {
+      *fp64_out = 4;
+      *L_out = fft_len_2d(0, ss);
+      return MQPIV_OK;
+    }
     // where FP32-level transforms meet the bar, the split-precision matrix-core
     // form when it takes the job (it runs faster than the FP32 transforms); for
     // ZNCC with SN, that form with the IAs it flags recomputed by the FP64
@@ -454,7 +461,7 @@ static int launch_job(mqpiv_ctx* c, Job& j, hipStream_t s, char* ws = nullptr, c
     {
       KernelTimer kt(c, s);
       e = launch_piv2d(fp64, L, j.alg, j, nullptr, nullptr, 1, j.n_ia, ns0, ns1, ss0, gz, s,
-                       fp64 == 3 ? refine_len(j) : 0);
+                       fp64 >= 3 ? refine_len(j) : 0);
     }
     if (e != hipSuccess) return fail(c, MQPIV_ERR_DEVICE, "2-D launch failed: %s", hipGetErrorString(e));
     return MQPIV_OK;
@@ -514,7 +521,11 @@ extern "C" int mqpiv_piv_device_noalloc(mqpiv_ctx* c, const mqpiv_params* p, con
     fp64 = 1;
     L = refine_len(j);
   }
-  if (fp64 != 3 && gz_scratch_2d(fp64, L, j.n_ia))
+  if (fp64 == 4 && L == 128)  // the FP64 recomputation needs global transform buffers there
+    return fail(c, MQPIV_ERR_UNSUPPORTED,
+                "the allocation-free entry point does not run FP64 transforms at search size 128 (global transform buffers)");
+  if (fp64 == 4 && gz_scratch_2d(4, L, j.n_ia) > MQPIV_NOALLOC_SCRATCH_BYTES) fp64 = 1;
+  if (fp64 < 3 && gz_scratch_2d(fp64, L, j.n_ia))
     return fail(c, MQPIV_ERR_UNSUPPORTED,
                 "the allocation-free entry point does not run FP64 transforms at search size 128 (global transform buffers)");
   void* ptr = nullptr;
@@ -531,7 +542,7 @@ extern "C" int mqpiv_piv_device_noalloc(mqpiv_ctx* c, const mqpiv_params* p, con
                 (unsigned long long)MQPIV_NOALLOC_SCRATCH_BYTES);
   hipStream_t s = stream ? (hipStream_t)stream : c->stream;
   const int form[2] = {fp64, L};
-  return launch_job(c, j, s, need ? (char*)ptr : nullptr, form, fp64 == 3 ? (char*)ptr + need : nullptr);
+  return launch_job(c, j, s, need ? (char*)ptr : nullptr, form, fp64 >= 3 ? (char*)ptr + need : nullptr);
 }
 
 extern "C" int mqpiv_piv(mqpiv_ctx* c, const mqpiv_params* p, const int64_t* dims, int32_t in_dtype,
@@ -859,7 +870,7 @@ static int plan_batch(mqpiv_ctx* c, const mqpiv_request* r, int n, BatchPlan& P)
     }
     // precision 3 groups by its FP64 length too: a recomputed IA then gets the
     // values of the same call made alone
-    const int Lr = fp64 == 3 ? refine_len(j) : 0;
+    const int Lr = fp64 >= 3 ? refine_len(j) : 0;
     auto key = std::make_tuple(fp64, L, (int)j.alg, Lr);
     auto it = key2g.find(key);
     int g;
@@ -886,7 +897,7 @@ static int plan_batch(mqpiv_ctx* c, const mqpiv_request* r, int n, BatchPlan& P)
   }
   for (auto& g : P.groups) {
     if (lds_2d(g.fp64, g.L, g.alg, g.ns0cap, g.ns1cap, g.ss0cap) > 160 * 1024 ||
-        (g.fp64 == 3 && lds_2d(1, g.Lr, g.alg, g.ns0cap, g.ns1cap, g.ss0cap) > 160 * 1024))
+        (g.fp64 >= 3 && lds_2d(1, g.Lr, g.alg, g.ns0cap, g.ns1cap, g.ss0cap) > 160 * 1024))
       return fail(c, MQPIV_ERR_UNSUPPORTED, "batch group (L=%d) does not fit in LDS", g.L);
     g.table_off = P.table_bytes;
     P.gz_bytes = std::max(P.gz_bytes, gz_scratch_2d(g.fp64, g.L, g.n_ia));

@@ -6,5 +6,5 @@
 namespace mqpiv {
 template size_t lds_bytes_2d<128, float, Lpl2<float, 128>::v>(int, int, int, int);
 template hipError_t launch_2d_l<128, float, Lpl2<float, 128>::v>(int, const Job&, const Job*, const int64_t*, int, int64_t,
-                                                int, int, int, char*, hipStream_t);
+                                                int, int, int, char*, hipStream_t, int32_t*);
 }  // namespace mqpiv

@@ -6,5 +6,5 @@
 namespace mqpiv {
 template size_t lds_bytes_2d<16, float, Lpl2<float, 16>::v>(int, int, int, int);
 template hipError_t launch_2d_l<16, float, Lpl2<float, 16>::v>(int, const Job&, const Job*, const int64_t*, int, int64_t,
-                                                int, int, int, char*, hipStream_t);
+                                                int, int, int, char*, hipStream_t, int32_t*);
 }  // namespace mqpiv

@@ -6,7 +6,8 @@
 namespace mqpiv {
 template size_t lds_bytes_2d<48, double, Lpl2<double, 48>::v>(int, int, int, int);
 template hipError_t launch_2d_l<48, double, Lpl2<double, 48>::v>(int, const Job&, const Job*, const int64_t*, int, int64_t,
-                                                int, int, int, char*, hipStream_t);
-template hipError_t launch_2d_refine<48, double, Lpl2<double, 48>::v>(const Job&, const Job*, const int64_t*, int,
-                                                     int64_t, int, int, int, const int32_t*, hipStream_t);
+                                                int, int, int, char*, hipStream_t, int32_t*);
+template hipError_t launch_2d_refine<48, double, Lpl2<double, 48>::v>(int, const Job&, const Job*, const int64_t*,
+                                                     int, int64_t, int, int, int, const int32_t*, char*,
+                                                     hipStream_t);
 }  // namespace mqpiv

@@ -341,6 +341,20 @@ __host__ __device__ inline size_t region_bytes_2d(int L, int tsize, int ns0, int
 }
 constexpr int kRedBytes = 512;  // reduction slots per IA (4 x 16 doubles)
 
+// FP32 masked NSQECC with a refine list: an IA goes to the FP64 transforms when
+// its values could move an output past the FP64 results' bar.  fr(s) is the
+// bound on the normalised value's error per unit transform error (above); over
+// ~7,000 IAs of four masked particle pairs (interSize 16 to 64, 20 ellipses
+// covering 30%) the peak values moved by at most 4.5e-8 fr, and the SN
+// window's mean by at most 2.7e-7 mean(fr) (relative to the mean):
+// * peak: flagged when the runner-up is within kMaskRefinePeak fr(peak) of it;
+// * SN: flagged when kMaskRefineEps mean(fr) exceeds kMaskRefineSn |mean|;
+// * fit: flagged when a log error of 10 kMaskRefineEps fr(peak) could move the
+//   subpixel offset by 1e-3 px.
+constexpr double kMaskRefinePeak = 1e-6;
+constexpr double kMaskRefineEps = 3e-7;
+constexpr double kMaskRefineSn = 1e-5;
+
 // ---------------------------------------------------------------------------
 // The correlation of one IA.  LPL lanes (a "group", q = lane within it) share a
 // line of L samples, M = L/LPL each.  On entry lane (line = k0, q) holds the
@@ -511,6 +525,9 @@ __device__ __forceinline__ double sat_box(const double* __restrict__ S, int64_t 
 #ifndef MQPIV_ZNCC96_OCC
 #define MQPIV_ZNCC96_OCC 5
 #endif
+#ifndef MQPIV_MASK32_OCC
+#define MQPIV_MASK32_OCC 3
+#endif
 template <int L, int ALG, typename T, int LPL>
 constexpr int min_waves_2d() {
   // global transform buffer: one IA's block per CU, whose waves all fit the
@@ -524,7 +541,12 @@ constexpr int min_waves_2d() {
   constexpr int waves = blocks * (C::NT / 64) / 4;
   // FP32 ZNCC at length 96 keeps its means and window-sum state beside 12
   // values a lane: at the LDS-allowed 6 waves per SIMD (80 VGPRs) it spills
-  constexpr int cap = (L == 96 && ALG == ALG_ZNCC && std::is_same<T, float>::value) ? MQPIV_ZNCC96_OCC : MQPIV_OCC_CAP;
+  // masked FP32 at lengths 32 and 64 carries its two correlations' state: at the
+  // LDS-allowed 4 waves per SIMD (128 VGPRs) it spills
+  constexpr bool f32 = std::is_same<T, float>::value;
+  constexpr int cap = (L == 96 && ALG == ALG_ZNCC && f32)                   ? MQPIV_ZNCC96_OCC
+                      : (L <= 64 && ALG == ALG_MASK_NSQECC && f32 && L >= 32) ? MQPIV_MASK32_OCC
+                                                                              : MQPIV_OCC_CAP;
   return waves < 1 ? 1 : (waves > cap ? cap : waves);
 }
 
@@ -537,18 +559,24 @@ constexpr int min_waves_2d() {
 // list[0], global IA indices from list[kRefineHead]), written by an earlier
 // kernel on the stream; the grid is persistent and every block steps through
 // the list by the grid size until the count is reached.
+// refine_out (FP32 masked NSQECC): a refine list the kernel appends the IAs to
+// whose values may miss the FP64 results' bar, or nullptr.
 template <int L, int ALG, typename T, int LPL, bool SH, bool LIST>
 __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL>())) void piv2d_kernel(Job j0, const Job* __restrict__ jobs,
                                                                       const int64_t* __restrict__ prefix,
                                                                       int n_jobs, int64_t n_ia_total, int ns0_cap,
                                                                       int ns1_cap, char* gz, int gz_slots,
-                                                                      const int32_t* __restrict__ list) {
+                                                                      const int32_t* __restrict__ list,
+                                                                      int32_t* __restrict__ refine_out) {
   const int32_t vb = xcd_remap32((int32_t)blockIdx.x, (int32_t)gridDim.x);
   using C = Cfg2<L, T, LPL>;
   constexpr int H = C::H;
   constexpr bool GZ = gz_len<L, T>();
   static_assert(!GZ || C::IPB == 1, "a global transform buffer holds one IA");
-  static_assert(!(LIST && (GZ || SH)), "list launches use LDS buffers and unshifted regions");
+  static_assert(!(LIST && SH), "list launches take unshifted regions");
+  // FP32 masked NSQECC with a refine list: the per-IA quantities that bound its
+  // values' error (below) are gathered as well
+  constexpr bool kMaskRefine = ALG == ALG_MASK_NSQECC && std::is_same<T, float>::value && !LIST;
   extern __shared__ __align__(16) unsigned char smem[];
   int32_t lpos = vb;  // LIST: this block's current list position
 
@@ -716,6 +744,8 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
   // differences G - mean, F - mean themselves (below), as the reference does
   constexpr bool kLocalW = ALG == ALG_ZNCC && std::is_same<T, double>::value;
   double sa2 = 0.0;
+  // kMaskRefine: sums of (F m)^2, G^2, m^2 and G^4 over the IA / search window
+  float e_fm = 0.f, e_g2 = 0.f, e_m = 0.f, e_g4 = 0.f;
   // G'^2 of the lane's window samples, kept through the correlation for the
   // table (lengths whose buffer is in LDS; the others read the image again)
   constexpr bool kKeepG2 = kLocalW && !GZ;
@@ -790,6 +820,7 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
             a = (double)f * m;  // masked_nsqecc_ops.jl:19-25
             const double wi = m > 0 ? m : 1.0;
             sumF2 += a * a / wi;  // mask_nsqecc.jl:144-149
+            if constexpr (kMaskRefine) e_fm += (float)(a * a);
           } else {
             a = (double)f;
             if constexpr (ALG != ALG_FFTCC) sumF2 += a * a;
@@ -800,6 +831,7 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
             g = std::is_same<TIN, float>::value ? (double)((float)gv[j] - meanG32) : (double)gv[j] - meanG64;
           else
             g = (double)gv[j];
+          if constexpr (kMaskRefine) e_g2 += (float)(g * g);
         }
         if constexpr (kKeepG2) gq[k] = g * g;
         x[k] = cpx<T>{(T)a, (T)g};
@@ -848,6 +880,10 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
           if (g_at(k1)) {
             const double g = (double)gv[j];
             g2 = g * g;
+          }
+          if constexpr (kMaskRefine) {
+            e_m += (float)(m * m);
+            e_g4 += (float)(g2 * g2);
           }
           x[k] = cpx<T>{(T)m, (T)g2};
         }
@@ -1108,12 +1144,32 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
     }
   } else if constexpr (ALG == ALG_MASK_NSQECC) {
     const double sqF = sqrt(sumF2);
+    // kMaskRefine: an FP32 transform moves a correlation value by about eps |a|
+    // |b| (its two windows' norms), so num by eps (2 |F m| |G| + |m| |G^2|) =
+    // eps E12 and den by den eps E2 / (2 C2); the ratio den / (num + den) then
+    // moves by eps times fr below, which is kept in C1 for the flag test
+    double E12 = 0.0, E2 = 0.0;
+    if constexpr (kMaskRefine) {
+      if (refine_out) {
+        double a = e_fm, b = e_g2, c = e_m, d = e_g4;
+        Group<C::TPI>::reduce2(a, b, OpAdd{}, OpAdd{}, red, t);
+        Group<C::TPI>::reduce2(c, d, OpAdd{}, OpAdd{}, red, t);
+        E2 = sqrt(c * d);
+        E12 = 2.0 * sqrt(a * b) + E2;
+      }
+    }
     for (int i = t; i < nE; i += C::TPI) {
       const double c1 = (double)C1[i];
       const double c2 = (double)S[i];
       const double num = (-2.0 * c1 + sumF2) + c2;  // masked_nsqecc_ops.jl:130-132
       const double den = sqrt(fabs(c2)) * sqF;      // masked_nsqecc_ops.jl:135-136
       S[i] = (T)(den / (num + den));
+      if constexpr (kMaskRefine) {
+        if (refine_out) {
+          const double q = num + den;
+          C1[i] = (T)((den * E12 + num * den * E2 / (2.0 * fabs(c2))) / (q * q));
+        }
+      }
       visit(i, w0, w1, (T)(den / (num + den)));
       step();
     }
@@ -1163,10 +1219,32 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
     }
   }
   double sn = 0.0;
-  if (want_sn) {
-    const int u0 = shi0 - lo0 + 1, u1 = shi1 - lo1 + 1;
-    const int nsn = (u0 > 0 && u1 > 0) ? u0 * u1 : 0;
-    sn = nsn > 0 ? mx / ((smv - mx) / (double)(nsn - 1)) : (double)NAN;
+  const int u0 = shi0 - lo0 + 1, u1 = shi1 - lo1 + 1;
+  const int nsn = (u0 > 0 && u1 > 0) ? u0 * u1 : 0;
+  if (want_sn) sn = nsn > 0 ? mx / ((smv - mx) / (double)(nsn - 1)) : (double)NAN;
+  // kMaskRefine: the runner-up of the peak window (the largest value at another
+  // position; +inf when a value there is not finite) and the sum of fr over the
+  // SN window
+  double t2 = -INFINITY, sfr = 0.0;
+  if constexpr (kMaskRefine) {
+    if (refine_out) {
+      int a0 = t % ns0, a1 = t / ns0;
+      for (int e = t; e < nE; e += C::TPI) {
+        if (a0 >= lo0 && a0 <= hi0 && a1 >= lo1 && a1 <= hi1) {
+          const double v = (double)S[e];
+          if (!(__builtin_fabs(v) <= 1.7976931348623157e308)) t2 = INFINITY;
+          else if (e != bst.i && v > t2) t2 = v;
+        }
+        if (a0 >= lo0 && a0 <= shi0 && a1 >= lo1 && a1 <= shi1) sfr += (double)C1[e];
+        a0 += st0;
+        a1 += st1;
+        if (a0 >= ns0) {
+          a0 -= ns0;
+          ++a1;
+        }
+      }
+      Group<C::TPI>::reduce2(t2, sfr, OpMax{}, OpAdd{}, red, t);
+    }
   }
 
   // ---- Gaussian subpixel + write (piv_ops.jl:249-326, 236-243).  The 3x3
@@ -1218,6 +1296,31 @@ __global__ __launch_bounds__((Cfg2<L, T, LPL>::NT), (min_waves_2d<L, ALG, T, LPL
       r0 += (double)sh0;
       r1 += (double)sh1;
     }
+    if constexpr (kMaskRefine) {
+      // (an IA whose IA mask is all 0 has den = 0 and only zeros in both forms)
+      if (refine_out && !skip && sumF2 > 0.0) {
+        bool f = false;
+        if (has_peak) {
+          const double fpk = (double)C1[bst.i];
+          f |= !((double)bst.v - t2 >= kMaskRefinePeak * fpk);
+          if (interior) {
+            const double lmx = lfit[0], up = lfit[1], dn = lfit[2], lf = lfit[3], rt = lfit[4];
+            const double D0 = 2 * up - 4 * lmx + 2 * dn, D1 = 2 * lf - 4 * lmx + 2 * rt;
+            const double e = kMaskRefineEps * 10.0 * fpk;  // log error, ten times the value error bound
+            f |= !(1e3 * e * (2.0 + 8.0 * __builtin_fabs((up - dn) / D0)) <= __builtin_fabs(D0)) ||
+                 !(1e3 * e * (2.0 + 8.0 * __builtin_fabs((lf - rt) / D1)) <= __builtin_fabs(D1));
+          }
+        }
+        if (want_sn && nsn > 1) {
+          const double avg = (smv - mx) / (double)(nsn - 1);
+          f |= !(kMaskRefineEps * (sfr / (double)nsn) <= kMaskRefineSn * __builtin_fabs(avg));
+        }
+        if (f) {
+          const int32_t pos = atomicAdd(refine_out, 1);
+          refine_out[kRefineHead + pos] = ia_g;
+        }
+      }
+    }
     if (J.peaks) {
       J.peaks[2 * ia + 0] = has_peak ? pk0 + 1 : 0;
       J.peaks[2 * ia + 1] = has_peak ? pk1 + 1 : 0;
@@ -1263,7 +1366,7 @@ size_t lds_bytes_2d(int alg, int ns0, int ns1, int ss0) {
 
 template <int L, int ALG, typename T, int LPL, bool SH>
 hipError_t launch_2d_k(const Job& j0, const Job* d_jobs, const int64_t* d_prefix, int n_jobs, int64_t n_ia_total,
-                       int ns0, int ns1, size_t lds, char* gz, hipStream_t s) {
+                       int ns0, int ns1, size_t lds, char* gz, hipStream_t s, int32_t* refine_out) {
   using C = Cfg2<L, T, LPL>;
   const void* fn = (const void*)piv2d_kernel<L, ALG, T, LPL, SH, false>;
   allow_full_lds(fn);
@@ -1274,59 +1377,88 @@ hipError_t launch_2d_k(const Job& j0, const Job* d_jobs, const int64_t* d_prefix
     slots = gz_slots_2d(n_ia_total);
   }
   hipLaunchKernelGGL((piv2d_kernel<L, ALG, T, LPL, SH, false>), dim3((unsigned)nb), dim3(C::NT), lds, s, j0, d_jobs,
-                     d_prefix, n_jobs, n_ia_total, ns0, ns1, gz, slots, (const int32_t*)nullptr);
+                     d_prefix, n_jobs, n_ia_total, ns0, ns1, gz, slots, (const int32_t*)nullptr, refine_out);
   return hipGetLastError();
 }
 
-// ZNCC over the IAs of a refine list (refine_list layout) that an earlier
-// kernel on the stream wrote: a persistent grid of as many blocks as the
-// transform buffers let the CUs hold, each stepping through the list.
-template <int L, typename T, int LPL>
-hipError_t launch_2d_refine(const Job& j0, const Job* d_jobs, const int64_t* d_prefix, int n_jobs,
-                            int64_t n_ia_total, int ns0, int ns1, int ss0, const int32_t* list, hipStream_t s) {
-  static_assert(!gz_len<L, T>(), "LDS transform buffers only");
+// ZNCC or masked NSQECC over the IAs of a refine list (refine_list layout)
+// that an earlier kernel on the stream wrote: a persistent grid of as many
+// blocks as the transform buffers (LDS, or for gz_len lengths the global slots
+// in gz) let the CUs hold, each stepping through the list.
+template <int L, int ALG, typename T, int LPL>
+hipError_t launch_2d_refine_a(const Job& j0, const Job* d_jobs, const int64_t* d_prefix, int n_jobs,
+                              int64_t n_ia_total, int ns0, int ns1, int ss0, const int32_t* list, char* gz,
+                              hipStream_t s) {
   using C = Cfg2<L, T, LPL>;
-  const void* fn = (const void*)piv2d_kernel<L, ALG_ZNCC, T, LPL, false, true>;
+  const void* fn = (const void*)piv2d_kernel<L, ALG, T, LPL, false, true>;
   allow_full_lds(fn);
-  const size_t lds = lds_bytes_2d<L, T, LPL>(ALG_ZNCC, ns0, ns1, ss0);
+  const size_t lds = lds_bytes_2d<L, T, LPL>(ALG, ns0, ns1, ss0);
   if (lds > 160 * 1024 || !list) return hipErrorInvalidValue;
-  const int64_t by_lds = (int64_t)((160 * 1024) / (lds > 0 ? lds : 1));
-  const int64_t by_waves = (int64_t)min_waves_2d<L, ALG_ZNCC, T, LPL>() * 4 * 64 / C::NT;
-  int64_t per_cu = by_lds < by_waves ? by_lds : by_waves;
+  int64_t per_cu;
+  int slots = 0;
+  if constexpr (gz_len<L, T>()) {
+    if (!gz) return hipErrorInvalidValue;  // the caller provides gz_scratch_2d bytes
+    per_cu = kGzBlocksPerCu;
+    slots = gz_slots_2d(n_ia_total);
+  } else {
+    const int64_t by_lds = (int64_t)((160 * 1024) / (lds > 0 ? lds : 1));
+    const int64_t by_waves = (int64_t)min_waves_2d<L, ALG, T, LPL>() * 4 * 64 / C::NT;
+    per_cu = by_lds < by_waves ? by_lds : by_waves;
+  }
   if (per_cu < 1) per_cu = 1;
   int64_t nb = (n_ia_total + C::IPB - 1) / C::IPB;
   if (nb > per_cu * device_cus()) nb = per_cu * device_cus();
+  if (gz_len<L, T>() && nb > slots) nb = slots;  // every resident block finds a free slot
   if (nb < 1) return hipSuccess;
-  hipLaunchKernelGGL((piv2d_kernel<L, ALG_ZNCC, T, LPL, false, true>), dim3((unsigned)nb), dim3(C::NT), lds, s, j0,
-                     d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, (char*)nullptr, 0, list);
+  hipLaunchKernelGGL((piv2d_kernel<L, ALG, T, LPL, false, true>), dim3((unsigned)nb), dim3(C::NT), lds, s, j0,
+                     d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, gz, slots, list, (int32_t*)nullptr);
   return hipGetLastError();
+}
+
+template <int L, typename T, int LPL>
+hipError_t launch_2d_refine(int alg, const Job& j0, const Job* d_jobs, const int64_t* d_prefix, int n_jobs,
+                            int64_t n_ia_total, int ns0, int ns1, int ss0, const int32_t* list, char* gz,
+                            hipStream_t s) {
+  if (alg == ALG_ZNCC)
+    return launch_2d_refine_a<L, ALG_ZNCC, T, LPL>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, list, gz,
+                                                  s);
+  if (alg == ALG_MASK_NSQECC)
+    return launch_2d_refine_a<L, ALG_MASK_NSQECC, T, LPL>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0,
+                                                         list, gz, s);
+  return hipErrorInvalidValue;
 }
 
 // A single-job launch whose job carries search-region shifts runs the SH
 // instantiation (batches never carry shifts).
 template <int L, int ALG, typename T, int LPL>
 hipError_t launch_2d_t(const Job& j0, const Job* d_jobs, const int64_t* d_prefix, int n_jobs,
-                       int64_t n_ia_total, int ns0, int ns1, int ss0, char* gz, hipStream_t s) {
+                       int64_t n_ia_total, int ns0, int ns1, int ss0, char* gz, hipStream_t s, int32_t* refine_out) {
   size_t lds = lds_bytes_2d<L, T, LPL>(ALG, ns0, ns1, ss0);
   if (lds > 160 * 1024) return hipErrorInvalidValue;
+  if (refine_out && (j0.shift || ALG != ALG_MASK_NSQECC || !std::is_same<T, float>::value))
+    return hipErrorInvalidValue;  // FP32 masked NSQECC flags IAs, unshifted
   if (j0.shift && !d_jobs)
-    return launch_2d_k<L, ALG, T, LPL, true>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, lds, gz, s);
-  return launch_2d_k<L, ALG, T, LPL, false>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, lds, gz, s);
+    return launch_2d_k<L, ALG, T, LPL, true>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, lds, gz, s, nullptr);
+  return launch_2d_k<L, ALG, T, LPL, false>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, lds, gz, s,
+                                            refine_out);
 }
 
 template <int L, typename T, int LPL>
 hipError_t launch_2d_l(int alg, const Job& j0, const Job* d_jobs, const int64_t* d_prefix, int n_jobs,
-                       int64_t n_ia_total, int ns0, int ns1, int ss0, char* gz, hipStream_t s) {
+                       int64_t n_ia_total, int ns0, int ns1, int ss0, char* gz, hipStream_t s, int32_t* refine_out) {
+  if (refine_out && alg != ALG_MASK_NSQECC) return hipErrorInvalidValue;
   switch (alg) {
     case ALG_FFTCC:
-      return launch_2d_t<L, ALG_FFTCC, T, LPL>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, gz, s);
+      return launch_2d_t<L, ALG_FFTCC, T, LPL>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, gz, s,
+                                               nullptr);
     case ALG_ZNCC:
-      return launch_2d_t<L, ALG_ZNCC, T, LPL>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, gz, s);
+      return launch_2d_t<L, ALG_ZNCC, T, LPL>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, gz, s, nullptr);
     case ALG_NSQECC:
-      return launch_2d_t<L, ALG_NSQECC, T, LPL>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, gz, s);
+      return launch_2d_t<L, ALG_NSQECC, T, LPL>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1, ss0, gz, s,
+                                                nullptr);
     case ALG_MASK_NSQECC:
       return launch_2d_t<L, ALG_MASK_NSQECC, T, LPL>(j0, d_jobs, d_prefix, n_jobs, n_ia_total, ns0, ns1,
-                                                    ss0, gz, s);
+                                                    ss0, gz, s, refine_out);
   }
   return hipErrorInvalidValue;
 }

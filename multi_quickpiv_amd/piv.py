@@ -63,14 +63,15 @@ def to_c_params(p: PIVParameters, corr_code: Optional[int] = None, mode: str = "
     return cp
 
 
-TRANSFORM_FORMS = {0: "f32", 1: "f64", 2: "split-f16", 3: "split-f16+f64"}
+TRANSFORM_FORMS = {0: "f32", 1: "f64", 2: "split-f16", 3: "split-f16+f64", 4: "f32+f64"}
 
 
 def transform_form(pivparams: PIVParameters, shape: Tuple[int, ...], mode: str = "auto",
                    masked: bool = False) -> Tuple[str, int]:
     """The transform form and length a call with these parameters (and a mask,
     if ``masked``) runs on an image of ``shape``: ("f32" | "f64" | "split-f16" |
-    "split-f16+f64": the split form with its flagged IAs recomputed in FP64,
+    "split-f16+f64": the split form with its flagged IAs recomputed in FP64 |
+    "f32+f64": FP32 transforms with their flagged IAs recomputed in FP64,
     length).  Host-only (mqpiv_transform_form); raises as the call would for an
     unsupported job."""
     p = copy.copy(pivparams)

@@ -40,10 +40,15 @@ def _check(oracle, a, b, alg, isz, sm, st, mask=None, mode="auto", omode=None):
 def test_search_size_128_fp64(oracle, alg):
     a, b = _pair((320, 288), (3.3, -4.6), 41)
     mask = synth.ellipse_mask(a.shape, n=6, cover=0.3, seed=5).numpy() if alg == "mask_nsqecc" else None
-    mode = "auto" if alg in ("mask_nsqecc", "zncc") else "fp64"
+    mode = "auto" if alg == "zncc" else "fp64"  # AUTO runs ZNCC+SN at search size 128 in FP64
     rep = _check(oracle, a, b, alg, 64, 32, 32, mask=mask, mode=mode)
     assert rep.ok(), rep
     assert rep.vf_maxerr <= 1e-6, rep
+    if alg == "mask_nsqecc":
+        # AUTO: FP32 transforms with the IAs they flag recomputed at length 128
+        # in FP64 (global transform buffers, persistent list launch)
+        rep = _check(oracle, a, b, alg, 64, 32, 32, mask=mask, mode="auto")
+        assert rep.ok(), rep
 
 
 def test_anisotropic_window_at_128(oracle):
@@ -53,11 +58,14 @@ def test_anisotropic_window_at_128(oracle):
     assert rep.ok(), rep
 
 
-def test_persistent_grid_many_ias(oracle):
+@pytest.mark.parametrize("mode", ["fp64", "auto"])
+def test_persistent_grid_many_ias(oracle, mode):
     """2,745 IAs (more than one pass of the resident blocks) of masked NSQECC,
-    checked against the oracle's FFT mode (pinned to its direct mode)."""
+    checked against the oracle's FFT mode (pinned to its direct mode); in AUTO
+    the FP64 recomputation of the flagged IAs runs a persistent grid over the
+    refine list with the global transform buffers."""
     a, b = _pair((1024, 768), (1.7, -2.9), 47)
     mask = synth.ellipse_mask(a.shape, n=20, cover=0.3, seed=9).numpy()
-    rep = _check(oracle, a, b, "mask_nsqecc", 64, 32, 16, mask=mask, omode=oracle.FFT)
+    rep = _check(oracle, a, b, "mask_nsqecc", 64, 32, 16, mask=mask, mode=mode, omode=oracle.FFT)
     assert rep.n == 61 * 45
     assert rep.ok(), rep

@@ -98,3 +98,39 @@ def test_batch_and_graph_replays_equal_single_calls():
         for r, one in zip(reqs, singles):
             _same(r.results(), one)
     g.close()
+
+
+def _run_masked(a, b, m, p, mode):
+    r = DeviceRequest(a, b, p, mask=m, mode=mode, want_peaks=True)
+    piv_batch([r])
+    torch.cuda.synchronize()
+    return r.results()
+
+
+@pytest.mark.parametrize("isz", [16, 32, 64])
+def test_masked_each_ia_from_one_form_and_at_the_bar(isz):
+    """Masked NSQECC in AUTO (form 4): FP32 transforms, the IAs they flag
+    recomputed by the FP64 transforms of the same length."""
+    shape = (768, 704)
+    n = synth.n_particles(shape, 0.05)
+    a, b = synth.particle_pair(shape, n, 1.2, synth.uniform((2.7, -4.1)), 950 + isz, device="cuda",
+                               dtype=torch.float32)
+    m = synth.ellipse_mask(shape, n=20, cover=0.3, seed=960 + isz, device="cuda")
+    p = setPIVParameters(corr_alg="nsqecc", interSize=isz, searchMargin=isz // 2, step=isz // 2, computeSN=True)
+    assert transform_form(p, shape, "auto", masked=True)[0] == "f32+f64"
+    auto = _run_masked(a, b, m, p, "auto")
+    f32 = _run_masked(a, b, m, p, "fp32")
+    f64 = _run_masked(a, b, m, p, "fp64")
+    refined = _split(*auto, f32)
+    nia = refined.size
+    print(f"masked isz={isz}: {int(refined.sum())} of {nia} IAs recomputed", flush=True)
+    assert refined.sum() < nia // 4
+    assert not _split(*auto, f64)[refined].any()
+    VF, SN, PK = auto
+    VF6, SN6, PK6 = f64
+    assert np.array_equal(PK, PK6)
+    both_nan = np.isnan(SN) & np.isnan(SN6)
+    rel = np.where(both_nan, 0.0, np.abs(SN / SN6 - 1))
+    assert np.nanmax(rel) <= 1e-4
+    err = np.where(np.isnan(VF) & np.isnan(VF6), 0.0, np.abs(VF - VF6))
+    assert np.max(err) <= 0.01

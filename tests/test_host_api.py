@@ -159,7 +159,8 @@ def test_vf_size_and_param_errors_without_gpu():
 
 
 def test_transform_form_follows_the_precision_mode():
-    """mqpiv_transform_form (host only): AUTO keeps FP64 for masked NSQECC, runs
+    """mqpiv_transform_form (host only): AUTO runs masked NSQECC in FP32 transforms
+    with FP64 recomputation of the IAs they flag, runs
     ZNCC with SN in the split-precision form with FP64 recomputation of the IAs
     it flags where that form takes the job (FP64 beyond), the split-precision
     form elsewhere where it takes the job (interSize <= 32 with search size
@@ -180,7 +181,9 @@ def test_transform_form_follows_the_precision_mode():
     assert form("zncc", 32, 16, sn=False) == ("split-f16", 64)
     assert form("fft", 16, 8) == ("split-f16", 64)
     assert form("nsqecc", 64, 32) == ("f32", 128)  # margin 32: beyond the split form
-    assert form("nsqecc", 32, 16, masked=True) == ("f64", 64)
+    assert form("nsqecc", 32, 16, masked=True) == ("f32+f64", 64)  # FP32, flagged IAs in FP64
+    assert form("nsqecc", 64, 32, masked=True) == ("f32+f64", 128)
+    assert form("nsqecc", 64, 32, masked=True, mode="fp64") == ("f64", 128)
     assert form("nsqecc", 32, 16, mode="fp32") == ("f32", 64)
     assert form("nsqecc", 32, 16, mode="fp64") == ("f64", 64)
     assert form("nsqecc", 64, 16, mode="mfma") == ("split-f16", 65)
