@@ -522,12 +522,6 @@ __device__ __forceinline__ double sat_box(const double* __restrict__ S, int64_t 
 #ifndef MQPIV_OCC_CAP
 #define MQPIV_OCC_CAP 8
 #endif
-#ifndef MQPIV_ZNCC96_OCC
-#define MQPIV_ZNCC96_OCC 5
-#endif
-#ifndef MQPIV_MASK32_OCC
-#define MQPIV_MASK32_OCC 3
-#endif
 template <int L, int ALG, typename T, int LPL>
 constexpr int min_waves_2d() {
   // global transform buffer: one IA's block per CU, whose waves all fit the
@@ -539,15 +533,12 @@ constexpr int min_waves_2d() {
   constexpr size_t per_block = (z + c1 + kRedBytes) * C::IPB;
   constexpr int blocks = (int)((160 * 1024) / per_block);
   constexpr int waves = blocks * (C::NT / 64) / 4;
-  // FP32 ZNCC at length 96 keeps its means and window-sum state beside 12
-  // values a lane: at the LDS-allowed 6 waves per SIMD (80 VGPRs) it spills
-  // masked FP32 at lengths 32 and 64 carries its two correlations' state: at the
-  // LDS-allowed 4 waves per SIMD (128 VGPRs) it spills
-  constexpr bool f32 = std::is_same<T, float>::value;
-  constexpr int cap = (L == 96 && ALG == ALG_ZNCC && f32)                   ? MQPIV_ZNCC96_OCC
-                      : (L <= 64 && ALG == ALG_MASK_NSQECC && f32 && L >= 32) ? MQPIV_MASK32_OCC
-                                                                              : MQPIV_OCC_CAP;
-  return waves < 1 ? 1 : (waves > cap ? cap : waves);
+  // A lower target only pays in whole blocks per CU: FP32 ZNCC at length 96
+  // (12-wave blocks, 6 waves per SIMD by LDS) spills 2 VGPRs at 80, but at 5
+  // waves per SIMD only one block fits a CU and the kernel runs 1.6x longer;
+  // masked FP32 at lengths 32/64 (4-wave blocks) spills a little at 128 VGPRs
+  // and still runs 10% faster than at 3 waves per SIMD.
+  return waves < 1 ? 1 : (waves > MQPIV_OCC_CAP ? MQPIV_OCC_CAP : waves);
 }
 
 // One IA per block-slot.  SH: search regions shifted per IA (Job::shift,
