@@ -54,9 +54,9 @@ def test_random_geometry_matches_oracle(oracle, i):
     VFo, SNo, aux = oracle.piv(a, b, op, mask=mask, mode=oracle.DIRECT, nthreads=8, want_aux=True)
     rep = compare(VF, SN, PK, VFo, SNo, aux)
     print(rep, flush=True)
-    if mode == "fp32" and alg in ("zncc", "mask_nsqecc"):
-        # FP32 transforms: ZNCC's SN and masked NSQECC's cancelling numerator
-        # are outside the FP32 bar (the FP64 modes cover them)
-        assert rep.peak_mismatch_clear <= max(1, rep.n // 100) and rep.vf_maxerr <= 0.01, rep
-    else:
-        assert rep.ok(), rep
+    # FP32 transforms do not keep ZNCC's SN or masked NSQECC's cancelling
+    # numerator at the bar in general (AUTO recomputes such IAs in FP64), but
+    # these seeded fp32 cases (9, 11, 21, 23) do: no flipped peak, SN within
+    # 2.5e-6, VF within 1.5e-6 px.  They are held to the full bar, so that a
+    # drift of the FP32 kernels fails here.
+    assert rep.ok(), rep
