@@ -282,10 +282,36 @@ __global__ __launch_bounds__((Cfg3<L, T, LPL>::TPL)) void slab_fwd_kernel(Job J,
 #pragma unroll
     for (int k = 0; k < M; ++k) x[k] = Z[line * LD + LPL * k + q];
     line_fwd<L, LPL, T>(x, q);
-    // plane layout in scratch: [i2][k1][k0]
-    cpx<T>* W = (cpx<T>*)sc.w + il * sc.w_stride + (int64_t)i2 * L * L + (int64_t)line * L;
+  }
+  // plane layout in scratch: [i2][k1][k0].  A lane holds M consecutive k0 of
+  // its row k1, so storing them directly would scatter every store over 64
+  // rows; the plane goes through LDS densely instead (k0 XOR (k1 & 3), which
+  // spreads the lanes of a write over the banks) and leaves in whole 16-byte
+  // vectors, consecutive lanes on consecutive addresses.
+  static_assert(L % 4 == 0 && (L * L * sizeof(cpx<T>)) % 16 == 0, "16-byte vectors over whole groups of 4");
+  __syncthreads();  // every lane has read Z
+  if (line < L) {
 #pragma unroll
-    for (int k = 0; k < M; ++k) W[cM + k] = x[k];
+    for (int k = 0; k < M; ++k) Z[line * L + ((cM + k) ^ (line & 3))] = x[k];
+  }
+  __syncthreads();
+  {
+    cpx<T>* Wp = (cpx<T>*)sc.w + il * sc.w_stride + (int64_t)i2 * L * L;
+    constexpr int EPV = 16 / (int)sizeof(cpx<T>);  // complex values per vector: 2 (FP32) or 1 (FP64)
+    constexpr int NV = L * L / EPV;
+    for (int v = t; v < NV; v += C::TPL) {
+      const int e = v * EPV;  // first element: row k1 = e / L, column k0 = e % L
+      const int r = e / L, c0 = e - r * L;
+      const int sw = r & 3;
+      if constexpr (EPV == 2) {
+        // elements c0 and c0 + 1 (c0 even) sit in one aligned pair at
+        // c0 ^ (sw & 2), swapped when sw is odd
+        const uint4 raw = *reinterpret_cast<const uint4*>(&Z[r * L + (c0 ^ (sw & 2))]);
+        *reinterpret_cast<uint4*>(&Wp[e]) = (sw & 1) ? uint4{raw.z, raw.w, raw.x, raw.y} : raw;
+      } else {
+        Wp[e] = Z[r * L + (c0 ^ sw)];
+      }
+    }
   }
   if constexpr (kBox) {
     // The slab's window sums of G^2 (ZNCC: of G'^2, the rounded differences, as
@@ -487,6 +513,7 @@ __global__ __launch_bounds__(256) void finalize3d_kernel(Job J, Scratch3 sc, int
   // IA constants from the per-slab partials
   double sumF2 = 0.0, mx1 = -INFINITY, mx2 = -INFINITY;
   const double* part = sc.part + il * (3 * L);
+#pragma unroll 8
   for (int i = 0; i < L; ++i) {
     sumF2 += part[i];
     mx1 = fmax(mx1, part[L + i]);
@@ -521,7 +548,11 @@ __global__ __launch_bounds__(256) void finalize3d_kernel(Job J, Scratch3 sc, int
     const double* B = sc.box + il * sc.box_stride;
     for (int e = t; e < nP; e += NT) {
       double hi = 0.0, lo = 0.0;  // sums over planes < s2 + is2 and < s2
+      // (unrolled: the loads of a group are in flight together, in the order
+      // of the sum)
+#pragma unroll 8
       for (int k = 0; k < is2; ++k) hi += B[(int64_t)k * nP + e];
+#pragma unroll 4
       for (int s2 = 0; s2 < ns2; ++s2) {
         if (s2 > 0) {
           hi += B[(int64_t)(s2 + is2 - 1) * nP + e];
