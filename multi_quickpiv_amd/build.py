@@ -61,6 +61,7 @@ SOURCES = [
     "mqpiv_sat.hip",
     "mqpiv_multipass.hip",
     "mqpiv_skip.hip",
+    "mqpiv_means3d.hip",
     "mqpiv_api.cpp",
     "mqpiv_server.cpp",
     "mqpiv_post.hip",

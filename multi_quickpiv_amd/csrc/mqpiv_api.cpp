@@ -42,6 +42,8 @@ int fft_len_3d(int fp64, int ss);
 size_t scratch_3d_bytes(int fp64, int L, const Job& j);
 hipError_t launch_piv3d(int fp64, int L, const Job& j, void* scratch, const double* sat1_img1, hipStream_t s);
 hipError_t launch_skip_flags(const Job& j, const void* mask_vf, uint8_t* flags, hipStream_t s);
+size_t means3d_bytes(const Job& j);
+hipError_t launch_means3d(const Job& j, void* buf, hipStream_t s);
 hipError_t launch_coarse_to_shift(const double* vfc, const int32_t* nc, const int32_t* isc, const int32_t* stc,
                                   const int32_t* nf, const int32_t* isf, const int32_t* stf, int nd,
                                   int64_t n_fine, int32_t* out, hipStream_t s);
@@ -307,6 +309,14 @@ static int prepare_sats(mqpiv_ctx* c, Job& j, hipStream_t s, char* ws = nullptr,
   char* b = ws ? ws : (char*)c->d_sat;
   double* t = (double*)(b + nsat * one);
   if (j.ndims == 3) {  // ZNCC: sums of image 2 and of image 1 (the means)
+    // unshifted regions: the per-IA sums by separable passes, in the same buffer
+    if (!j.shift && means3d_bytes(j) <= nsat * one + tmp) {
+      hipError_t e = launch_means3d(j, b, s);
+      if (e != hipSuccess) return fail(c, MQPIV_ERR_DEVICE, "3-D mean sums failed: %s", hipGetErrorString(e));
+      j.sat1 = nullptr;
+      j.sat1f = (const double*)b;
+      return MQPIV_OK;
+    }
     hipError_t e = build_sat(j.img2, j.in_dtype, 1, j.dims, 3, (double*)b, t, s);
     if (e == hipSuccess) e = build_sat(j.img1, j.in_dtype, 1, j.dims, 3, (double*)(b + one), t, s);
     if (e != hipSuccess) return fail(c, MQPIV_ERR_DEVICE, "integral array build failed: %s", hipGetErrorString(e));

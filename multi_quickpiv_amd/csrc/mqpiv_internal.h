@@ -28,7 +28,10 @@ struct Job {
   const int32_t* shift;    // optional: 3 int32 per IA, search-region shift (multipass)
   const double* sat1;      // FP64 integral array of img2 (sum of pixels), or nullptr
   const double* sat2;      // FP64 integral array of img2^2, or nullptr
-  const double* sat1f;     // FP64 integral array of img1 (3-D ZNCC means), or nullptr
+  const double* sat1f;     // 3-D ZNCC means: FP64 integral array of img1 (with sat1 the
+                           // one of img2), or, with sat1 NULL, the per-IA sums (img1
+                           // over the IA at [ia], img2 over the search region at
+                           // [n_ia + ia]); else nullptr
   int64_t dims[3];
   int32_t is[3], sm[3], st[3];
   int32_t vf_n[3];         // vector-field size per dim

@@ -124,20 +124,26 @@ __device__ __forceinline__ double sat_box3(const double* __restrict__ S, int64_t
          ((at(a0, a1, b2) + at(a0, b1, a2)) + at(b0, a1, a2)) - at(a0, a1, a2);
 }
 
-// ZNCC means of one IA from the integral arrays of both images (sum over the IA
-// and over the clipped search region), rounded in the input element type like
-// the reference (zncc_ops.jl:10, 22).
+// ZNCC means of one IA (sum over the IA and over the clipped search region),
+// rounded in the input element type like the reference (zncc_ops.jl:10, 22).
+// The sums come from the per-IA sums (J.sat1 NULL; s1 = sums, image 1 at [ia],
+// image 2 at [n_ia + ia]) or, for shifted search regions, from the integral
+// arrays of both images (s1 = image 1's, J.sat1 = image 2's).
 struct Means3 {
   float f32, g32;
   double f64, g64;
 };
-__device__ __forceinline__ Means3 means3(const Job& J, const Geo3& G, const double* sat1_img1) {
+__device__ __forceinline__ Means3 means3(const Job& J, const Geo3& G, const double* s1, int64_t ia) {
   const int64_t D0 = J.dims[0] + 1, D01 = D0 * (J.dims[1] + 1);
   const Geo *g0 = &G.g[0], *g1 = &G.g[1], *g2 = &G.g[2];
-  double sf = sat_box3(sat1_img1, D0, D01, g0->tlf, g0->tlf + J.is[0], g1->tlf, g1->tlf + J.is[1], g2->tlf,
-                       g2->tlf + J.is[2]);
-  double sg = sat_box3(J.sat1, D0, D01, g0->sr, g0->sr + g0->gsz, g1->sr, g1->sr + g1->gsz, g2->sr,
-                       g2->sr + g2->gsz);
+  double sf, sg;
+  if (!J.sat1) {
+    sf = s1[ia];
+    sg = s1[J.n_ia + ia];
+  } else {
+    sf = sat_box3(s1, D0, D01, g0->tlf, g0->tlf + J.is[0], g1->tlf, g1->tlf + J.is[1], g2->tlf, g2->tlf + J.is[2]);
+    sg = sat_box3(J.sat1, D0, D01, g0->sr, g0->sr + g0->gsz, g1->sr, g1->sr + g1->gsz, g2->sr, g2->sr + g2->gsz);
+  }
   const double nF = (double)J.is[0] * J.is[1] * J.is[2];
   const double nG = (double)g0->gsz * g1->gsz * g2->gsz;
   Means3 m;
@@ -175,7 +181,7 @@ __global__ __launch_bounds__((Cfg3<L, T, LPL>::TPL)) void slab_fwd_kernel(Job J,
   const int64_t d0 = J.dims[0], d01 = J.dims[0] * J.dims[1];
 
   Means3 mu{0.f, 0.f, 0.0, 0.0};
-  if constexpr (ALG == ALG_ZNCC) mu = means3(J, G, sat1_img1);
+  if constexpr (ALG == ALG_ZNCC) mu = means3(J, G, sat1_img1, ia);
 
   const int k0 = line;
   const bool f_pl = i2 < J.is[2];
