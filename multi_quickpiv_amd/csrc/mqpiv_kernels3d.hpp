@@ -331,7 +331,10 @@ __global__ __launch_bounds__((Cfg3<L, T, LPL>::TPL)) void slab_fwd_kernel(Job J,
     // clipped windows.  Each prefix pass splits a line over PS adjacent lanes:
     // every lane scans its segment in registers and adds the totals of the
     // line's earlier segments.
-    static_assert((size_t)L * L * sizeof(double) <= (size_t)L * LD * sizeof(cpx<T>), "table fits the buffer");
+    // (columns of the table at stride L + 1: the prefix passes' lanes then
+    // fall on distinct LDS banks)
+    constexpr int PL = L + 1;
+    static_assert((size_t)L * PL * sizeof(double) <= (size_t)L * LD * sizeof(cpx<T>), "table fits the buffer");
     static_assert(C::TPL % L == 0, "whole lanes per line");
     constexpr int PS = C::TPL / L, RS = L / PS;
     static_assert(RS * PS == L && (PS & (PS - 1)) == 0, "equal segments, a power of two of lanes");
@@ -339,7 +342,7 @@ __global__ __launch_bounds__((Cfg3<L, T, LPL>::TPL)) void slab_fwd_kernel(Job J,
     __syncthreads();  // every lane has read the buffer
     if (line < L) {
 #pragma unroll
-      for (int k = 0; k < M; ++k) P[line + L * (LPL * k + q)] = gq[k];
+      for (int k = 0; k < M; ++k) P[line + PL * (LPL * k + q)] = gq[k];
     }
     __syncthreads();
     const int seg = t % PS, ln = t / PS;
@@ -362,11 +365,11 @@ __global__ __launch_bounds__((Cfg3<L, T, LPL>::TPL)) void slab_fwd_kernel(Job J,
       for (int k = 0; k < RS; ++k) P[ln * line_stride + (seg * RS + k) * elem_stride] = v[k] + off;
       __syncthreads();
     };
-    pass(L, 1);  // down each column (dim 1)
-    pass(1, L);  // along each row (dim 2)
+    pass(PL, 1);  // down each column (dim 1)
+    pass(1, PL);  // along each row (dim 2)
     const int ns0 = 2 * J.sm[0] + 1, ns1 = 2 * J.sm[1] + 1;
     const int is0 = J.is[0], is1 = J.is[1];
-    auto at = [&](int a, int bb) { return (a < 0 || bb < 0) ? 0.0 : P[a + L * bb]; };
+    auto at = [&](int a, int bb) { return (a < 0 || bb < 0) ? 0.0 : P[a + PL * bb]; };
     double* B = sc.box + il * sc.box_stride + (int64_t)i2 * ns0 * ns1;
     for (int e = t; e < ns0 * ns1; e += C::TPL) {
       const int s0 = e % ns0, s1 = e / ns0;
