@@ -276,8 +276,13 @@ static int sat_count(int alg, int ndims, int fp64 = 0) {
   return alg == ALG_NSQECC ? 1 : 0;
 }
 
-// Workspace a job's integral arrays take (as request_ws_bytes).
+// 3-D ZNCC with unshifted search regions takes its means from per-IA sums
+// (launch_means3d) instead of the two integral arrays.
+static bool means3d_job(const Job& j) { return j.ndims == 3 && j.alg == ALG_ZNCC && !j.shift; }
+
+// Workspace a job's integral arrays (or 3-D mean sums) take (as request_ws_bytes).
 static size_t job_ws_bytes(const Job& j, int fp64) {
+  if (means3d_job(j)) return means3d_bytes(j);
   const int nsat = sat_count(j.alg, j.ndims, fp64);
   if (!nsat) return 0;
   const size_t one = (sat_bytes(j.dims, j.ndims) + 255) & ~(size_t)255;
@@ -287,6 +292,18 @@ static size_t job_ws_bytes(const Job& j, int fp64) {
 
 size_t request_ws_bytes(const mqpiv_request& r) {
   const int nd = r.params.ndims;
+  if (nd == 3 && r.params.corr_alg == MQPIV_ZNCC) {  // requests carry no shifts
+    Job j{};
+    j.ndims = 3;
+    j.alg = ALG_ZNCC;
+    j.n_ia = 1;
+    for (int d = 0; d < 3; ++d) {
+      j.dims[d] = r.dims[d];
+      j.vf_n[d] = (int32_t)vf_len(r.dims[d], r.params.inter_size[d], r.params.step[d]);
+      j.n_ia *= j.vf_n[d];
+    }
+    return means3d_bytes(j);
+  }
   const int nsat = sat_count(r.params.corr_alg, nd);
   if (!nsat) return 0;
   const size_t one = (sat_bytes(r.dims, nd) + 255) & ~(size_t)255;
@@ -300,6 +317,17 @@ static int prepare_sats(mqpiv_ctx* c, Job& j, hipStream_t s, char* ws = nullptr,
   j.sat1 = j.sat2 = j.sat1f = nullptr;
   const int nsat = sat_count(j.alg, j.ndims, fp64);
   if (!nsat) return MQPIV_OK;
+  if (means3d_job(j)) {  // the per-IA sums of the means, by separable passes
+    if (!ws) {
+      int rc = ensure(c, &c->d_sat, &c->sat_cap, means3d_bytes(j), s);
+      if (rc) return rc;
+    }
+    char* b = ws ? ws : (char*)c->d_sat;
+    hipError_t e = launch_means3d(j, b, s);
+    if (e != hipSuccess) return fail(c, MQPIV_ERR_DEVICE, "3-D mean sums failed: %s", hipGetErrorString(e));
+    j.sat1f = (const double*)b;
+    return MQPIV_OK;
+  }
   const size_t one = (sat_bytes(j.dims, j.ndims) + 255) & ~(size_t)255;
   const size_t tmp = (sat_tmp_bytes(j.dims, j.ndims) + 255) & ~(size_t)255;
   if (!ws) {
@@ -308,15 +336,7 @@ static int prepare_sats(mqpiv_ctx* c, Job& j, hipStream_t s, char* ws = nullptr,
   }
   char* b = ws ? ws : (char*)c->d_sat;
   double* t = (double*)(b + nsat * one);
-  if (j.ndims == 3) {  // ZNCC: sums of image 2 and of image 1 (the means)
-    // unshifted regions: the per-IA sums by separable passes, in the same buffer
-    if (!j.shift && means3d_bytes(j) <= nsat * one + tmp) {
-      hipError_t e = launch_means3d(j, b, s);
-      if (e != hipSuccess) return fail(c, MQPIV_ERR_DEVICE, "3-D mean sums failed: %s", hipGetErrorString(e));
-      j.sat1 = nullptr;
-      j.sat1f = (const double*)b;
-      return MQPIV_OK;
-    }
+  if (j.ndims == 3) {  // ZNCC with shifted regions: sums of image 2 and of image 1 (the means)
     hipError_t e = build_sat(j.img2, j.in_dtype, 1, j.dims, 3, (double*)b, t, s);
     if (e == hipSuccess) e = build_sat(j.img1, j.in_dtype, 1, j.dims, 3, (double*)(b + one), t, s);
     if (e != hipSuccess) return fail(c, MQPIV_ERR_DEVICE, "integral array build failed: %s", hipGetErrorString(e));
