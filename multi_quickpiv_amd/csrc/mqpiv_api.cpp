@@ -978,16 +978,16 @@ static int enqueue_2d(mqpiv_ctx* c, BatchPlan& P, char* sat_base, char* d_table,
     if (e != hipSuccess) return fail(c, MQPIV_ERR_DEVICE, "integral arrays: %s", hipGetErrorString(e));
   }
   for (auto& g : P.groups) {
-    const Job* dj = (const Job*)(d_table + g.table_off);
-    const int64_t* dp = (const int64_t*)(d_table + g.table_off + g.members.size() * sizeof(Job));
+    // a group of one pair passes its job in the kernel arguments (read through
+    // the constant cache) instead of the device table, whose record every
+    // workgroup would load before it can address its windows
+    const bool one = g.members.size() == 1;
+    const Job* dj = one ? nullptr : (const Job*)(d_table + g.table_off);
+    const int64_t* dp = one ? nullptr : (const int64_t*)(d_table + g.table_off + g.members.size() * sizeof(Job));
     hipError_t e;
     {
       KernelTimer kt(c, s);
-      // a group of one pair passes its job in the kernel arguments (read
-      // through the constant cache) instead of the device table, whose record
-      // every workgroup would load before it can address its windows
-      const bool one = g.members.size() == 1;
-      e = launch_piv2d(g.fp64, g.L, g.alg, P.jobs2d[g.members[0]], one ? nullptr : dj, one ? nullptr : dp,
+      e = launch_piv2d(g.fp64, g.L, g.alg, P.jobs2d[g.members[0]], dj, dp,
                        (int)g.members.size(), g.n_ia, g.ns0cap, g.ns1cap, g.ss0cap, gz, s, g.Lr);
     }
     if (e != hipSuccess) return fail(c, MQPIV_ERR_DEVICE, "batch launch failed: %s", hipGetErrorString(e));
@@ -1008,27 +1008,36 @@ int batch_device_ws(mqpiv_ctx* c, const mqpiv_request* reqs, int32_t n, void* st
       rc = ensure(c, &c->d_bsat, &c->bsat_cap, P.sat_total + P.sat_tmp + 256, s);
       if (rc) return rc;
     }
-    rc = ensure(c, &c->d_table, &c->table_cap, std::max(P.table_bytes + 256, kMinTableBytes), s);
-    if (rc) return rc;
     if (P.gz_bytes) {
       rc = ensure_gz(c, P.gz_bytes, s);
       if (rc) return rc;
     }
-    // pinned staging for the job tables; the previous batch's copy must be done
-    if (c->table_done) hipEventSynchronize(c->table_done);
-    if (P.table_bytes > c->h_table_cap) {
-      const size_t want = std::max(P.table_bytes, kMinTableBytes);
-      if (c->h_table) hipHostFree(c->h_table);
-      c->h_table = nullptr;
-      c->h_table_cap = 0;
-      HIP_OR_FAIL(c, hipHostMalloc(&c->h_table, want, hipHostMallocDefault));
-      c->h_table_cap = want;
+    // Groups of one pair take their job in the kernel arguments; only groups of
+    // several read the device job tables, and only then are they copied.
+    bool tables = false;
+    for (const auto& g : P.groups) tables |= g.members.size() > 1;
+    if (tables) {
+      rc = ensure(c, &c->d_table, &c->table_cap, std::max(P.table_bytes + 256, kMinTableBytes), s);
+      if (rc) return rc;
+      // pinned staging for the job tables; the previous batch's copy must be done
+      if (c->table_done) hipEventSynchronize(c->table_done);
+      if (P.table_bytes > c->h_table_cap) {
+        const size_t want = std::max(P.table_bytes, kMinTableBytes);
+        if (c->h_table) hipHostFree(c->h_table);
+        c->h_table = nullptr;
+        c->h_table_cap = 0;
+        HIP_OR_FAIL(c, hipHostMalloc(&c->h_table, want, hipHostMallocDefault));
+        c->h_table_cap = want;
+      }
+      if (!c->table_done) HIP_OR_FAIL(c, hipEventCreateWithFlags(&c->table_done, hipEventDisableTiming));
+      fill_tables(P, (char*)c->d_bsat, (char*)c->h_table, ws);
+      HIP_OR_FAIL(c, hipMemcpyAsync(c->d_table, c->h_table, P.table_bytes, hipMemcpyHostToDevice, s));
+      HIP_OR_FAIL(c, hipEventRecord(c->table_done, s));
+    } else {
+      std::vector<char> host(P.table_bytes + 1);  // the jobs' integral-array pointers only
+      fill_tables(P, (char*)c->d_bsat, host.data(), ws);
     }
-    if (!c->table_done) HIP_OR_FAIL(c, hipEventCreateWithFlags(&c->table_done, hipEventDisableTiming));
-    fill_tables(P, (char*)c->d_bsat, (char*)c->h_table, ws);
-    HIP_OR_FAIL(c, hipMemcpyAsync(c->d_table, c->h_table, P.table_bytes, hipMemcpyHostToDevice, s));
-    HIP_OR_FAIL(c, hipEventRecord(c->table_done, s));
-    rc = enqueue_2d(c, P, (char*)c->d_bsat, (char*)c->d_table, (char*)c->d_gz, s, ws != nullptr);
+    rc = enqueue_2d(c, P, (char*)c->d_bsat, tables ? (char*)c->d_table : nullptr, (char*)c->d_gz, s, ws != nullptr);
     if (rc) return rc;
   }
   for (size_t k = 0; k < P.jobs3d.size(); ++k) {
