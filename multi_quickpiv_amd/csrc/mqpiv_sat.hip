@@ -153,6 +153,8 @@ __global__ __launch_bounds__(256) void sat_chunk_fix_kernel(double* __restrict__
   const int64_t c = gid / n_pos + 1;
   if (c >= nch) return;
   double off = 0.0;
+  // (unrolled: the loads of a group are in flight together, in the order of the sum)
+#pragma unroll 8
   for (int k = 0; k < c; ++k) off += tot[p * nch + k];
   const int64_t base = (p % pos_mod) + (p / pos_mod) * pos_mul;
   const int64_t j0 = c * chunk;
