@@ -593,6 +593,7 @@ __global__ __launch_bounds__(256) void finalize3d_kernel(Job J, Scratch3 sc, int
   const int nwin = (w0 > 0 && w1 > 0 && w2 > 0) ? w0 * w1 * w2 : 0;
   Best<T> bst{T(0), INT32_MAX};
   BoxWalk pw(t, NT, w0, w1);
+#pragma unroll 4
   for (int e = t; e < nwin; e += NT, pw.step()) {
     const int p0 = lo0 + pw.a, p1 = lo1 + pw.b, p2 = lo2 + pw.c;
     const int li = p0 + ns0 * (p1 + ns1 * p2);
@@ -623,6 +624,7 @@ __global__ __launch_bounds__(256) void finalize3d_kernel(Job J, Scratch3 sc, int
     const int nsn = (u0 > 0 && u1 > 0 && u2 > 0) ? u0 * u1 * u2 : 0;
     double mx = -INFINITY, sm = 0.0;
     BoxWalk sw(t, NT, u0, u1);
+#pragma unroll 4
     for (int e = t; e < nsn; e += NT, sw.step()) {
       const int p0 = lo0 + sw.a, p1 = lo1 + sw.b, p2 = lo2 + sw.c;
       const double v = (double)S[p0 + ns0 * (p1 + ns1 * p2)];
