@@ -634,7 +634,7 @@ def main() -> int:
     rate_b = n32 / (sum(kern_ms[k] for k in i32) * 1e-3)
     rate_p = n32 / sum(jobs[k]["n"] / forms_is32[f'{jobs[k]["alg"]}{jobs[k]["isz"]}']["fp32"]["ia_per_s_per_call"]
                        for k in i32)
-    north_star = {"what": "IA/s of the batched split-precision form (the step's interSize-32 runs, ZNCC+SN "
+    split_vs_plain = {"what": "IA/s of the batched split-precision form (the step's interSize-32 runs, ZNCC+SN "
                           "with its FP64 recomputation and NSQECC) over the plain per-frame FP32 path (mode "
                           "fp32, one call per run) on the 2048^2 pair",
                   "batched_split_ia_per_s": round(rate_b, 1), "plain_fp32_ia_per_s": round(rate_p, 1),
@@ -729,7 +729,7 @@ def main() -> int:
                      "ms_per_step": round(wall_calls * 1e3 / args.steps, 4),
                      "what": "the same four runs as four separate PIV calls per step"},
         "transform_forms": forms_is32,
-        "north_star": north_star,
+        "split_vs_plain_fp32": split_vs_plain,
         "outputs_finite": finite,
         "volume_3d": volume,
         "batched_masked": batched,
